@@ -1,0 +1,183 @@
+// Host-side parsing of the four parameter arrays into device records.
+//
+// Integer fields are stored as floats and read with floor(x + small eps), as
+// the reference does (PEGen_RayTracing_CircularHole/PESysMatGen.cu:961-964,
+// :1377-1383); positions get the FOV-to-collimator distance F added to y
+// (PESysMatGen.cu:1059, :1160-1161).  Shared by the C ABI and the host unit
+// tests of the lane arithmetic.
+#pragma once
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "smg_geometry.hpp"
+
+namespace smg {
+
+// Host copy of the parsed geometry.
+struct HostGeom {
+    GeomDesc g{};
+    std::vector<Bar<float>> bars_f;
+    std::vector<Bar<double>> bars_d;
+    std::vector<Hole<float>> holes_f;
+    std::vector<Hole<double>> holes_d;
+    std::vector<Layer<float>> layers_f;
+    std::vector<Layer<double>> layers_d;
+    int n_rot = 1;
+    int n_holes_l0 = 0;
+    std::vector<float> coll, det, img, phys;  // raw arrays (first 80000/80000/100/100)
+};
+
+// Error codes match smg_status.
+constexpr int kBadParam = 1;
+constexpr int kCapacity = 2;
+
+inline int perr(std::string& why, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    why = buf;
+    return code;
+}
+
+// Parse the four parameter arrays the way the reference reads them.
+inline int parse_geometry(const float* coll, const float* det, const float* img, const float* phys,
+                          HostGeom& hg, std::string& why) {
+    GeomDesc& g = hg.g;
+    std::memset(&g, 0, sizeof g);
+    hg.coll.assign(coll, coll + 80000);
+    hg.det.assign(det, det + 80000);
+    hg.img.assign(img, img + 100);
+    hg.phys.assign(phys, phys + 100);
+    // integer fields stored as floats (PESysMatGen.cu:961-964, :1377-1383)
+    g.n_layers = (int)std::floor((double)coll[0] + 1e-6);
+    g.n_det = (int)std::floor((double)det[0] + 1e-4);
+    g.nx = (int)std::floor((double)img[0] + 1e-3);
+    g.ny = (int)std::floor((double)img[1] + 1e-3);
+    g.nz = (int)std::floor((double)img[2] + 1e-3);
+    hg.n_rot = (int)std::floor((double)img[6] + 1e-3);
+    if (hg.n_rot < 1) hg.n_rot = 1;
+    if (g.n_layers < 0 || g.n_layers > kMaxLayers)
+        return perr(why, kCapacity, "collimator layers %d outside [0, %d]", g.n_layers, kMaxLayers);
+    if (g.n_det < 1 || g.n_det > kMaxBars)
+        return perr(why, kCapacity, "detector bars %d outside [1, %d]", g.n_det, kMaxBars);
+    if (g.nx < 1 || g.ny < 1 || g.nz < 1)
+        return perr(why, kBadParam, "image dimensions %d x %d x %d", g.nx, g.ny, g.nz);
+    g.n_vox = (int64_t)g.nx * g.ny * g.nz;
+    g.wx = img[3];
+    g.wy = img[4];
+    g.wz = img[5];
+    g.dphi = img[7];
+    g.sx = img[8];
+    g.sy = img[9];
+    g.sz = img[10];
+    g.fov2coll = img[11];
+    const double F = g.fov2coll;
+
+    hg.layers_d.resize(g.n_layers);
+    hg.layers_f.resize(g.n_layers);
+    int first = 0;
+    for (int m = 0; m < g.n_layers; ++m) {
+        const float* Lp = coll + (m + 1) * 10;
+        int nh = (int)std::floor((double)Lp[0]);  // PESysMatGen.cu:982
+        if (nh < 0) return perr(why, kBadParam, "layer %d: negative hole count", m);
+        g.ranges[m].first = first;
+        g.ranges[m].count = nh;
+        first += nh;
+        Layer<double>& L = hg.layers_d[m];
+        L.lo[0] = -(double)Lp[1] / 2.0;
+        L.hi[0] = (double)Lp[1] / 2.0;
+        L.lo[1] = -(double)Lp[2] / 2.0 + F + (double)Lp[4];
+        L.hi[1] = (double)Lp[2] / 2.0 + F + (double)Lp[4];
+        L.lo[2] = -(double)Lp[3] / 2.0;
+        L.hi[2] = (double)Lp[3] / 2.0;
+        L.thick = Lp[2];
+        L.mu = Lp[5];
+        Layer<float>& Lf = hg.layers_f[m];
+        for (int k = 0; k < 3; ++k) {
+            Lf.lo[k] = (float)L.lo[k];
+            Lf.hi[k] = (float)L.hi[k];
+        }
+        Lf.thick = (float)L.thick;
+        Lf.mu = (float)L.mu;
+    }
+    g.n_holes = first;
+    if (g.n_holes > kMaxHoles)
+        return perr(why, kCapacity, "%d holes exceed the parameter array (%d)", g.n_holes, kMaxHoles);
+    hg.n_holes_l0 = (int)std::floor((double)coll[10] + 0.01);  // scatter.cu:2233
+    if (g.n_layers == 0) hg.n_holes_l0 = 0;
+    if (hg.n_holes_l0 > g.n_holes) hg.n_holes_l0 = g.n_holes;
+
+    hg.holes_d.resize(g.n_holes);
+    hg.holes_f.resize(g.n_holes);
+    for (int h = 0; h < g.n_holes; ++h) {
+        const float* hp = coll + 100 + 9 * h;
+        Hole<double>& o = hg.holes_d[h];
+        o.xc = hp[0];
+        o.y1 = (double)hp[1] + F;
+        o.y2 = (double)hp[2] + F;
+        o.zc = hp[3];
+        o.r = hp[4];
+        o.mu = hp[5];
+        o.ymid = (o.y1 + o.y2) / 2.0;
+        o.crit = std::sqrt(o.r * o.r * 4 + (o.y1 - o.y2) * (o.y1 - o.y2)) / 2.0;
+        Hole<float>& of = hg.holes_f[h];
+        of.xc = (float)o.xc;
+        of.y1 = (float)o.y1;
+        of.y2 = (float)o.y2;
+        of.zc = (float)o.zc;
+        of.r = (float)o.r;
+        of.mu = (float)o.mu;
+        of.ymid = (float)o.ymid;
+        of.crit = (float)o.crit;
+    }
+
+    hg.bars_d.resize(g.n_det);
+    hg.bars_f.resize(g.n_det);
+    for (int k = 0; k < g.n_det; ++k) {
+        const float* b = det + 12 * k;
+        Bar<double>& B = hg.bars_d[k];
+        std::memset(&B, 0, sizeof B);
+        B.cx = b[1];
+        B.cy = (double)b[2] + F;
+        B.cz = b[3];
+        B.w = b[4];
+        B.t = b[5];
+        B.h = b[6];
+        B.mut = b[7];
+        B.mupe = b[8];
+        B.muc = b[9];
+        B.res = b[10];
+        B.beta = b[11];
+        B.cb = std::cos(B.beta);
+        B.sb = std::sin(B.beta);
+        B.rdiag = std::sqrt(B.w * B.w + B.h * B.h + B.t * B.t);
+        if (!(B.w > 0 && B.t > 0 && B.h > 0))
+            return perr(why, kBadParam, "bar %d has a non-positive size", k);
+        Bar<float>& Bf = hg.bars_f[k];
+        Bf.cx = (float)B.cx;
+        Bf.cy = (float)B.cy;
+        Bf.cz = (float)B.cz;
+        Bf.w = (float)B.w;
+        Bf.t = (float)B.t;
+        Bf.h = (float)B.h;
+        Bf.mut = (float)B.mut;
+        Bf.mupe = (float)B.mupe;
+        Bf.muc = (float)B.muc;
+        Bf.res = (float)B.res;
+        Bf.cb = (float)B.cb;
+        Bf.sb = (float)B.sb;
+        Bf.rdiag = (float)B.rdiag;
+        Bf.beta = (float)B.beta;
+        Bf.pad0 = Bf.pad1 = 0.0f;
+    }
+    return 0;
+}
+
+}  // namespace smg

@@ -1,0 +1,361 @@
+// Per-lane body of the photoelectric (PE) matrix kernel.
+//
+// One lane owns one matrix element A[d][v] (detector bar d, voxel v) and
+// evaluates the reference element formula (PEGen_RayTracing_CircularHole/
+// PESysMatGen.cu:951-1364) for one detector sub-cell at a time:
+//
+//   A[d][v] = sum over sub-cells of  Omega * exp(-att) * (1 - exp(-mu_t l_cell)) mu_pe / mu_t
+//
+// att = plate gates + hole chords + plate material + other bars + own bar.
+// Positions and ray directions are formed in FP64; the bulk of the chord
+// arithmetic runs in T (float on the fast path, double on the exact path).
+// Hole chords and the plate-material term always run in FP64.
+#pragma once
+
+#include "smg_geometry.hpp"
+#include "smg_rays.hpp"
+
+#if defined(__HIPCC__)
+#define SMG_HD __host__ __device__
+#else
+#define SMG_HD
+#endif
+
+namespace smg {
+
+struct PeGeom {
+    GeomDesc g;
+    const Bar<float>* bars_f;
+    const Bar<double>* bars_d;
+    const Hole<float>* holes_f;
+    const Hole<double>* holes_d;
+    const Layer<float>* layers_f;
+    const Layer<double>* layers_d;
+};
+
+// Voxel centre, FOV shift, rotation about z by rot * dphi
+// (PESysMatGen.cu:1028-1053).
+SMG_FN void voxel_position(const GeomDesc& g, int ix, int iy, int iz, double cphi, double sphi,
+                           double p[3]) {
+    double x = (ix - g.nx / 2.0 + 0.5) * g.wx + g.sx;
+    double y = (iy - g.ny / 2.0 + 0.5) * g.wy + g.sy;
+    double z = (iz - g.nz / 2.0 + 0.5) * g.wz + g.sz;
+    p[0] = x * cphi - y * sphi;
+    p[1] = x * sphi + y * cphi;
+    p[2] = z;
+}
+
+// World -> bar frame: rotation by -beta about y around the bar centre
+// (PESysMatGen.cu:1071-1073).
+template <class B>
+SMG_FN void to_frame(const B& b, const double u[3], double out[3]) {
+    double cb = b.cb, sb = -b.sb;  // cos(-beta), sin(-beta)
+    double x = u[0] - b.cx, z = u[2] - b.cz;
+    out[0] = x * cb - z * sb;
+    out[1] = u[1] - b.cy;
+    out[2] = x * sb + z * cb;
+}
+
+// Bar frame -> world (PESysMatGen.cu:1211-1217).
+template <class B>
+SMG_FN void from_frame(const B& b, const double s[3], double out[3]) {
+    out[0] = b.cx + s[0] * b.cb - s[2] * b.sb;
+    out[1] = b.cy + s[1];
+    out[2] = b.cz + s[0] * b.sb + s[2] * b.cb;
+}
+
+// Sub-cell (a, b, c) of an (nsx, nsy, nsz) split: centre and box in the bar
+// frame (PESysMatGen.cu:1207-1209, :1329-1336).
+SMG_FN void sub_cell(const Bar<double>& b, int a, int bb, int c, int nsx, int nsy, int nsz,
+                     double ctr[3], double lo[3], double hi[3]) {
+    ctr[0] = -b.w / 2.0 + (a + 0.5) / nsx * b.w;
+    ctr[1] = -b.t / 2.0 + (bb + 0.5) / nsy * b.t;
+    ctr[2] = -b.h / 2.0 + (c + 0.5) / nsz * b.h;
+    lo[0] = ((double)a / nsx - 0.5) * b.w;
+    hi[0] = (((double)a + 1.0) / nsx - 0.5) * b.w;
+    lo[1] = ((double)bb / nsy - 0.5) * b.t;
+    hi[1] = (((double)bb + 1.0) / nsy - 0.5) * b.t;
+    lo[2] = ((double)c / nsz - 0.5) * b.h;
+    hi[2] = (((double)c + 1.0) / nsz - 0.5) * b.h;
+}
+
+// ---- culling (PESysMatGen.cu:1097-1186) ------------------------------------------
+//
+// The reference keeps bar k (k != d) for element (d, v) when
+//   |u_k - u_d| <= rdiag_k / (2 D_k) + rdiag_d / (2 D_d),
+// with u the unit vector from the voxel to a bar centre and D its distance,
+// and hole h when |u_h - u_d| <= crit_h / D_h + rdiag_d / (2 D_d).  Neither
+// test is conservative for the chord it gates, so the GPU must take exactly
+// the same decisions: FP32 with a tolerance, FP64 when inside it.
+
+struct SelfCull {
+    double ud[3];
+    double crit;
+};
+
+SMG_FN void self_cull(const Bar<double>& bd, const double p[3], SelfCull& s) {
+    double dx = bd.cx - p[0], dy = bd.cy - p[1], dz = bd.cz - p[2];
+    double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    s.crit = bd.rdiag / dist / 2.0;
+    s.ud[0] = dx / dist;
+    s.ud[1] = dy / dist;
+    s.ud[2] = dz / dist;
+}
+
+// Angular distance and threshold of one (voxel, sphere) pair in T.
+template <class T>
+SMG_FN bool cull_test(T px, T py, T pz, T cx, T cy, T cz, T crit_num, T half, const T ud[3],
+                      T crit_self, bool& near) {
+    T dx = px - cx, dy = py - cy, dz = pz - cz;
+    T dk = sqrt(dx * dx + dy * dy + dz * dz);
+    T crit = half ? crit_num / dk / T(2) : crit_num / dk;
+    T ux = (cx - px) / dk, uy = (cy - py) / dk, uz = (cz - pz) / dk;
+    T s = sqrt((uz - ud[2]) * (uz - ud[2]) + (uy - ud[1]) * (uy - ud[1]) + (ux - ud[0]) * (ux - ud[0]));
+    T lim = crit + crit_self;
+    return g_le(s, lim, T(64) * kUnit<T> * (s + lim + T(1)), near);
+}
+
+// Exact decision for bar k: FP32 first (fast path), FP64 inside the tolerance.
+template <bool kExact>
+SMG_FN bool cull_bar(const PeGeom& G, int k, const double p[3], const float pf[3],
+                     const SelfCull& sc, const float udf[3], float critf) {
+    if constexpr (!kExact) {
+        const Bar<float>& b = G.bars_f[k];
+        bool near = false;
+        bool keep = cull_test<float>(pf[0], pf[1], pf[2], b.cx, b.cy, b.cz, b.rdiag, 1.0f, udf,
+                                     critf, near);
+        if (!near) return keep;
+    }
+    const Bar<double>& b = G.bars_d[k];
+    bool near = false;
+    return cull_test<double>(p[0], p[1], p[2], b.cx, b.cy, b.cz, b.rdiag, 1.0, sc.ud, sc.crit, near);
+}
+
+template <bool kExact>
+SMG_FN bool cull_hole(const PeGeom& G, int h, const double p[3], const float pf[3],
+                      const SelfCull& sc, const float udf[3], float critf) {
+    if constexpr (!kExact) {
+        const Hole<float>& o = G.holes_f[h];
+        bool near = false;
+        bool keep = cull_test<float>(pf[0], pf[1], pf[2], o.xc, o.ymid, o.zc, o.crit, 0.0f, udf,
+                                     critf, near);
+        if (!near) return keep;
+    }
+    const Hole<double>& o = G.holes_d[h];
+    bool near = false;
+    return cull_test<double>(p[0], p[1], p[2], o.xc, o.ymid, o.zc, o.crit, 0.0, sc.ud, sc.crit, near);
+}
+
+// ---- exp(-x) for x >= 0 given in FP64, result in T ---------------------------------
+// FP32: e^-x = 2^-(x log2 e) split into integer and fraction in FP64, so the
+// rounding of a large x (a ray through 5 mm of tungsten has x ~ 20) does not
+// turn into a relative error of the result.
+template <class T>
+SMG_FN T exp_neg(double x) {
+    if constexpr (std::is_same<T, double>::value) {
+        return exp(-x);
+    } else {
+        double y = x * 1.4426950408889634;  // log2(e)
+        if (y > 200.0) return 0.0f;
+        double n = floor(y);
+        float f = (float)(y - n);
+        return ldexpf(exp2f(-f), -(int)n);
+    }
+}
+
+// ---- one sub-cell ray ------------------------------------------------------------------
+
+// Per sub-cell quantities shared by every lane (detector-side geometry).
+struct CellCtx {
+    double q[3];      // sub-cell centre, world
+    double qs[3];     // sub-cell centre, frame of bar d
+    double lo[3], hi[3];  // sub-cell box, frame of bar d
+    double area[3];   // projected cell-face areas over 4 pi: x, y, z faces
+};
+
+SMG_FN void make_cell_ctx(const Bar<double>& bd, int a, int b, int c, int nsx, int nsy, int nsz,
+                          CellCtx& cc) {
+    sub_cell(bd, a, b, c, nsx, nsy, nsz, cc.qs, cc.lo, cc.hi);
+    from_frame(bd, cc.qs, cc.q);
+    cc.area[0] = bd.t * bd.h / nsz / nsy;
+    cc.area[1] = bd.w * bd.h / nsx / nsz;
+    cc.area[2] = bd.w * bd.t / nsx / nsy;
+}
+
+// Candidate lists of one lane.  Provides the bars and holes the ray must be
+// tested against; on(j) tells whether candidate j is in this element's
+// reference culling set.
+//   int nbars() const; int bar(int j) const; bool bar_on(int j) const;
+//   int nholes() const; int hole(int j) const; bool hole_on(int j) const;
+
+template <class T, class Cands>
+SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[3],
+                     const CellCtx& cc, const Cands& C, bool& near, double* dl_layer = nullptr,
+                     double* dl_bar = nullptr, double* aux = nullptr) {
+    const GeomDesc& g = G.g;
+    const Bar<double>& bd = G.bars_d[d];
+    double D[3] = {cc.q[0] - p[0], cc.q[1] - p[1], cc.q[2] - p[2]};
+    double dsq = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+    double dd = sqrt(dsq);
+    double cosang = D[1] / dd;
+
+    // solid angle: largest of the three projected cell faces (PESysMatGen.cu:1223-1231)
+    T om;
+    {
+        T cb = T(bd.cb), sb = T(bd.sb);
+        T Dx = T(D[0]), Dy = T(D[1]), Dz = T(D[2]);
+        T tdsq = T(dsq), tdd = T(dd);
+        const T four_pi = T(4.0 * 3.14159265358979323846);
+        T omx = T(cc.area[0]) / (four_pi * tdsq) * tabs(Dx * cb - Dz * sb) / tdd;
+        T omz = T(cc.area[2]) / (four_pi * tdsq) * tabs(Dx * sb + Dz * cb) / tdd;
+        T omy = T(cc.area[1]) / (four_pi * tdsq) * tabs(Dy) / tdd;
+        om = tmax2(tmax2(omx, omy), omz);
+    }
+
+    double att = 0.0;
+    // world-frame direction data, shared by the plates and unrotated bars
+    SegDir<T> sw;
+    T tol_d = T(4) * kUnit<T> * (tabs(T(D[0])) + tabs(T(D[1])) + tabs(T(D[2])));
+    segdir_init<T>(sw, T(D[0]), T(D[1]), T(D[2]), tol_d, near);
+    const T pw[3] = {T(p[0]), T(p[1]), T(p[2])};
+
+    // plate gate: a ray that does not cross a plate is blocked (PESysMatGen.cu:1240-1254)
+    for (int m = 0; m < g.n_layers; ++m) {
+        T lo[3], hi[3];
+        if constexpr (std::is_same<T, float>::value) {
+            const Layer<float>& L = G.layers_f[m];
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = L.lo[k];
+                hi[k] = L.hi[k];
+            }
+        } else {
+            const Layer<double>& L = G.layers_d[m];
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = L.lo[k];
+                hi[k] = L.hi[k];
+            }
+        }
+        T l = seg_box<T>(sw, pw, lo, hi, near);
+        if (g_lt(l, T(0.1f), T(1e-4), near)) att += 100000.0;
+    }
+
+    // holes and plate material, FP64 (PESysMatGen.cu:1257-1286)
+    for (int m = 0; m < g.n_layers; ++m) {
+        const Layer<double>& L = G.layers_d[m];
+        int first = g.ranges[m].first, last = first + g.ranges[m].count;
+        double in_coll = L.thick / cosang;
+        double sum = 0.0;
+        for (int j = 0; j < C.nholes(); ++j) {
+            int h = C.hole(j);
+            if (h < first || h >= last || !C.hole_on(j)) continue;
+            const Hole<double>& o = G.holes_d[h];
+            double l = cyl_chord(p, D, dd, o.xc, o.y1, o.y2, o.zc, o.r);
+            sum += l;
+            att += o.mu * l;
+        }
+        double rest = in_coll - sum;
+        bool counted = rest >= (double)0.00001f;
+        if (counted) att += L.mu * rest;
+        if (dl_layer) dl_layer[m] = counted ? rest : 0.0;
+    }
+
+    // other bars on the way (PESysMatGen.cu:1290-1325)
+    for (int j = 0; j < C.nbars(); ++j) {
+        if (!C.bar_on(j)) {
+            if (dl_bar) dl_bar[j] = 0.0;
+            continue;
+        }
+        int k = C.bar(j);
+        const Bar<double>& bk = G.bars_d[k];
+        T l;
+        if constexpr (std::is_same<T, float>::value) {
+            const Bar<float>& bf = G.bars_f[k];
+            // local origin: the point of the ray closest to the bar centre
+            double rel[3] = {bk.cx - p[0], bk.cy - p[1], bk.cz - p[2]};
+            double t0 = (rel[0] * D[0] + rel[1] * D[1] + rel[2] * D[2]) / dd;
+            double s0 = t0 / dd;
+            double o[3] = {D[0] * s0 - rel[0], D[1] * s0 - rel[1], D[2] * s0 - rel[2]};
+            float lo[3] = {-0.5f * bf.w, -0.5f * bf.t, -0.5f * bf.h};
+            float hi[3] = {0.5f * bf.w, 0.5f * bf.t, 0.5f * bf.h};
+            float lim_hi = float(dd - t0), lim_lo = float(double(kSlabEps<float>) - t0);
+            if (bk.sb == 0.0 && bk.cb == 1.0) {
+                float of[3] = {float(o[0]), float(o[1]), float(o[2])};
+                l = seg_box_local(sw, of, lim_lo, lim_hi, lo, hi, near);
+            } else {
+                double cb = bk.cb, sb = -bk.sb;
+                float of[3] = {float(o[0] * cb - o[2] * sb), float(o[1]), float(o[0] * sb + o[2] * cb)};
+                SegDir<float> sr;
+                segdir_init<float>(sr, float(D[0] * cb - D[2] * sb), float(D[1]),
+                                   float(D[0] * sb + D[2] * cb), tol_d, near);
+                l = seg_box_local(sr, of, lim_lo, lim_hi, lo, hi, near);
+            }
+        } else {
+            double a[3], b[3];
+            to_frame(bk, p, a);
+            to_frame(bk, cc.q, b);
+            double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
+            double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
+            SegDir<double> sr;
+            segdir_init<double>(sr, b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.0, near);
+            l = seg_box<double>(sr, a, lo, hi, near);
+        }
+        if (dl_bar) dl_bar[j] = double(l);
+        att += double(l) * bk.mut;
+    }
+
+    // own bar from its surface to the sub-cell (PESysMatGen.cu:1329-1346), and the
+    // absorption chord along the ray extended 1000 mm (PESysMatGen.cu:1350-1357)
+    double Ds[3] = {cc.qs[0] - ps[0], cc.qs[1] - ps[1], cc.qs[2] - ps[2]};
+    double ext[3], De[3];
+    for (int k = 0; k < 3; ++k) {
+        ext[k] = cc.qs[k] + 1000.0 * (cc.qs[k] - ps[k]) / dd;
+        De[k] = ext[k] - ps[k];
+    }
+    T l1, l2, la;
+    if constexpr (std::is_same<T, float>::value) {
+        const Bar<float>& bf = G.bars_f[d];
+        SegDir<float> ss;
+        float tds = 4.0f * kUnit<float> * (tabs(float(Ds[0])) + tabs(float(Ds[1])) + tabs(float(Ds[2])));
+        segdir_init<float>(ss, float(Ds[0]), float(Ds[1]), float(Ds[2]), tds, near);
+        float of[3] = {float(cc.qs[0]), float(cc.qs[1]), float(cc.qs[2])};
+        float blo[3] = {-bf.w / 2.0f, -bf.t / 2.0f, -bf.h / 2.0f};
+        float bhi[3] = {bf.w / 2.0f, bf.t / 2.0f, bf.h / 2.0f};
+        float clo[3] = {float(cc.lo[0]), float(cc.lo[1]), float(cc.lo[2])};
+        float chi[3] = {float(cc.hi[0]), float(cc.hi[1]), float(cc.hi[2])};
+        float lim_lo = float(double(kSlabEps<float>) - dd);
+        float fdd = float(dd);
+        l1 = seg_box_inside_local(ss, of, fdd, lim_lo, 0.0f, blo, bhi, near);
+        l2 = seg_box_inside_local(ss, of, fdd, lim_lo, 0.0f, clo, chi, near);
+        SegDir<float> se = ss;
+        double le = sqrt(De[0] * De[0] + De[1] * De[1] + De[2] * De[2]);
+        se.len = float(le);
+        for (int k = 0; k < 3; ++k) {
+            se.D[k] = float(De[k]);
+            se.inv[k] = float(le / De[k]);
+        }
+        la = seg_box_local(se, of, lim_lo, float(le - dd), clo, chi, near);
+    } else {
+        double blo[3] = {-bd.w / 2.0, -bd.t / 2.0, -bd.h / 2.0};
+        double bhi[3] = {bd.w / 2.0, bd.t / 2.0, bd.h / 2.0};
+        SegDir<double> ss, se;
+        segdir_init<double>(ss, Ds[0], Ds[1], Ds[2], 0.0, near);
+        l1 = seg_box_inside<double>(ss, ps, blo, bhi, near);
+        l2 = seg_box_inside<double>(ss, ps, cc.lo, cc.hi, near);
+        segdir_init<double>(se, De[0], De[1], De[2], 0.0, near);
+        la = seg_box<double>(se, ps, cc.lo, cc.hi, near);
+    }
+    att += (double(l1) - double(l2)) * bd.mut;
+    T eff = exp_neg<T>(att);
+    T mut = T(bd.mut), mupe = T(bd.mupe);
+    T ea = exp_neg<T>(double(la) * bd.mut);
+    T absf = (T(1) - ea) * mupe / mut;
+    if (aux) {  // quantities the adjoint needs
+        aux[0] = double(eff) * double(om);  // Omega e^-att
+        aux[1] = double(l1) - double(l2);   // own-bar path
+        aux[2] = double(la);                // absorption chord
+        aux[3] = double(ea);
+    }
+    return eff * om * absf;
+}
+
+}  // namespace smg

@@ -1,0 +1,140 @@
+/*
+ * smg: C ABI of the MI355X (gfx950) SPECT/PET system-matrix generator.
+ *
+ * Plain C types only: float/double/uint32 pointers and sizes.  One context is
+ * bound to one GPU and one HIP stream; calls on a context are synchronous and
+ * must come from one host thread at a time.  A pointer argument flagged
+ * "_is_device" must be device memory of the context's GPU (for example a
+ * hipMalloc'ed buffer or a torch tensor's data_ptr()); otherwise it is host
+ * memory and the call copies through the context's own device buffers.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   PESysMatGen()  PEGen_RayTracing_CircularHole/PESysMatGen.h:4-5,
+ *                  defined PEGen_RayTracing_CircularHole/PESysMatGen.cu:1368-1437
+ *   scatter()      ScatterGen_RayTracing_CircularHole/scatter.h:4-5 (combined),
+ *                  ScatterGen_Collimator/scatter.h:4-5, ScatterGen_Crystal/scatter.h:4
+ * The C++ shims with the reference's exact signatures are declared in
+ * smg_reference.h and built on top of these entry points.
+ *
+ * Matrix layout (all outputs): FP32 [detector][voxel], voxel index
+ * ix + iy*NX + iz*NX*NY, one rotation per call
+ * (PEGen_RayTracing_CircularHole/PESysMatGen.cu:1022-1034).
+ */
+#ifndef SMG_H
+#define SMG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMG_ABI_VERSION 1
+
+typedef enum smg_status {
+    SMG_OK = 0,
+    SMG_ERR_BAD_PARAM = 1,  /* invalid argument or parameter array content */
+    SMG_ERR_CAPACITY = 2,   /* geometry beyond a fixed capacity */
+    SMG_ERR_IO = 3,         /* file could not be read or written */
+    SMG_ERR_HIP = 4,        /* HIP runtime error */
+    SMG_ERR_NO_DEVICE = 5,  /* no GPU, or device index out of range */
+    SMG_ERR_STATE = 6       /* call order: e.g. no geometry set yet */
+} smg_status;
+
+typedef enum smg_precision {
+    SMG_PREC_FP32 = 0,  /* FP32 arithmetic, compensated sums, FP64 for
+                           near-threshold and near-grazing rays */
+    SMG_PREC_FP64 = 1   /* all-FP64 arithmetic */
+} smg_precision;
+
+typedef enum smg_coll_variant {
+    SMG_COLL_COMBINED = 0,   /* r = 0.1 mm virtual disc, cos factor
+                                (ScatterGen_RayTracing_CircularHole/scatter.cu:1702-1716) */
+    SMG_COLL_STANDALONE = 1  /* r = 0.2 mm, (r^2 + T r)/(4 L^2)
+                                (ScatterGen_Collimator/scatter.cu:1207-1215) */
+} smg_coll_variant;
+
+typedef struct smg_stats {
+    double kernel_ms;          /* GPU time of the main kernel(s), HIP events */
+    double total_ms;           /* host wall time of the call */
+    uint64_t elements;         /* matrix elements produced */
+    uint64_t rays;             /* element x sub-cell (x scatterer) terms */
+    uint64_t fp64_fallbacks;   /* FP32 terms recomputed in FP64 */
+    uint64_t overflow_waves;   /* waves whose candidate lists overflowed */
+} smg_stats;
+
+typedef struct smg_ctx smg_ctx;
+
+/* ---- context --------------------------------------------------------- */
+smg_status smg_device_count(int* count);
+smg_status smg_create(int device, smg_ctx** ctx);
+smg_status smg_destroy(smg_ctx* ctx);
+const char* smg_last_error(const smg_ctx* ctx);
+const char* smg_status_string(smg_status status);
+/* The context's stream (a hipStream_t) for callers that time or order work. */
+void* smg_stream(smg_ctx* ctx);
+smg_status smg_synchronize(smg_ctx* ctx);
+
+/* ---- geometry ----------------------------------------------------------
+ * The four parameter arrays of Params_{Collimator,Detector,Image,Physics}.dat
+ * (PEGen_RayTracing_CircularHole/PEGen_CircularHole.cpp:33-56): at least
+ * 80000, 80000, 100 and 100 floats.  Copied into the context (FP32 and FP64
+ * records on the device).  img[20] (rotation index) is ignored: the rotation
+ * is an explicit argument of each entry point. */
+smg_status smg_set_geometry(smg_ctx* ctx, const float* coll, size_t n_coll, const float* det,
+                            size_t n_det, const float* img, size_t n_img, const float* phys,
+                            size_t n_phys);
+/* Sizes parsed from the geometry: bars N, voxels V, rotations R, layer-0 holes. */
+smg_status smg_get_dims(const smg_ctx* ctx, int* n_det, int64_t* n_vox, int* n_rot,
+                        int* n_holes_l0);
+/* Sub-cells per bar of the PE matrix.  The reference hard-codes 8 x 16 x 8
+ * (PEGen_RayTracing_CircularHole/PESysMatGen.cu:1037), the default here. */
+smg_status smg_set_pe_subsamples(smg_ctx* ctx, int sx, int sy, int sz);
+/* Detection sub-cells of the scatter matrices; the reference uses 1 x 1 x 1
+ * (ScatterGen_RayTracing_CircularHole/scatter.cu:1140), the default here. */
+smg_status smg_set_scatter_subsamples(smg_ctx* ctx, int sx, int sy, int sz);
+
+/* ---- photoelectric matrix (replaces PESysMatGen) ---------------------------
+ * Writes the N x V block of rotation `rotation` to dst. */
+smg_status smg_pe_forward(smg_ctx* ctx, int rotation, smg_precision prec, float* dst,
+                          int dst_is_device, smg_stats* stats);
+
+/* dL/dmu of L = sum_dv w[d][v] A[d][v] (adjoint of smg_pe_forward), for the
+ * mu of each collimator layer, mu_total and mu_pe of each bar:
+ * grad[0 .. Lc-1], grad[Lc .. Lc+N-1], grad[Lc+N .. Lc+2N-1].  w is N x V.
+ * loss (optional) receives L.  Deterministic: fixed-order reductions. */
+smg_status smg_pe_backward(smg_ctx* ctx, int rotation, const float* w, int w_is_device,
+                           double* grad, double* loss, smg_stats* stats);
+
+/* ---- geometry bitmaps (the scatter kernels' attenuation culling) ------------
+ * Crystal bitmap: N(N+1)/2 * N bits; collimator bitmap: H0 * N * N bits,
+ * packed little-endian in 32-bit words, the reference's cache-file format
+ * (ScatterGen_RayTracing_CircularHole/scatter.cu:2053-2063, :2233-2243). */
+int64_t smg_bitmap_crystal_words(int n_det);
+int64_t smg_bitmap_collimator_words(int n_holes_l0, int n_det);
+/* Compute the bitmap on the GPU, keep it in the context, and copy it to
+ * words (may be NULL) of n_words words. */
+smg_status smg_bitmap_crystal(smg_ctx* ctx, uint32_t* words, int64_t n_words, int words_is_device);
+smg_status smg_bitmap_collimator(smg_ctx* ctx, uint32_t* words, int64_t n_words,
+                                 int words_is_device);
+/* Install a bitmap read from a cache file instead of computing it. */
+smg_status smg_load_bitmap_crystal(smg_ctx* ctx, const uint32_t* words, int64_t n_words);
+smg_status smg_load_bitmap_collimator(smg_ctx* ctx, const uint32_t* words, int64_t n_words);
+
+/* ---- primary Compton scatter (replaces the kernels behind scatter()) --------
+ * Crystal scatter: pe is the N x V PE block used as scatter weights
+ * (ScatterGen_RayTracing_CircularHole/scatter.cu:1077-1437).  Uses the
+ * crystal bitmap of the context (computed on first use if none is set). */
+smg_status smg_scatter_crystal(smg_ctx* ctx, int rotation, smg_precision prec, const float* pe,
+                               int pe_is_device, float* dst, int dst_is_device, smg_stats* stats);
+/* Collimator scatter at the layer-0 hole centres (scatter.cu:1544-1936).
+ * Uses the collimator bitmap of the context (computed on first use). */
+smg_status smg_scatter_collimator(smg_ctx* ctx, int rotation, smg_coll_variant variant,
+                                  smg_precision prec, float* dst, int dst_is_device,
+                                  smg_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SMG_H */

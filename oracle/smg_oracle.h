@@ -1,0 +1,99 @@
+/*
+ * FP64 host oracle for the SPECT/PET system-matrix generator.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only the tests, the smoke check and the CPU
+ * baseline leg of the benchmark load this library, and only as the checker.
+ * The shipped library never links or calls it.
+ *
+ * Every function restates, in double precision, the formula of the
+ * reference kernel it cites (paths relative to the reference repository).
+ * The geometry parameters come in as the same float32 arrays the GPU path
+ * reads, converted to double without rounding.
+ *
+ * Gradient layout (adjoint): [mu of each collimator layer (Lc)]
+ *                            [mu_total of each bar (N)] [mu_pe of each bar (N)].
+ */
+#ifndef SMG_ORACLE_H
+#define SMG_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct smg_o_geom smg_o_geom;
+
+/* Parse the four parameter arrays (80000/80000/100/100 floats).  The arrays
+ * are referenced, not copied, and must outlive the handle.  Returns NULL if
+ * a count is out of range. */
+smg_o_geom* smg_o_geom_new(const float* coll, const float* det, const float* img,
+                           const float* phys);
+void smg_o_geom_free(smg_o_geom* g);
+int smg_o_n_det(const smg_o_geom* g);
+int smg_o_n_vox(const smg_o_geom* g);
+int smg_o_n_holes_l0(const smg_o_geom* g);
+int smg_o_n_layers(const smg_o_geom* g);
+
+/* ---- geometry primitives ------------------------------------------------ */
+double smg_o_box_ray(const double a[3], const double b[3], const double lo[3],
+                     const double hi[3]);
+double smg_o_box_ray_inside(const double a[3], const double b[3], const double lo[3],
+                            const double hi[3]);
+double smg_o_cylinder_ray(const double a[3], const double b[3], double xc, double y1,
+                          double y2, double zc, double radius);
+
+/* ---- Compton physics helpers ------------------------------------------- */
+double smg_o_kn(double theta, double e0);
+double smg_o_kn_sum(double e0, double lo, double hi, double step);
+double smg_o_cone_angle(const double src[3], const double a[3], const double det[3]);
+double smg_o_scatter_energy(double theta, double e0);
+double smg_o_gauss_window(double e, double res, double lo, double hi);
+
+/* ---- photoelectric matrix ---------------------------------------------- */
+double smg_o_pe_element(const smg_o_geom* g, int d, int v, int rot, int sx, int sy, int sz);
+/* rows[i] = detector index; out is [nrows][V]; OpenMP over rows. */
+void smg_o_pe_rows(const smg_o_geom* g, int rot, int sx, int sy, int sz, const int* rows,
+                   int nrows, double* out, int nthreads);
+/* Elements listed as (d, v) pairs. */
+void smg_o_pe_pairs(const smg_o_geom* g, int rot, int sx, int sy, int sz, const int* d,
+                    const int* v, int n, double* out, int nthreads);
+/* Culling counts of one element (crystals kept, holes kept over all layers). */
+void smg_o_pe_cull_counts(const smg_o_geom* g, int d, int v, int rot, int* n_cryst,
+                          int* n_holes);
+
+/* dL/dmu for L = sum_dv w[d][v] A[d][v]; grad has Lc + 2N entries. */
+void smg_o_pe_grad(const smg_o_geom* g, int rot, int sx, int sy, int sz, const double* w,
+                   double* grad, double* loss, int nthreads);
+
+/* ---- geometry bitmaps --------------------------------------------------- */
+int64_t smg_o_bitmap_crystal_words(int n_det);
+int64_t smg_o_bitmap_collimator_words(int n_holes_l0, int n_det);
+void smg_o_bitmap_crystal(const smg_o_geom* g, uint32_t* words, int nthreads);
+void smg_o_bitmap_collimator(const smg_o_geom* g, uint32_t* words, int nthreads);
+int smg_o_bit_crystal(const uint32_t* words, int n_det, int i, int j, int k);
+int smg_o_bit_collimator(const uint32_t* words, int n_det, int i, int j, int k);
+
+/* ---- primary Compton scatter -------------------------------------------- */
+/* pe is the [N][V] PE slice of the voxel grid; sx,sy,sz = detection-cell
+ * subsampling (the reference uses 1,1,1). */
+double smg_o_crystal_scatter_element(const smg_o_geom* g, const float* pe,
+                                     const uint32_t* bitmap, int d, int v, int rot, int sx,
+                                     int sy, int sz);
+/* variant 0: combined generator (r = 0.1 mm); 1: standalone (r = 0.2 mm). */
+double smg_o_collimator_scatter_element(const smg_o_geom* g, const uint32_t* bitmap, int d,
+                                        int v, int rot, int variant, int sx, int sy, int sz);
+void smg_o_crystal_scatter_pairs(const smg_o_geom* g, const float* pe, const uint32_t* bitmap,
+                                 int rot, int sx, int sy, int sz, const int* d, const int* v,
+                                 int n, double* out, int nthreads);
+void smg_o_collimator_scatter_pairs(const smg_o_geom* g, const uint32_t* bitmap, int rot,
+                                    int variant, int sx, int sy, int sz, const int* d,
+                                    const int* v, int n, double* out, int nthreads);
+
+int smg_o_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
