@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X system-matrix generator.
+
+One step = one full pass of the photoelectric (PE) system-matrix kernel over
+the full-size geometry: 64 x 64 x 32 voxels at 0.5 mm, 2048 detector bars in
+two layers, 400-hole tungsten plate, 3 x 3 x 3 sub-cells per bar: 268,435,456
+matrix elements, 7.25e9 sub-cell rays.  The geometry and the output buffer
+(1 GiB) are resident on the GPU before the timed region; the output stays on
+the GPU.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision fp32|fp64]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "gpu-based-system-matrix-calculation-for-spect-pet_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "matrix elements/s (PE, scatter) at ≤1e-5 rel err vs FP64; fwd+grad wall time"
+# MI355X published peaks
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(p, sub, target_s: float):
+    """FP64 host oracle (OpenMP, all host cores) on a seeded sample of the same
+    workload, scaled to elements/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+
+    o = oracle.Oracle(p)
+    threads = oracle.max_threads()
+    rng = np.random.default_rng(3)
+    n = 4096
+    t_used = 0.0
+    done = 0
+    while True:
+        d = rng.integers(0, o.n_det, n)
+        v = rng.integers(0, o.n_vox, n)
+        t0 = time.perf_counter()
+        o.pe_pairs(d, v, sub=sub, nthreads=threads)
+        dt = time.perf_counter() - t0
+        t_used += dt
+        done += n
+        if t_used >= target_s:
+            break
+        n = int(min(max(n * 2, 4096), max(4096, n * (target_s - t_used) / max(dt, 1e-3))))
+    return {
+        "value": done / t_used,
+        "unit": "matrix elements/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{done} seeded random (bar, voxel) elements of the same geometry and "
+                  f"sub-cells, FP64 oracle, {t_used:.1f} s on {threads} host threads",
+    }
+
+
+def ops_per_ray(p, sub):
+    """Algorithmic FP operations per sub-cell ray of the reference formula,
+    counted by the oracle's op counter on a seeded sample of elements."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+
+    o = oracle.Oracle(p)
+    rng = np.random.default_rng(12)
+    d = rng.integers(0, o.n_det, 512)
+    v = rng.integers(0, o.n_vox, 512)
+    return o.pe_ops_per_ray(d, v, sub=sub)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="host time budget of the CPU-baseline sample (0 to skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by detector rows, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import smg
+    from smg import params
+
+    p = params.config(2)
+    sub = p.meta["subsamples"]
+    ctx = smg.open_context(p, device=0)
+    ctx.set_pe_subsamples(*sub)
+    n_el = ctx.dims.n_det * ctx.dims.n_vox
+    out = torch.empty(n_el, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    for i in range(args.warmup):
+        _, st = ctx.pe_forward(0, args.precision, device_ptr=out.data_ptr())
+        log(f"warmup {i}: kernel {st['kernel_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    kernel_ms = []
+    fallbacks = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _, st = ctx.pe_forward(0, args.precision, device_ptr=out.data_ptr())
+        kernel_ms.append(st["kernel_ms"])
+        fallbacks += st["fp64_fallbacks"]
+        log(f"step {i}: kernel {st['kernel_ms']:.1f} ms")
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rays = st["rays"]
+    ms_per_step = wall / args.steps * 1e3
+    value = n_el * args.steps / wall
+    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+
+    f_ray = ops_per_ray(p, sub)
+    achieved = f_ray * rays / avg_kernel_s / 1e12
+    roofline = {
+        "bound": "valu",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+        "traffic": None,
+        "kernel": "pe_forward_kernel",
+        "flops_per_ray": round(f_ray, 1),
+        "rays_per_launch": rays,
+        "write_GBps": round(n_el * 4 / avg_kernel_s / 1e9, 2),
+    }
+    cpu = cpu_baseline(p, sub, args.cpu_seconds) if args.cpu_seconds > 0 else None
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "matrix elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "PE matrix, full size: 64x64x32 voxels @0.5 mm x 2048 bars "
+                        "(2 layers of 32x32, 2.5x8x2.5 mm), 80x80x8 mm tungsten plate with "
+                        "400 holes, 3x3x3 sub-cells, 1 rotation",
+            "elements": n_el,
+            "rays_per_step": rays,
+            "precision": args.precision,
+            "fp64_fallback_rays_per_step": fallbacks // max(args.steps, 1),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

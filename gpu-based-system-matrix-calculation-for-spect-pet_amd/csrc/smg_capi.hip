@@ -68,6 +68,13 @@ struct smg_ctx {
     bool have_bm_crystal = false, have_bm_coll = false;
     DevBuf<double> scratch;   // adjoint partials
     DevBuf<double> grad;
+    DevBuf<Sphere<float>> sph_f;       // bars, raw frame (bitmaps)
+    DevBuf<Sphere<double>> sph_d;
+    DevBuf<Sphere<float>> hsph_f;      // layer-0 hole centres, raw frame
+    DevBuf<Sphere<double>> hsph_d;
+    DevBuf<ScatterPt<double>> pts_bar;   // bar centres (crystal scatter)
+    DevBuf<ScatterPt<double>> pts_hole;  // layer-0 hole centres (collimator scatter)
+    DevBuf<PairTerm> pair;
 };
 
 namespace {
@@ -143,6 +150,189 @@ smg_status check_ready(smg_ctx* c, int rotation) {
     return SMG_OK;
 }
 
+
+// Records of the bitmaps and scatter kernels.
+void make_scatter_records(const HostGeom& hg, std::vector<Sphere<float>>& sf,
+                          std::vector<Sphere<double>>& sd, std::vector<Sphere<float>>& hf,
+                          std::vector<Sphere<double>>& hd, std::vector<ScatterPt<double>>& pb,
+                          std::vector<ScatterPt<double>>& ph) {
+    const GeomDesc& g = hg.g;
+    const double F = g.fov2coll;
+    for (int k = 0; k < g.n_det; ++k) {
+        const float* b = hg.det.data() + 12 * k;
+        double w = b[4], t = b[5], h = b[6];
+        Sphere<double> S{(double)b[1], (double)b[2], (double)b[3], std::sqrt(w * w + h * h + t * t) / 2.0};
+        sd.push_back(S);
+        sf.push_back(Sphere<float>{(float)S.x, (float)S.y, (float)S.z, (float)S.r});
+        const Bar<double>& B = hg.bars_d[k];
+        pb.push_back(ScatterPt<double>{B.cx, B.cy, B.cz, 0.0});
+    }
+    for (int i = 0; i < hg.n_holes_l0; ++i) {
+        const float* hp = hg.coll.data() + 100 + 9 * i;
+        double y = ((double)hp[1] + (double)hp[2]) / 2.0;
+        Sphere<double> S{(double)hp[0], y, (double)hp[3], 0.0};
+        hd.push_back(S);
+        hf.push_back(Sphere<float>{(float)S.x, (float)S.y, (float)S.z, 0.0f});
+        ph.push_back(ScatterPt<double>{(double)hp[0], y + F, (double)hp[3],
+                                       (double)hp[2] - (double)hp[1]});
+    }
+}
+
+// Sum of KN(theta) sin(theta) * step over [0, pi] with the reference's float
+// constants (ScatterGen_RayTracing_CircularHole/scatter.cu:1169, :980-993).
+double kn_norm(double e0) {
+    const double step = (double)0.01f, hi = (double)(float)3.14159265358979323846;
+    int n = (int)(hi / step);
+    double s = 0.0, alpha = e0 / 511.0;
+    for (int i = 0; i <= n; ++i) {
+        double th = i * step;
+        double c = std::cos(th);
+        double f1 = alpha * (1.0 - c), f2 = 1.0 + c * c;
+        s += f2 / (1.0 + f1) / (1.0 + f1) * (1.0 + f1 * f1 / f2 / (1.0 + f1)) * std::sin(th) * step;
+    }
+    return s;
+}
+
+smg_status ensure_bitmap(smg_ctx* c, bool collimator) {
+    const HostGeom& hg = c->hg;
+    const int N = hg.g.n_det;
+    if (collimator ? c->have_bm_coll : c->have_bm_crystal) return SMG_OK;
+    int64_t n_words = collimator ? smg_bitmap_collimator_words(hg.n_holes_l0, N)
+                                 : smg_bitmap_crystal_words(N);
+    DevBuf<uint32_t>& buf = collimator ? c->bm_coll : c->bm_crystal;
+    SMG_HIP(c, buf.reserve((size_t)n_words));
+    BitmapLaunch B;
+    B.spheres_f = c->sph_f.ptr;
+    B.spheres_d = c->sph_d.ptr;
+    B.holes_f = c->hsph_f.ptr;
+    B.holes_d = c->hsph_d.ptr;
+    B.words = buf.ptr;
+    B.n_words = n_words;
+    B.n_det = N;
+    B.n_holes = hg.n_holes_l0;
+    SMG_HIP(c, launch_bitmap(B, collimator, c->stream));
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    (collimator ? c->have_bm_coll : c->have_bm_crystal) = true;
+    return SMG_OK;
+}
+
+smg_status bitmap_entry(smg_ctx* c, bool collimator, uint32_t* words, int64_t n_words,
+                        int words_is_device) {
+    smg_status s = check_ready(c, 0);
+    if (s != SMG_OK) return s;
+    const HostGeom& hg = c->hg;
+    int64_t need = collimator ? smg_bitmap_collimator_words(hg.n_holes_l0, hg.g.n_det)
+                              : smg_bitmap_crystal_words(hg.g.n_det);
+    if (words && n_words < need)
+        return fail(c, SMG_ERR_BAD_PARAM, "bitmap buffer holds %lld words, %lld needed",
+                    (long long)n_words, (long long)need);
+    if (collimator)
+        c->have_bm_coll = false;
+    else
+        c->have_bm_crystal = false;
+    if ((s = ensure_bitmap(c, collimator)) != SMG_OK) return s;
+    if (words && need > 0) {
+        const DevBuf<uint32_t>& buf = collimator ? c->bm_coll : c->bm_crystal;
+        SMG_HIP(c, hipMemcpyAsync(words, buf.ptr, need * sizeof(uint32_t),
+                                  words_is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                  c->stream));
+        SMG_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return SMG_OK;
+}
+
+smg_status load_bitmap(smg_ctx* c, bool collimator, const uint32_t* words, int64_t n_words) {
+    smg_status s = check_ready(c, 0);
+    if (s != SMG_OK) return s;
+    const HostGeom& hg = c->hg;
+    int64_t need = collimator ? smg_bitmap_collimator_words(hg.n_holes_l0, hg.g.n_det)
+                              : smg_bitmap_crystal_words(hg.g.n_det);
+    if (!words || n_words < need)
+        return fail(c, SMG_ERR_BAD_PARAM, "bitmap holds %lld words, %lld needed",
+                    (long long)n_words, (long long)need);
+    DevBuf<uint32_t>& buf = collimator ? c->bm_coll : c->bm_crystal;
+    SMG_HIP(c, buf.reserve((size_t)need));
+    SMG_HIP(c, hipMemcpyAsync(buf.ptr, words, need * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    (collimator ? c->have_bm_coll : c->have_bm_crystal) = true;
+    return SMG_OK;
+}
+
+smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
+                         smg_precision prec, const float* pe, int pe_is_device, float* dst,
+                         int dst_is_device, smg_stats* stats) {
+    double t_start = now_ms();
+    smg_status s = check_ready(c, rotation);
+    if (s != SMG_OK) return s;
+    if (!dst) return fail(c, SMG_ERR_BAD_PARAM, "dst is NULL");
+    if (!collimator && !pe) return fail(c, SMG_ERR_BAD_PARAM, "PE block is NULL");
+    const HostGeom& hg = c->hg;
+    const GeomDesc& g = hg.g;
+    const size_t n_el = (size_t)g.n_det * (size_t)g.n_vox;
+    if ((s = ensure_bitmap(c, collimator)) != SMG_OK) return s;
+    ScatterLaunch A;
+    A.geom = device_geom(c);
+    A.kind = collimator ? 1 : 0;
+    A.variant = variant;
+    A.pts_d = collimator ? c->pts_hole.ptr : c->pts_bar.ptr;
+    A.n_pts = collimator ? hg.n_holes_l0 : g.n_det;
+    A.bitmap = collimator ? c->bm_coll.ptr : c->bm_crystal.ptr;
+    A.pe = nullptr;
+    if (!collimator) {
+        if (pe_is_device) {
+            A.pe = pe;
+        } else {
+            SMG_HIP(c, c->in.reserve(n_el));
+            SMG_HIP(c, hipMemcpyAsync(c->in.ptr, pe, n_el * sizeof(float), hipMemcpyHostToDevice, c->stream));
+            A.pe = c->in.ptr;
+        }
+    }
+    const int ncell = c->sc_sub[0] * c->sc_sub[1] * c->sc_sub[2];
+    SMG_HIP(c, c->pair.reserve((size_t)g.n_det * (size_t)(A.n_pts > 0 ? A.n_pts : 1) * ncell));
+    A.pair = c->pair.ptr;
+    float* out = dst;
+    if (!dst_is_device) {
+        SMG_HIP(c, c->out.reserve(n_el));
+        out = c->out.ptr;
+    }
+    A.dst = out;
+    A.counters = c->counters.ptr;
+    double phi = (double)rotation * g.dphi;
+    A.cphi = std::cos(phi);
+    A.sphi = std::sin(phi);
+    for (int k = 0; k < 3; ++k) A.sub[k] = c->sc_sub[k];
+    const float* ph = hg.phys.data();
+    A.compton_on = (int)std::floor((double)ph[0] + 0.5) == 1;
+    A.same_window = (int)std::floor((double)ph[4] + 0.5) > 0;
+    A.win_lo = ph[5];
+    A.win_hi = ph[6];
+    A.e0 = ph[7];
+    A.kn_norm = kn_norm(A.e0);
+    A.coll_ratio = (double)hg.coll[17] / (double)hg.coll[15];
+    if (collimator && !(hg.coll[15] != 0.0f)) A.coll_ratio = 0.0;
+    SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    SMG_HIP(c, launch_pair_terms(A, c->stream));
+    SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    if (!dst_is_device)
+        SMG_HIP(c, hipMemcpyAsync(dst, out, n_el * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    SMG_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t_start;
+        stats->elements = n_el;
+        stats->rays = n_el * (uint64_t)(A.n_pts > 0 ? A.n_pts : 0) * (uint64_t)ncell;
+        stats->fp64_fallbacks = cnt[0];
+        stats->overflow_waves = 0;
+    }
+    return SMG_OK;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -204,6 +394,13 @@ smg_status smg_destroy(smg_ctx* c) {
     c->bm_coll.release();
     c->scratch.release();
     c->grad.release();
+    c->sph_f.release();
+    c->sph_d.release();
+    c->hsph_f.release();
+    c->hsph_d.release();
+    c->pts_bar.release();
+    c->pts_hole.release();
+    c->pair.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -244,6 +441,17 @@ smg_status smg_set_geometry(smg_ctx* c, const float* coll, size_t n_coll, const 
     if ((s = upload(c, c->holes_d, hg.holes_d)) != SMG_OK) return s;
     if ((s = upload(c, c->layers_f, hg.layers_f)) != SMG_OK) return s;
     if ((s = upload(c, c->layers_d, hg.layers_d)) != SMG_OK) return s;
+    std::vector<Sphere<float>> sf;
+    std::vector<Sphere<double>> sd, hd;
+    std::vector<Sphere<float>> hf;
+    std::vector<ScatterPt<double>> pb, ph;
+    make_scatter_records(hg, sf, sd, hf, hd, pb, ph);
+    if ((s = upload(c, c->sph_f, sf)) != SMG_OK) return s;
+    if ((s = upload(c, c->sph_d, sd)) != SMG_OK) return s;
+    if ((s = upload(c, c->hsph_f, hf)) != SMG_OK) return s;
+    if ((s = upload(c, c->hsph_d, hd)) != SMG_OK) return s;
+    if ((s = upload(c, c->pts_bar, pb)) != SMG_OK) return s;
+    if ((s = upload(c, c->pts_hole, ph)) != SMG_OK) return s;
     SMG_HIP(c, hipStreamSynchronize(c->stream));
     c->hg = std::move(hg);
     c->have_geom = true;
@@ -327,6 +535,47 @@ smg_status smg_pe_forward(smg_ctx* c, int rotation, smg_precision prec, float* d
                     "%llu waves exceeded the candidate capacity (%d bars, %d holes)", cnt[1],
                     kCapBars, kCapHoles);
     return SMG_OK;
+}
+
+int64_t smg_bitmap_crystal_words(int n_det) {
+    if (n_det < 0) return 0;
+    int64_t bits = (int64_t)n_det * (n_det + 1) / 2 * n_det;
+    return (bits + 31) / 32;
+}
+
+int64_t smg_bitmap_collimator_words(int n_holes_l0, int n_det) {
+    if (n_det < 0 || n_holes_l0 < 0) return 0;
+    int64_t bits = (int64_t)n_holes_l0 * n_det * n_det;
+    return (bits + 31) / 32;
+}
+
+smg_status smg_bitmap_crystal(smg_ctx* c, uint32_t* words, int64_t n_words, int words_is_device) {
+    return bitmap_entry(c, false, words, n_words, words_is_device);
+}
+
+smg_status smg_bitmap_collimator(smg_ctx* c, uint32_t* words, int64_t n_words, int words_is_device) {
+    return bitmap_entry(c, true, words, n_words, words_is_device);
+}
+
+smg_status smg_load_bitmap_crystal(smg_ctx* c, const uint32_t* words, int64_t n_words) {
+    return load_bitmap(c, false, words, n_words);
+}
+
+smg_status smg_load_bitmap_collimator(smg_ctx* c, const uint32_t* words, int64_t n_words) {
+    return load_bitmap(c, true, words, n_words);
+}
+
+smg_status smg_scatter_crystal(smg_ctx* c, int rotation, smg_precision prec, const float* pe,
+                               int pe_is_device, float* dst, int dst_is_device, smg_stats* stats) {
+    return scatter_entry(c, false, rotation, 0, prec, pe, pe_is_device, dst, dst_is_device, stats);
+}
+
+smg_status smg_scatter_collimator(smg_ctx* c, int rotation, smg_coll_variant variant,
+                                  smg_precision prec, float* dst, int dst_is_device,
+                                  smg_stats* stats) {
+    if (variant != SMG_COLL_COMBINED && variant != SMG_COLL_STANDALONE)
+        return fail(c, SMG_ERR_BAD_PARAM, "unknown collimator variant %d", (int)variant);
+    return scatter_entry(c, true, rotation, (int)variant, prec, nullptr, 0, dst, dst_is_device, stats);
 }
 
 }  // extern "C"

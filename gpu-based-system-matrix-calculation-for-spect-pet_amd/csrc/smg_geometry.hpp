@@ -42,6 +42,21 @@ struct Layer {
     T thick, mu;      // thickness (y) and mu_total of the plate
 };
 
+// Bounding sphere of a bar, or centre of a layer-0 hole, in the raw
+// parameter frame (no F added): the bitmaps use these positions
+// (ScatterGen_RayTracing_CircularHole/scatter.cu:1465-1467, :1963-1965).
+template <class T>
+struct Sphere {
+    T x, y, z, r;
+};
+
+// A scatter point: bar centre (crystal scatter) or layer-0 hole centre
+// (collimator scatter), world frame with F; dy = y2 - y1 of the hole.
+template <class T>
+struct ScatterPt {
+    T x, y, z, dy;
+};
+
 struct LayerRange {
     int32_t first, count;  // holes of this layer in the global hole table
 };

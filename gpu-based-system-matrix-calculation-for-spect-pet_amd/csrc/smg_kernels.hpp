@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "smg_pe_body.hpp"
+#include "smg_scatter_body.hpp"
 
 namespace smg {
 
@@ -25,5 +26,39 @@ struct PeLaunch {
 };
 
 hipError_t launch_pe_forward(const PeLaunch& a, bool exact, hipStream_t stream);
+
+struct BitmapLaunch {
+    const Sphere<float>* spheres_f;   // bars
+    const Sphere<double>* spheres_d;
+    const Sphere<float>* holes_f;     // layer-0 hole centres
+    const Sphere<double>* holes_d;
+    uint32_t* words;
+    int64_t n_words;
+    int n_det, n_holes;
+};
+
+hipError_t launch_bitmap(const BitmapLaunch& a, bool collimator, hipStream_t stream);
+
+struct ScatterLaunch {
+    PeGeom geom;
+    const ScatterPt<double>* pts_d;  // scatter points
+    int n_pts;
+    int kind;                        // 0: crystal scatter, 1: collimator scatter
+    int variant;                     // collimator: 0 combined (r = 0.1), 1 standalone (r = 0.2)
+    const float* pe;                 // crystal: PE block [N][V]
+    const uint32_t* bitmap;          // crystal or collimator bitmap
+    PairTerm* pair;                  // [N][n_pts][cells]
+    float* dst;                      // [N][V]
+    unsigned long long* counters;    // [0] FP64 fallback terms
+    double cphi, sphi;
+    int sub[3];                      // detection sub-cells
+    double e0, win_lo, win_hi;       // photopeak energy, energy window
+    double kn_norm;                  // KN sum over [0, pi]
+    double coll_ratio;               // mu_compton / mu_total of collimator layer 0
+    int same_window, compton_on;
+};
+
+hipError_t launch_pair_terms(const ScatterLaunch& a, hipStream_t stream);
+hipError_t launch_scatter(const ScatterLaunch& a, bool exact, hipStream_t stream);
 
 }  // namespace smg

@@ -1,0 +1,225 @@
+// Primary-Compton scatter matrices for gfx950: crystal scatter (scatter point
+// = centre of another bar, weighted by the PE matrix) and collimator scatter
+// (scatter point = centre of a layer-0 hole, weighted by a virtual-disc solid
+// angle).  References: ScatterGen_RayTracing_CircularHole/scatter.cu:1077-1437
+// and :1544-1936, ScatterGen_Collimator/scatter.cu:1056-1435.
+//
+// Two kernels:
+//   pair_terms_kernel  one thread per (bar d, scatter point s, sub-cell): the
+//                      voxel-independent second-leg sums S_pe, S_c over the
+//                      bars flagged in the geometry bitmap, the own-bar path
+//                      and the absorption chord, in FP64.
+//   scatter_kernel     one wave per (bar d, 64 consecutive voxels); each lane
+//                      owns one element and loops over scatter points and
+//                      sub-cells, summing with a two-sum compensated sum.
+//                      The reference instead runs one thread per
+//                      (d, v, s) and sums with float atomicAdd, in an order
+//                      that changes from run to run (scatter.cu:1427).
+#include <hip/hip_runtime.h>
+
+#include "smg_kernels.hpp"
+#include "smg_scatter_body.hpp"
+
+namespace smg {
+
+namespace {
+
+__device__ __forceinline__ bool bit_set(const uint32_t* words, int64_t b) {
+    return (words[b >> 5] >> (b & 31)) & 1u;
+}
+
+__global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const PeGeom& G = A.geom;
+    const int N = G.g.n_det;
+    const int ncell = A.sub[0] * A.sub[1] * A.sub[2];
+    const int64_t total = (int64_t)N * A.n_pts * ncell;
+    if (tid >= total) return;
+    const int cell = (int)(tid % ncell);
+    const int s = (int)((tid / ncell) % A.n_pts);
+    const int d = (int)(tid / ((int64_t)ncell * A.n_pts));
+    PairTerm out{0.0, 0.0, 0.0, 0.0};
+    if (A.kind == 0 && s == d) {
+        A.pair[tid] = out;
+        return;
+    }
+    const Bar<double>& bd = G.bars_d[d];
+    const int a = cell % A.sub[0], b = (cell / A.sub[0]) % A.sub[1], c = cell / (A.sub[0] * A.sub[1]);
+    CellCtx cc;
+    make_cell_ctx(bd, a, b, c, A.sub[0], A.sub[1], A.sub[2], cc);
+    const ScatterPt<double>& sp = A.pts_d[s];
+    double qs[3] = {sp.x, sp.y, sp.z};
+    bool near = false;
+    // bars flagged between the scatter point and bar d (scatter.cu:1343-1401)
+    int64_t base;
+    if (A.kind == 0) {
+        int64_t i = s < d ? s : d, j = s < d ? d : s;
+        base = (i * N - i * (i + 1) / 2 + j) * N;
+    } else {
+        base = ((int64_t)s * N + d) * N;
+    }
+    for (int k = 0; k < N; ++k) {
+        if (k == d || (A.kind == 0 && k == s)) continue;
+        if (!bit_set(A.bitmap, base + k)) continue;
+        const Bar<double>& bk = G.bars_d[k];
+        double pa[3], pb[3];
+        to_frame(bk, qs, pa);
+        to_frame(bk, cc.q, pb);
+        double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
+        double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
+        SegDir<double> sd;
+        segdir_init<double>(sd, pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2], 0.0, near);
+        double l = seg_box<double>(sd, pa, lo, hi, near);
+        out.s_pe += l * bk.mupe;
+        out.s_c += l * bk.muc;
+    }
+    // own bar from its surface to the sub-cell (scatter.cu:1405-1417)
+    double qss[3];
+    to_frame(bd, qs, qss);
+    double blo[3] = {-bd.w / 2.0, -bd.t / 2.0, -bd.h / 2.0};
+    double bhi[3] = {bd.w / 2.0, bd.t / 2.0, bd.h / 2.0};
+    SegDir<double> so;
+    segdir_init<double>(so, cc.qs[0] - qss[0], cc.qs[1] - qss[1], cc.qs[2] - qss[2], 0.0, near);
+    double l1 = seg_box_inside<double>(so, qss, blo, bhi, near);
+    double l2 = seg_box_inside<double>(so, qss, cc.lo, cc.hi, near);
+    out.s_pe += (l1 - l2) * bd.mupe;
+    out.s_c += (l1 - l2) * bd.muc;
+    // absorption chord, ray extended 1000 mm through the sub-cell (scatter.cu:1420-1424)
+    double dx = qs[0] - cc.q[0], dy = qs[1] - cc.q[1], dz = qs[2] - cc.q[2];
+    out.len = sqrt(dx * dx + dy * dy + dz * dz);
+    double De[3];
+    for (int k = 0; k < 3; ++k) De[k] = (cc.qs[k] + 1000.0 * (cc.qs[k] - qss[k]) / out.len) - qss[k];
+    SegDir<double> se;
+    segdir_init<double>(se, De[0], De[1], De[2], 0.0, near);
+    out.la = seg_box<double>(se, qss, cc.lo, cc.hi, near);
+    A.pair[tid] = out;
+}
+
+template <bool kExact>
+__global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A) {
+    using T = typename std::conditional<kExact, double, float>::type;
+    const PeGeom& G = A.geom;
+    const GeomDesc& g = G.g;
+    const int lane = threadIdx.x;
+    const int64_t waves_per_det = (g.n_vox + 63) / 64;
+    const int d = (int)(blockIdx.x / waves_per_det);
+    const int64_t v = (blockIdx.x % waves_per_det) * 64 + lane;
+    const bool valid = v < g.n_vox;
+    float* out = A.dst + (int64_t)d * g.n_vox;
+    const Bar<double>& bd = G.bars_d[d];
+    if (!A.compton_on || !(bd.mut > (double)0.01f)) {  // scatter.cu:1226-1228
+        if (valid) out[v] = 0.0f;
+        return;
+    }
+    const int64_t vv = valid ? v : 0;
+    const int nxy = g.nx * g.ny;
+    const int iz = (int)(vv / nxy), rem = (int)(vv % nxy);
+    double p[3], ps[3];
+    voxel_position(g, rem % g.nx, rem / g.nx, iz, A.cphi, A.sphi, p);
+    to_frame(bd, p, ps);
+
+    ScatterPhys ph;
+    ph.e0 = A.e0;
+    if (A.same_window) {
+        ph.lo = A.win_lo;
+        ph.hi = A.win_hi;
+    } else {  // per-bar window E0 (1 -+ res/2) (scatter.cu:1154-1163)
+        ph.lo = (1 - bd.res / 2.0) * A.e0;
+        ph.hi = (1 + bd.res / 2.0) * A.e0;
+    }
+    ph.norm = A.kn_norm;
+    ph.res = bd.res;
+    ph.mupe = bd.mupe;
+    ph.muc = bd.muc;
+    {
+        const double sx = A.sub[0], sy = A.sub[1], sz = A.sub[2];
+        ph.rcell = sqrt(bd.w * bd.w / sx / sx + bd.h * bd.h / sz / sz + bd.t * bd.t / sy / sy) / 2.0;
+    }
+    const int ncell = A.sub[0] * A.sub[1] * A.sub[2];
+    const T wscale = T(bd.muc / bd.mupe);  // crystal weight uses bar d's mu (scatter.cu:1218-1219)
+    KahanSum<T> acc;
+    unsigned fallbacks = 0;
+    for (int cell = 0; cell < ncell; ++cell) {
+        const int a = cell % A.sub[0], b = (cell / A.sub[0]) % A.sub[1], c = cell / (A.sub[0] * A.sub[1]);
+        CellCtx cc;
+        make_cell_ctx(bd, a, b, c, A.sub[0], A.sub[1], A.sub[2], cc);
+        ScatterCell sc;
+        for (int k = 0; k < 3; ++k) {
+            sc.q[k] = cc.q[k];
+            sc.lo[k] = cc.lo[k];
+            sc.hi[k] = cc.hi[k];
+        }
+        for (int corner = 0; corner < 8; ++corner) {
+            sc.corner[corner][0] = (corner & 1) ? cc.hi[0] : cc.lo[0];
+            sc.corner[corner][1] = (corner & 2) ? cc.hi[1] : cc.lo[1];
+            sc.corner[corner][2] = (corner & 4) ? cc.hi[2] : cc.lo[2];
+        }
+        for (int s = 0; s < A.n_pts; ++s) {
+            if (A.kind == 0 && s == d) continue;
+            const ScatterPt<double>& sp = A.pts_d[s];
+            sc.qs[0] = sp.x;
+            sc.qs[1] = sp.y;
+            sc.qs[2] = sp.z;
+            to_frame(bd, sc.qs, sc.qss);
+            sc.pt = A.pair[((int64_t)d * A.n_pts + s) * ncell + cell];
+            if (!valid) continue;
+            // weight of the scatter point for this voxel
+            T wgt;
+            if (A.kind == 0) {
+                wgt = T(A.pe[(int64_t)s * g.n_vox + v]) * wscale;
+            } else {
+                double dx = p[0] - sp.x, dy = p[1] - sp.y, dz = p[2] - sp.z;
+                double lv = sqrt(dx * dx + dy * dy + dz * dz);
+                double om;
+                if (A.variant == 0) {  // scatter.cu:1711-1716
+                    double r = (double)0.1f;
+                    om = (r * r * 3.14159265358979323846 + sp.dy * r) /
+                         (4.0 * 3.14159265358979323846 * lv * lv) * fabs(sp.y - p[1]) / lv;
+                } else {  // ScatterGen_Collimator/scatter.cu:1212-1215
+                    double r = (double)0.2f;
+                    om = (r * r + sp.dy * r) / (4.0 * lv * lv);
+                }
+                wgt = T(om * A.coll_ratio);
+            }
+            if (wgt == T(0)) continue;
+            bool near = false;
+            T val = scatter_term<T>(ph, p, ps, sc, wgt, near);
+            if constexpr (!kExact) {
+                if (near) {
+                    bool n2 = false;
+                    val = T(scatter_term<double>(ph, p, ps, sc, double(wgt), n2));
+                    ++fallbacks;
+                }
+            }
+            acc.add(val);
+        }
+    }
+    if (valid) out[v] = float(acc.value());
+    if constexpr (!kExact) {
+        unsigned tot = fallbacks;
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+        if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_pair_terms(const ScatterLaunch& A, hipStream_t stream) {
+    const int64_t total = (int64_t)A.geom.g.n_det * A.n_pts * A.sub[0] * A.sub[1] * A.sub[2];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pair_terms_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       stream, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const ScatterLaunch& A, bool exact, hipStream_t stream) {
+    const int64_t blocks = (int64_t)A.geom.g.n_det * ((A.geom.g.n_vox + 63) / 64);
+    if (blocks <= 0) return hipSuccess;
+    if (exact)
+        hipLaunchKernelGGL(scatter_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A);
+    else
+        hipLaunchKernelGGL(scatter_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A);
+    return hipGetLastError();
+}
+
+}  // namespace smg

@@ -1,0 +1,235 @@
+// Per-lane body of the primary-Compton scatter kernels.
+//
+// Reference: crystalScatterSysMatCuda and collimatorScatterSysMatCuda
+// (ScatterGen_RayTracing_CircularHole/scatter.cu:1077-1437, :1544-1936;
+// standalone variant ScatterGen_Collimator/scatter.cu:1056-1435).  One term is
+// (voxel v, scatter point s, detection sub-cell of bar d):
+//
+//   w_s(v) * Phi/(2 pi) * K(theta_min, theta_max) * P_E * exp(-att2) * abs'
+//
+// The second-leg attenuation runs from s to the sub-cell, so its chords do
+// not depend on v; only the energy scaling does, through E'(theta(v)):
+//   att2 = (E0/E')^3 * S_pe + (E0/E') * S_c,
+//   S_pe = sum_k l_k mu_pe_k + (l1 - l2) mu_pe_d,  S_c likewise with mu_c,
+// the same sum the reference forms term by term (scatter.cu:1343-1417).
+// S_pe, S_c, the absorption chord and |s - q| are computed once per
+// (d, s, sub-cell) by the FP64 pair-terms kernel and read here.
+#pragma once
+
+#include "smg_pe_body.hpp"
+
+namespace smg {
+
+struct PairTerm {
+    double s_pe, s_c, la, len;  // len = |scatter point - sub-cell centre|
+};
+
+struct ScatterPhys {
+    double e0, lo, hi;     // photopeak energy and energy window (keV)
+    double norm;           // sum of KN sin over [0, pi] (scatter.cu:1169)
+    double res;            // energy resolution of bar d at E0
+    double rcell;          // bounding radius of a detection sub-cell
+    double mupe, muc;      // bar d
+};
+
+// Thresholds with their reference values.
+constexpr double kFwhm = (double)2.35482f;  // scatter.cu:1026, :1263
+constexpr double kKnStep = (double)0.01f;   // scatter.cu:1338 (0.01 passed as float)
+constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
+
+// Klein-Nishina in the reference's factorised form (scatter.cu:962-977), from
+// cos(theta).
+template <class T>
+SMG_FN T kn_of_cos(T c, T alpha) {
+    T f1 = alpha * (T(1) - c);
+    T f2 = T(1) + c * c;
+    return f2 / (T(1) + f1) / (T(1) + f1) * (T(1) + f1 * f1 / f2 / (T(1) + f1));
+}
+
+// Inclusive rectangle rule of KN sin on lo + i*step, i = 0..n with
+// n = (int)((hi - lo)/step) (scatter.cu:980-993).  FP32: flags a step count
+// within rounding distance of an integer.
+template <class T>
+SMG_FN T kn_sum(T lo, T hi, T alpha, bool& near) {
+    const T step = T(kKnStep);
+    T x = (hi - lo) / step;
+    int n = (int)x;
+    if constexpr (kGuarded<T>) {
+        T fr = x - T(n);
+        near |= (fr < T(2e-4) && n > 0) | (fr > T(1) - T(2e-4));
+    }
+    KahanSum<T> s;
+    for (int i = 0; i <= n; ++i) {
+        T th = lo + T(i) * step;
+        T c = cos(th), sn = sin(th);
+        s.add(kn_of_cos(c, alpha) * sn * step);
+    }
+    return s.value();
+}
+
+// Angle between u and w (vectors src -> a and a -> det), used for the polar
+// range.  FP64 follows the reference, acos of the clamped cosine
+// (scatter.cu:996-1013); FP32 uses atan2(|u x w|, u.w), which keeps its
+// precision at small angles where acos of a rounded cosine does not.
+template <class T>
+SMG_FN T cone_angle(const T u[3], const T w[3]) {
+    T dot = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+    if constexpr (std::is_same<T, double>::value) {
+        T nu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        T nw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        T c = dot / (nu * nw);
+        if (c > T(1)) c = T(1);
+        if (c < T(-1)) c = T(-1);
+        return acos(c);
+    } else {
+        T cx = u[1] * w[2] - u[2] * w[1];
+        T cy = u[2] * w[0] - u[0] * w[2];
+        T cz = u[0] * w[1] - u[1] * w[0];
+        return atan2(sqrt(cx * cx + cy * cy + cz * cz), dot);
+    }
+}
+
+// Per (scatter point, sub-cell) quantities shared by the lanes.
+struct ScatterCell {
+    double qs[3];        // scatter point, world
+    double qss[3];       // scatter point, frame of bar d
+    double q[3];         // sub-cell centre, world
+    double lo[3], hi[3]; // sub-cell box, frame of bar d
+    double corner[8][3]; // sub-cell corners, frame of bar d
+    PairTerm pt;
+};
+
+// One (voxel, scatter point, sub-cell) term.  wgt is the weight of the
+// scatter point for this voxel.  Returns 0 for terms outside the 2-sigma
+// energy window (scatter.cu:1263-1269).
+template <class T>
+SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[3],
+                      const ScatterCell& sc, T wgt, bool& near) {
+    // Cone angle at the scatter point (scatter.cu:1254), scattered energy
+    // (scatter.cu:1016-1023) and energy-window probability (scatter.cu:1025-1033).
+    // The window probability sits on the tail of a Gaussian in E', which
+    // multiplies the relative error of E' by ~50: E' and the window arguments
+    // are formed in FP64 on both paths.  The FP64 path follows the reference
+    // (acos of the clamped cosine, then cos of the angle); the fast path takes
+    // cos and sin of the angle straight from the vectors.
+    double a[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+        a[k] = sc.qs[k] - p[k];
+        b[k] = sc.q[k] - sc.qs[k];
+    }
+    const double alpha = ph.e0 / 511.0;
+    double omc, st;  // 1 - cos(theta), sin(theta)
+    {
+        double dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+        double na2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+        double nb2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+        if constexpr (std::is_same<T, double>::value) {
+            double c = dot / (sqrt(na2) * sqrt(nb2));
+            if (c > 1.0) c = 1.0;
+            if (c < -1.0) c = -1.0;
+            double theta = acos(c);
+            omc = 1.0 - cos(theta);
+            st = sin(theta);
+        } else {
+            double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2],
+                   cz = a[0] * b[1] - a[1] * b[0];
+            double inv = 1.0 / sqrt(na2 * nb2);
+            double c = dot * inv;
+            double s2 = (cx * cx + cy * cy + cz * cz) * inv * inv;
+            st = sqrt(s2);
+            omc = c < 0.5 ? 1.0 - c : s2 / (1.0 + c);
+        }
+    }
+    const double ep = ph.e0 / (1.0 + alpha * omc);
+    const double resp = ph.res * sqrt(ep / ph.e0);
+    if (ep * (1 + 2 * resp / kFwhm) <= ph.lo || ep * (1 - 2 * resp / kFwhm) >= ph.hi)
+        return T(0);  // 2-sigma energy-window early out (scatter.cu:1263-1269)
+    T pe_win;
+    {
+        double sigma = resp / kFwhm * ep;
+        double s2 = sigma * sqrt(2.0);
+        double z1 = (ph.lo - ep) / s2, z2 = (ph.hi - ep) / s2;
+        if constexpr (std::is_same<T, double>::value) {
+            pe_win = 0.5 * (erf(z2) - erf(z1));
+        } else {  // erfc keeps the difference accurate when both z lie on one side
+            float f1 = float(z1), f2 = float(z2);
+            if (z1 >= 0.0)
+                pe_win = 0.5f * (erfcf(f1) - erfcf(f2));
+            else if (z2 <= 0.0)
+                pe_win = 0.5f * (erfcf(-f2) - erfcf(-f1));
+            else
+                pe_win = 0.5f * (erff(f2) - erff(f1));
+        }
+    }
+
+    // azimuthal range from the sub-cell's bounding sphere (scatter.cu:1275-1287);
+    // asin is ill-conditioned next to 1, where 1 - r is formed in FP64
+    T phi;
+    {
+        const double L = sc.pt.len, rdet = ph.rcell;
+        if (L * st * 2.0 <= rdet) {
+            phi = T(kTwoPi);
+        } else {
+            double r = rdet / 2.0 / L / st;
+            if constexpr (std::is_same<T, double>::value) {
+                phi = 4.0 * asin(r <= 1.0 ? r : 1.0);
+            } else if (r < 0.75) {
+                phi = 4.0f * asinf(float(r));
+            } else {  // asin(r) = pi/2 - 2 asin(sqrt((1 - r)/2))
+                phi = float(kTwoPi) - 8.0f * asinf(float(sqrt((1.0 - r) * 0.5)));
+            }
+        }
+    }
+    const T alpha_t = T(alpha);
+
+    // polar range: cone angles to the 8 sub-cell corners (scatter.cu:1290-1335)
+    T us[3];
+    for (int k = 0; k < 3; ++k) us[k] = T(sc.qss[k] - ps[k]);
+    T tmin = T(0), tmax = T(0);
+    for (int c = 0; c < 8; ++c) {
+        T wc[3];
+        for (int k = 0; k < 3; ++k) wc[k] = T(sc.corner[c][k] - sc.qss[k]);
+        T th = cone_angle<T>(us, wc);
+        if (c == 0 || th > tmax) tmax = th;
+        if (c == 0 || th < tmin) tmin = th;
+    }
+    // the line voxel -> scatter point, extended 1000 mm, through the sub-cell
+    T lthru;
+    {
+        double dis2 = 0.0;
+        double D[3];
+        for (int k = 0; k < 3; ++k) D[k] = sc.qss[k] - ps[k];
+        dis2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+        double dis = sqrt(dis2);
+        double De[3];
+        for (int k = 0; k < 3; ++k) De[k] = (sc.qss[k] + 1000.0 * D[k] / dis) - ps[k];
+        if constexpr (std::is_same<T, float>::value) {
+            SegDir<float> se;
+            float tde = 4.0f * kUnit<float> * (tabs(float(De[0])) + tabs(float(De[1])) + tabs(float(De[2])));
+            segdir_init<float>(se, float(De[0]), float(De[1]), float(De[2]), tde, near);
+            double le = sqrt(De[0] * De[0] + De[1] * De[1] + De[2] * De[2]);
+            float of[3] = {float(sc.qss[0]), float(sc.qss[1]), float(sc.qss[2])};
+            float lo[3] = {float(sc.lo[0]), float(sc.lo[1]), float(sc.lo[2])};
+            float hi[3] = {float(sc.hi[0]), float(sc.hi[1]), float(sc.hi[2])};
+            lthru = seg_box_local(se, of, float(double(kSlabEps<float>) - dis), float(le - dis), lo, hi, near);
+        } else {
+            SegDir<double> se;
+            segdir_init<double>(se, De[0], De[1], De[2], 0.0, near);
+            lthru = seg_box<double>(se, ps, sc.lo, sc.hi, near);
+        }
+    }
+    if (g_gt(lthru, T(0.001f), T(1e-5), near)) tmin = T(0);
+    T K = kn_sum<T>(tmin, tmax, alpha_t, near) / T(ph.norm);
+
+    // second leg and absorption at E' (scatter.cu:1371-1427)
+    const double e1 = ph.e0 / ep;
+    const double e3 = ph.e0 * ph.e0 * ph.e0 / ep / ep / ep;
+    double att2 = e3 * sc.pt.s_pe + e1 * sc.pt.s_c;
+    double mupe = ph.mupe * e3;
+    double mut = mupe + ph.muc * e1;
+    T ea = exp_neg<T>(sc.pt.la * mut);
+    T absf = (T(1) - ea) * T(mupe / mut);
+    return wgt * phi / T(kTwoPi) * K * pe_win * exp_neg<T>(att2) * absf;
+}
+
+}  // namespace smg

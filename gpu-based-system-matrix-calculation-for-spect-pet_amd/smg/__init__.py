@@ -65,6 +65,15 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run `make -C {PKG_DIR}` "
                           "(or __graft_entry__.build())")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 (same
+    # soname, libamdhip64.so.7) and loads it by an unversioned name, so if
+    # libsmg pulled in the system copy first, torch would load a second
+    # runtime that cannot open the GPU.  Importing torch first makes libsmg
+    # bind to torch's copy.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, sz = C.c_void_p, C.c_int, C.c_int64, C.c_size_t
     fp, dp, up = C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_uint32)

@@ -63,6 +63,9 @@ def lib():
                                 _dp, C.c_int]
     L.smg_o_pe_pairs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip,
                                  C.c_int, _dp, C.c_int]
+    L.smg_o_pe_ops_per_ray.restype = C.c_double
+    L.smg_o_pe_ops_per_ray.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip,
+                                       C.c_int]
     L.smg_o_pe_cull_counts.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.smg_o_pe_grad.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp,
@@ -73,6 +76,10 @@ def lib():
     L.smg_o_bitmap_collimator_words.argtypes = [C.c_int, C.c_int]
     L.smg_o_bitmap_crystal.argtypes = [C.c_void_p, _up, C.c_int]
     L.smg_o_bitmap_collimator.argtypes = [C.c_void_p, _up, C.c_int]
+    L.smg_o_crystal_pred.restype = C.c_int
+    L.smg_o_crystal_pred.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.smg_o_coll_pred.restype = C.c_int
+    L.smg_o_coll_pred.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     L.smg_o_bit_crystal.restype = C.c_int
     L.smg_o_bit_crystal.argtypes = [_up, C.c_int, C.c_int, C.c_int, C.c_int]
     L.smg_o_bit_collimator.restype = C.c_int
@@ -175,6 +182,11 @@ class Oracle:
         lib().smg_o_pe_pairs(self._g, rot, *sub, d, v, d.size, out, nthreads)
         return out
 
+    def pe_ops_per_ray(self, d, v, rot=0, sub=(8, 16, 8)) -> float:
+        d = np.ascontiguousarray(d, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.int32)
+        return lib().smg_o_pe_ops_per_ray(self._g, rot, *sub, d, v, d.size)
+
     def cull_counts(self, d, v, rot=0):
         nc, nh = C.c_int(), C.c_int()
         lib().smg_o_pe_cull_counts(self._g, d, v, rot, C.byref(nc), C.byref(nh))
@@ -198,6 +210,12 @@ class Oracle:
         words = np.zeros(L.smg_o_bitmap_collimator_words(self.n_holes_l0, self.n_det), np.uint32)
         L.smg_o_bitmap_collimator(self._g, words, nthreads)
         return words
+
+    def crystal_pred(self, i, j, k) -> int:
+        return lib().smg_o_crystal_pred(self._g, i, j, k)
+
+    def coll_pred(self, h, j, k) -> int:
+        return lib().smg_o_coll_pred(self._g, h, j, k)
 
     def crystal_scatter_pairs(self, pe, bitmap, d, v, rot=0, sub=(1, 1, 1), nthreads=0):
         pe = np.ascontiguousarray(pe, dtype=np.float32).reshape(-1)

@@ -28,6 +28,18 @@
 #define KN_PI ((double)(float)M_PI) /* SC:1169, M_PI passed as float */
 #define FWHM_SIGMA ((double)2.35482f) /* SC:1026, SC:1263 */
 
+/* ---- algorithmic op counter ---------------------------------------------
+ * When enabled (smg_o_pe_ops_per_ray), the element code adds the arithmetic
+ * operations (+ - * / sqrt exp; comparisons excluded) of the reference
+ * formula on the branch it takes.  Loop-invariant trig of the bar rotation
+ * and the culling scans are not counted. */
+static _Thread_local int g_count_ops = 0;
+static _Thread_local double g_ops = 0.0;
+#define OPS(n)                          \
+    do {                                \
+        if (g_count_ops) g_ops += (n);  \
+    } while (0)
+
 struct smg_o_geom {
     const float* coll;
     const float* det;
@@ -134,8 +146,10 @@ double smg_o_box_ray(const double a[3], const double b[3], const double lo[3],
     int small[3];
     for (int k = 0; k < 3; ++k) small[k] = fabs(b[k] - a[k]) < eps;
     int nsmall = small[0] + small[1] + small[2];
+    OPS(3); /* component differences */
     if (nsmall == 3) return 0.0;
     if (nsmall == 2) {
+        OPS(1);
         int k = !small[0] ? 0 : (!small[1] ? 1 : 2); /* the moving axis */
         int i = (k + 1) % 3, j = (k + 2) % 3;
         if (a[i] >= lo[i] && a[i] <= hi[i] && a[j] >= lo[j] && a[j] <= hi[j]) {
@@ -163,6 +177,8 @@ double smg_o_box_ray(const double a[3], const double b[3], const double lo[3],
     double len = sqrt(len2);
     double tmin, tmax;
     slab_interval(a, b, lo, hi, axes, naxes, len, &tmin, &tmax);
+    /* length (naxes mul, naxes-1 add, sqrt), naxes div, 2 naxes (sub, mul), chord */
+    OPS(naxes + (naxes - 1) + 1 + naxes + 4 * naxes + 1);
     if (tmax - tmin < eps) return 0.0;
     if (tmin >= len || tmax >= len) return 0.0;
     if (tmin <= eps || tmax <= eps) return 0.0;
@@ -176,6 +192,7 @@ double smg_o_box_ray(const double a[3], const double b[3], const double lo[3],
 double smg_o_box_ray_inside(const double a[3], const double b[3], const double lo[3],
                             const double hi[3]) {
     const double eps = SLAB_EPS;
+    OPS(3);
     if (fabs(b[1] - a[1]) < INSIDE_DY) return 0.0;
     int small[3];
     for (int k = 0; k < 3; ++k) small[k] = fabs(b[k] - a[k]) < eps;
@@ -213,6 +230,7 @@ double smg_o_box_ray_inside(const double a[3], const double b[3], const double l
     double len = sqrt(len2);
     double tmin, tmax;
     slab_interval(a, b, lo, hi, axes, naxes, len, &tmin, &tmax);
+    OPS(naxes + (naxes - 1) + 1 + naxes + 4 * naxes + 1);
     if (tmax - tmin < eps) return 0.0;
     if (tmin >= len) return 0.0;
     if (tmax <= eps) return 0.0;
@@ -229,7 +247,10 @@ double smg_o_box_ray_inside(const double a[3], const double b[3], const double l
  * infinite cylinder, cut by whichever plane point is inside. */
 double smg_o_cylinder_ray(const double a[3], const double b[3], double xc, double y1,
                           double y2, double zc, double radius) {
+    OPS(1);
     if (fabs(y1 - y2) < CYL_DY) return 0.0;
+    /* length and unit direction 12, plane points 16, slab 4, circle tests 12 */
+    OPS(44);
     double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
     double len = sqrt(dx * dx + dy * dy + dz * dz);
     double kx = dx / len, ky = dy / len, kz = dz / len;
@@ -243,15 +264,18 @@ double smg_o_cylinder_ray(const double a[3], const double b[3], double xc, doubl
     int in_l = (xl - xc) * (xl - xc) + (zl - zc) * (zl - zc) <= r2;
     int in_r = (xr - xc) * (xr - xc) + (zr - zc) * (zr - zc) <= r2;
     if (in_l && in_r) {
+        OPS(1);
         if (tmin <= CYL_TEPS || tmax <= CYL_TEPS) return 0.0;
         if (tmin >= len || tmax >= len) return 0.0;
         return fabs(tmax - tmin);
     }
+    OPS(12); /* offsets and discriminant */
     double x0 = a[0] - xc, z0 = a[2] - zc;
     double qa = kx * kx + kz * kz;
     double cr = kx * z0 - kz * x0;
     double disc = qa * r2 - cr * cr;
     if (disc <= CYL_DEPS) return 0.0;
+    OPS(20); /* sqrt, two roots, chord */
     double sq = sqrt(disc);
     double qb = kx * x0 + kz * z0;
     double t1 = (-qb - sq) / qa;
@@ -509,6 +533,8 @@ static double pe_element_core(const smg_o_geom* g, int d, int v, int rot, int sx
                 double qs[3], clo[3], chi[3], q[3];
                 cell_of(&bd, a, bb, c, sx, sy, sz, qs, clo, chi);
                 from_bar_frame(&bd, qs, q);
+                /* sub-cell point 24, distance 10, solid angle 32 */
+                OPS(66);
                 double D[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
                 double dsq = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
                 double dd = sqrt(dsq);
@@ -525,6 +551,7 @@ static double pe_element_core(const smg_o_geom* g, int d, int v, int rot, int sx
                     const float* L = g->coll + (m + 1) * 10;
                     double lo[3] = {-(double)L[1] / 2.0, -(double)L[2] / 2.0 + g->F + L[4], -(double)L[3] / 2.0};
                     double hi[3] = {(double)L[1] / 2.0, (double)L[2] / 2.0 + g->F + L[4], (double)L[3] / 2.0};
+                    OPS(10); /* plate box */
                     if (smg_o_box_ray(p, q, lo, hi) < (double)0.1f) att += 100000.0;
                 }
                 /* holes and plate material (PE:1257-1286) */
@@ -532,8 +559,10 @@ static double pe_element_core(const smg_o_geom* g, int d, int v, int rot, int sx
                     const float* L = g->coll + (m + 1) * 10;
                     double in_coll = (double)L[2] / cosang;
                     double sum = 0.0;
+                    OPS(5); /* plate path and its attenuation */
                     for (int i = 0; i < cl.n_holes[m]; ++i) {
                         const float* hp = g->coll + 100 + 9 * cl.holes[m][i];
+                        OPS(7);
                         double l = smg_o_cylinder_ray(p, q, hp[0], (double)hp[1] + g->F,
                                                       (double)hp[2] + g->F, hp[3], hp[4]);
                         sum += l;
@@ -553,11 +582,13 @@ static double pe_element_core(const smg_o_geom* g, int d, int v, int rot, int sx
                     to_bar_frame(&bk, q, qa);
                     double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
                     double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
+                    OPS(31); /* two frame transforms, box, attenuation */
                     double l = smg_o_box_ray(pa, qa, lo, hi);
                     bar_len[i] = l;
                     att += l * bk.mut;
                 }
                 /* own bar up to the sub-cell (PE:1329-1346) */
+                OPS(21 + 2 + 20 + 3); /* cell box, exp, extension, absorption, sum */
                 double l1 = smg_o_box_ray_inside(ps, qs, blo, bhi);
                 double l2 = smg_o_box_ray_inside(ps, qs, clo, chi);
                 att += (l1 - l2) * bd.mut;
@@ -588,6 +619,20 @@ static double pe_element_core(const smg_o_geom* g, int d, int v, int rot, int sx
 
 double smg_o_pe_element(const smg_o_geom* g, int d, int v, int rot, int sx, int sy, int sz) {
     return pe_element_core(g, d, v, rot, sx, sy, sz, 0.0, NULL);
+}
+
+double smg_o_pe_ops_per_ray(const smg_o_geom* g, int rot, int sx, int sy, int sz, const int* d,
+                            const int* v, int n) {
+    g_count_ops = 1;
+    g_ops = 0.0;
+    double rays = 0.0;
+    for (int i = 0; i < n; ++i) {
+        if (!((double)g->det[12 * d[i] + 7] > (double)0.01f)) continue;
+        pe_element_core(g, d[i], v[i], rot, sx, sy, sz, 0.0, NULL);
+        rays += (double)sx * sy * sz;
+    }
+    g_count_ops = 0;
+    return rays > 0 ? g_ops / rays : 0.0;
 }
 
 void smg_o_pe_rows(const smg_o_geom* g, int rot, int sx, int sy, int sz, const int* rows,
@@ -684,45 +729,55 @@ static void bar_center_raw(const smg_o_geom* g, int k, double c[3], double* r) {
 /* Bar k lies on the line between bars i < j if its bounding sphere, seen from
  * i, overlaps the direction to j by the angular test, or the same seen from j
  * (SC:1440-1540).  Positions omit F (a translation that cancels). */
+int smg_o_crystal_pred(const smg_o_geom* g, int i, int j, int k) {
+    double ci[3], ri, cj[3], rj, ck[3], rk;
+    bar_center_raw(g, i, ci, &ri);
+    bar_center_raw(g, j, cj, &rj);
+    bar_center_raw(g, k, ck, &rk);
+    double lij = dist3(ci, cj);
+    double crit_i_j = ri / lij, crit_j_i = rj / lij;
+    double uij[3] = {(ci[0] - cj[0]) / lij, (ci[1] - cj[1]) / lij, (ci[2] - cj[2]) / lij};
+    double uji[3] = {(cj[0] - ci[0]) / lij, (cj[1] - ci[1]) / lij, (cj[2] - ci[2]) / lij};
+    double lik = dist3(ck, ci), ljk = dist3(ck, cj);
+    double uki[3] = {(ck[0] - ci[0]) / lik, (ck[1] - ci[1]) / lik, (ck[2] - ci[2]) / lik};
+    double ukj[3] = {(ck[0] - cj[0]) / ljk, (ck[1] - cj[1]) / ljk, (ck[2] - cj[2]) / ljk};
+    if (dist3(uki, uji) <= rk / lik + crit_j_i) return 1;
+    return dist3(ukj, uij) <= rk / ljk + crit_i_j;
+}
+
 void smg_o_bitmap_crystal(const smg_o_geom* g, uint32_t* words, int nthreads) {
     const int N = g->n_det;
     int64_t nw = smg_o_bitmap_crystal_words(N);
     memset(words, 0, sizeof(uint32_t) * (size_t)nw);
     if (nthreads <= 0) nthreads = smg_o_max_threads();
-    /* each (i, j) pair owns N consecutive bits; pairs are written by one thread
-     * each, but neighbouring pairs share a word, so bits are OR-ed atomically */
+    /* neighbouring pairs share a word, so bits are OR-ed atomically */
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads)
-    for (int i = 0; i < N; ++i) {
-        double ci[3], ri;
-        bar_center_raw(g, i, ci, &ri);
-        for (int j = i + 1; j < N; ++j) {
-            double cj[3], rj;
-            bar_center_raw(g, j, cj, &rj);
-            double lij = dist3(ci, cj);
-            double crit_i_j = ri / lij, crit_j_i = rj / lij;
-            double uij[3] = {(ci[0] - cj[0]) / lij, (ci[1] - cj[1]) / lij, (ci[2] - cj[2]) / lij};
-            double uji[3] = {(cj[0] - ci[0]) / lij, (cj[1] - ci[1]) / lij, (cj[2] - ci[2]) / lij};
+    for (int i = 0; i < N; ++i)
+        for (int j = i + 1; j < N; ++j)
             for (int k = 0; k < N; ++k) {
-                if (k == i || k == j) continue;
-                double ck[3], rk;
-                bar_center_raw(g, k, ck, &rk);
-                double lik = dist3(ck, ci), ljk = dist3(ck, cj);
-                double uki[3] = {(ck[0] - ci[0]) / lik, (ck[1] - ci[1]) / lik, (ck[2] - ci[2]) / lik};
-                double ukj[3] = {(ck[0] - cj[0]) / ljk, (ck[1] - cj[1]) / ljk, (ck[2] - cj[2]) / ljk};
-                int hit = dist3(uki, uji) <= rk / lik + crit_j_i;
-                if (!hit) hit = dist3(ukj, uij) <= rk / ljk + crit_i_j;
-                if (hit) {
-                    int64_t b = crystal_bit_index(N, i, j, k);
+                if (k == i || k == j || !smg_o_crystal_pred(g, i, j, k)) continue;
+                int64_t b = crystal_bit_index(N, i, j, k);
 #pragma omp atomic
-                    words[b / 32] |= (1u << (b % 32));
-                }
+                words[b / 32] |= (1u << (b % 32));
             }
-        }
-    }
 }
 
 /* Bar k lies on the ray from layer-0 hole i toward bar j (SC:1939-2009):
  * angular test at the hole centre (x, (y1+y2)/2, z), F omitted. */
+int smg_o_coll_pred(const smg_o_geom* g, int i, int j, int k) {
+    const float* hp = g->coll + 100 + 9 * i;
+    double ci[3] = {hp[0], ((double)hp[1] + (double)hp[2]) / 2.0, hp[3]};
+    double cj[3], rj, ck[3], rk;
+    bar_center_raw(g, j, cj, &rj);
+    bar_center_raw(g, k, ck, &rk);
+    double lij = dist3(ci, cj);
+    double crit_j_i = rj / lij;
+    double uji[3] = {(cj[0] - ci[0]) / lij, (cj[1] - ci[1]) / lij, (cj[2] - ci[2]) / lij};
+    double lik = dist3(ck, ci);
+    double uki[3] = {(ck[0] - ci[0]) / lik, (ck[1] - ci[1]) / lik, (ck[2] - ci[2]) / lik};
+    return dist3(uki, uji) <= rk / lik + crit_j_i;
+}
+
 void smg_o_bitmap_collimator(const smg_o_geom* g, uint32_t* words, int nthreads) {
     const int N = g->n_det, H = g->n_holes_l0;
     int64_t nw = smg_o_bitmap_collimator_words(H, N);
@@ -730,26 +785,13 @@ void smg_o_bitmap_collimator(const smg_o_geom* g, uint32_t* words, int nthreads)
     if (nthreads <= 0) nthreads = smg_o_max_threads();
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads) collapse(2)
     for (int i = 0; i < H; ++i)
-        for (int j = 0; j < N; ++j) {
-            const float* hp = g->coll + 100 + 9 * i;
-            double ci[3] = {hp[0], ((double)hp[1] + (double)hp[2]) / 2.0, hp[3]};
-            double cj[3], rj;
-            bar_center_raw(g, j, cj, &rj);
-            double lij = dist3(ci, cj);
-            double crit_j_i = rj / lij;
-            double uji[3] = {(cj[0] - ci[0]) / lij, (cj[1] - ci[1]) / lij, (cj[2] - ci[2]) / lij};
+        for (int j = 0; j < N; ++j)
             for (int k = 0; k < N; ++k) {
-                double ck[3], rk;
-                bar_center_raw(g, k, ck, &rk);
-                double lik = dist3(ck, ci);
-                double uki[3] = {(ck[0] - ci[0]) / lik, (ck[1] - ci[1]) / lik, (ck[2] - ci[2]) / lik};
-                if (dist3(uki, uji) <= rk / lik + crit_j_i) {
-                    int64_t b = (int64_t)i * N * N + (int64_t)j * N + k;
+                if (!smg_o_coll_pred(g, i, j, k)) continue;
+                int64_t b = (int64_t)i * N * N + (int64_t)j * N + k;
 #pragma omp atomic
-                    words[b / 32] |= (1u << (b % 32));
-                }
+                words[b / 32] |= (1u << (b % 32));
             }
-        }
 }
 
 /* ======================================================================== */

@@ -63,6 +63,12 @@ void smg_o_pe_pairs(const smg_o_geom* g, int rot, int sx, int sy, int sz, const 
 void smg_o_pe_cull_counts(const smg_o_geom* g, int d, int v, int rot, int* n_cryst,
                           int* n_holes);
 
+/* Algorithmic arithmetic operations per sub-cell ray of the reference PE
+ * formula (branches taken; culling scans and loop-invariant trig excluded),
+ * averaged over the listed elements. */
+double smg_o_pe_ops_per_ray(const smg_o_geom* g, int rot, int sx, int sy, int sz, const int* d,
+                            const int* v, int n);
+
 /* dL/dmu for L = sum_dv w[d][v] A[d][v]; grad has Lc + 2N entries. */
 void smg_o_pe_grad(const smg_o_geom* g, int rot, int sx, int sy, int sz, const double* w,
                    double* grad, double* loss, int nthreads);
@@ -72,6 +78,9 @@ int64_t smg_o_bitmap_crystal_words(int n_det);
 int64_t smg_o_bitmap_collimator_words(int n_holes_l0, int n_det);
 void smg_o_bitmap_crystal(const smg_o_geom* g, uint32_t* words, int nthreads);
 void smg_o_bitmap_collimator(const smg_o_geom* g, uint32_t* words, int nthreads);
+/* The predicates behind single bits (i < j for the crystal bitmap). */
+int smg_o_crystal_pred(const smg_o_geom* g, int i, int j, int k);
+int smg_o_coll_pred(const smg_o_geom* g, int i, int j, int k);
 int smg_o_bit_crystal(const uint32_t* words, int n_det, int i, int j, int k);
 int smg_o_bit_collimator(const uint32_t* words, int n_det, int i, int j, int k);
 

@@ -1,0 +1,98 @@
+"""Shared test setup.
+
+`-m "not gpu"` tests run anywhere (oracle against closed forms and golden
+vectors, host logic, the C ABI's exports).  `-m gpu` tests call the HIP
+kernels through the C ABI on an MI355X and compare with the FP64 oracle.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "gpu-based-system-matrix-calculation-for-spect-pet_amd")
+ORACLE = os.path.join(ROOT, "oracle")
+for path in (PKG, ORACLE):
+    if path not in sys.path:
+        sys.path.insert(0, path)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP kernels through the C ABI)")
+
+
+# Relative-error bar of the FP32 path against the FP64 oracle (BASELINE
+# metric: <= 1e-5 max relative error), and of the FP64 path (output rounded
+# to float32: half an ulp plus the oracle's own rounding).
+TOL_FP32 = 1e-5
+TOL_FP64 = 2e-7
+TINY = 1e-30
+
+
+def rel_errors(got, ref):
+    """Max and median relative error over |ref| >= 1e-30; elements below must
+    match to 1e-30 absolute."""
+    got = np.asarray(got, np.float64).ravel()
+    ref = np.asarray(ref, np.float64).ravel()
+    big = np.abs(ref) >= TINY
+    small_ok = bool(np.all(np.abs(got[~big] - ref[~big]) <= TINY))
+    if not big.any():
+        return 0.0, 0.0, small_ok
+    rel = np.abs(got[big] - ref[big]) / np.abs(ref[big])
+    return float(rel.max()), float(np.median(rel)), small_ok
+
+
+def assert_parity(got, ref, tol, what=""):
+    mx, med, small_ok = rel_errors(got, ref)
+    print(f"{what}: max rel {mx:.3e} median {med:.3e}", flush=True)
+    assert small_ok, f"{what}: elements with |ref| < 1e-30 differ"
+    assert mx <= tol, f"{what}: max relative error {mx:.3e} > {tol:.1e} (median {med:.3e})"
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    import oracle
+    oracle.lib()
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def params_mod():
+    from smg import params
+    return params
+
+
+@pytest.fixture(scope="session")
+def lane_lib():
+    """Host build of the GPU lane arithmetic (tests/lane_harness.cpp)."""
+    import ctypes as C
+    out_dir = os.path.join(ROOT, "tests", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    so = os.path.join(out_dir, "liblane.so")
+    src = os.path.join(ROOT, "tests", "lane_harness.cpp")
+    hdrs = [os.path.join(PKG, "csrc", f) for f in os.listdir(os.path.join(PKG, "csrc"))
+            if f.endswith(".hpp")]
+    newest = max(os.path.getmtime(p) for p in hdrs + [src])
+    if not os.path.exists(so) or os.path.getmtime(so) < newest:
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wno-unknown-pragmas",
+                        "-I", os.path.join(PKG, "csrc"), "-o", so, src], check=True)
+    L = C.CDLL(so)
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C")
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    up = np.ctypeslib.ndpointer(np.uint32, flags="C")
+    L.lane_pe_pairs.restype = C.c_long
+    L.lane_pe_pairs.argtypes = [fp, fp, fp, fp] + [C.c_int] * 5 + [ip, ip, C.c_int, dp]
+    L.lane_scatter_pairs.restype = C.c_long
+    L.lane_scatter_pairs.argtypes = [fp, fp, fp, fp, fp, up] + [C.c_int] * 7 + [ip, ip, C.c_int, dp]
+    return L
+
+
+@pytest.fixture(scope="session")
+def smg_mod():
+    """The product library; gpu tests fail loudly if it is not built."""
+    import smg
+    smg.lib()
+    return smg

@@ -12,6 +12,7 @@
 
 #include "smg_parse.hpp"
 #include "smg_pe_body.hpp"
+#include "smg_scatter_body.hpp"
 
 using namespace smg;
 
@@ -101,6 +102,186 @@ long lane_pe_pairs(const float* coll, const float* det, const float* img, const 
     for (int i = 0; i < n; ++i)
         out[i] = exact ? element<true>(c, d[i], v[i], rot, sx, sy, sz, &fb)
                        : element<false>(c, d[i], v[i], rot, sx, sy, sz, &fb);
+    return fb;
+}
+
+}  // extern "C"
+
+namespace {
+
+bool bit_set(const uint32_t* words, int64_t b) { return (words[b >> 5] >> (b & 31)) & 1u; }
+
+// Pair terms of one (d, scatter point, sub-cell), as the pair-terms kernel forms them.
+PairTerm pair_term(const Ctx& c, const uint32_t* bitmap, int kind, int d, int s, const double qs[3],
+                   const CellCtx& cc) {
+    const PeGeom& G = c.G;
+    const int N = G.g.n_det;
+    const Bar<double>& bd = G.bars_d[d];
+    PairTerm out{0, 0, 0, 0};
+    bool near = false;
+    int64_t base;
+    if (kind == 0) {
+        int64_t i = s < d ? s : d, j = s < d ? d : s;
+        base = (i * N - i * (i + 1) / 2 + j) * N;
+    } else {
+        base = ((int64_t)s * N + d) * N;
+    }
+    for (int k = 0; k < N; ++k) {
+        if (k == d || (kind == 0 && k == s) || !bit_set(bitmap, base + k)) continue;
+        const Bar<double>& bk = G.bars_d[k];
+        double pa[3], pb[3];
+        to_frame(bk, qs, pa);
+        to_frame(bk, cc.q, pb);
+        double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
+        double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
+        SegDir<double> sd;
+        segdir_init<double>(sd, pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2], 0.0, near);
+        double l = seg_box<double>(sd, pa, lo, hi, near);
+        out.s_pe += l * bk.mupe;
+        out.s_c += l * bk.muc;
+    }
+    double qss[3];
+    to_frame(bd, qs, qss);
+    double blo[3] = {-bd.w / 2.0, -bd.t / 2.0, -bd.h / 2.0};
+    double bhi[3] = {bd.w / 2.0, bd.t / 2.0, bd.h / 2.0};
+    SegDir<double> so;
+    segdir_init<double>(so, cc.qs[0] - qss[0], cc.qs[1] - qss[1], cc.qs[2] - qss[2], 0.0, near);
+    double l1 = seg_box_inside<double>(so, qss, blo, bhi, near);
+    double l2 = seg_box_inside<double>(so, qss, cc.lo, cc.hi, near);
+    out.s_pe += (l1 - l2) * bd.mupe;
+    out.s_c += (l1 - l2) * bd.muc;
+    double dx = qs[0] - cc.q[0], dy = qs[1] - cc.q[1], dz = qs[2] - cc.q[2];
+    out.len = std::sqrt(dx * dx + dy * dy + dz * dz);
+    double De[3];
+    for (int k = 0; k < 3; ++k) De[k] = (cc.qs[k] + 1000.0 * (cc.qs[k] - qss[k]) / out.len) - qss[k];
+    SegDir<double> se;
+    segdir_init<double>(se, De[0], De[1], De[2], 0.0, near);
+    out.la = seg_box<double>(se, qss, cc.lo, cc.hi, near);
+    return out;
+}
+
+double kn_norm(double e0) {
+    const double step = (double)0.01f, hi = (double)(float)3.14159265358979323846;
+    int n = (int)(hi / step);
+    double s = 0.0, alpha = e0 / 511.0;
+    for (int i = 0; i <= n; ++i) {
+        double th = i * step;
+        s += kn_of_cos(std::cos(th), alpha) * std::sin(th) * step;
+    }
+    return s;
+}
+
+template <class T>
+double scatter_element(const Ctx& c, const float* pe, const uint32_t* bitmap, int kind, int variant,
+                       int d, int v, int rot, int sx, int sy, int sz, long* fallbacks) {
+    const PeGeom& G = c.G;
+    const GeomDesc& g = G.g;
+    const HostGeom& hg = c.hg;
+    const Bar<double>& bd = G.bars_d[d];
+    const float* ph = hg.phys.data();
+    if ((int)std::floor((double)ph[0] + 0.5) != 1 || !(bd.mut > (double)0.01f)) return 0.0;
+    int nxy = g.nx * g.ny;
+    int iz = v / nxy, rem = v % nxy;
+    double phi = (double)rot * g.dphi;
+    double p[3], ps[3];
+    voxel_position(g, rem % g.nx, rem / g.nx, iz, std::cos(phi), std::sin(phi), p);
+    to_frame(bd, p, ps);
+    ScatterPhys P;
+    P.e0 = ph[7];
+    if ((int)std::floor((double)ph[4] + 0.5) > 0) {
+        P.lo = ph[5];
+        P.hi = ph[6];
+    } else {
+        P.lo = (1 - bd.res / 2.0) * P.e0;
+        P.hi = (1 + bd.res / 2.0) * P.e0;
+    }
+    P.norm = kn_norm(P.e0);
+    P.res = bd.res;
+    P.mupe = bd.mupe;
+    P.muc = bd.muc;
+    P.rcell = std::sqrt(bd.w * bd.w / sx / sx + bd.h * bd.h / sz / sz + bd.t * bd.t / sy / sy) / 2.0;
+    int n_pts = kind == 0 ? g.n_det : hg.n_holes_l0;
+    KahanSum<T> acc;
+    for (int cell = 0; cell < sx * sy * sz; ++cell) {
+        int a = cell % sx, b = (cell / sx) % sy, cz = cell / (sx * sy);
+        CellCtx cc;
+        make_cell_ctx(bd, a, b, cz, sx, sy, sz, cc);
+        ScatterCell sc;
+        for (int k = 0; k < 3; ++k) {
+            sc.q[k] = cc.q[k];
+            sc.lo[k] = cc.lo[k];
+            sc.hi[k] = cc.hi[k];
+        }
+        for (int corner = 0; corner < 8; ++corner) {
+            sc.corner[corner][0] = (corner & 1) ? cc.hi[0] : cc.lo[0];
+            sc.corner[corner][1] = (corner & 2) ? cc.hi[1] : cc.lo[1];
+            sc.corner[corner][2] = (corner & 4) ? cc.hi[2] : cc.lo[2];
+        }
+        for (int s = 0; s < n_pts; ++s) {
+            if (kind == 0 && s == d) continue;
+            double dyh = 0.0;
+            if (kind == 0) {
+                const Bar<double>& bs = G.bars_d[s];
+                sc.qs[0] = bs.cx;
+                sc.qs[1] = bs.cy;
+                sc.qs[2] = bs.cz;
+            } else {
+                const float* hp = hg.coll.data() + 100 + 9 * s;
+                sc.qs[0] = hp[0];
+                sc.qs[1] = ((double)hp[1] + (double)hp[2]) / 2.0 + g.fov2coll;
+                sc.qs[2] = hp[3];
+                dyh = (double)hp[2] - (double)hp[1];
+            }
+            to_frame(bd, sc.qs, sc.qss);
+            sc.pt = pair_term(c, bitmap, kind, d, s, sc.qs, cc);
+            T wgt;
+            if (kind == 0) {
+                wgt = T(pe[(int64_t)s * g.n_vox + v]) * T(bd.muc / bd.mupe);
+            } else {
+                double dx = p[0] - sc.qs[0], dy = p[1] - sc.qs[1], dz = p[2] - sc.qs[2];
+                double lv = std::sqrt(dx * dx + dy * dy + dz * dz);
+                double om;
+                if (variant == 0) {
+                    double r = (double)0.1f;
+                    om = (r * r * 3.14159265358979323846 + dyh * r) /
+                         (4.0 * 3.14159265358979323846 * lv * lv) * std::fabs(sc.qs[1] - p[1]) / lv;
+                } else {
+                    double r = (double)0.2f;
+                    om = (r * r + dyh * r) / (4.0 * lv * lv);
+                }
+                wgt = T(om * ((double)hg.coll[17] / (double)hg.coll[15]));
+            }
+            if (wgt == T(0)) continue;
+            bool near = false;
+            T val = scatter_term<T>(P, p, ps, sc, wgt, near);
+            if (std::is_same<T, float>::value && near) {
+                bool n2 = false;
+                val = T(scatter_term<double>(P, p, ps, sc, double(wgt), n2));
+                ++*fallbacks;
+            }
+            acc.add(val);
+        }
+    }
+    return (double)(float)acc.value();
+}
+
+}  // namespace
+
+extern "C" {
+
+// kind 0: crystal scatter (pe = N x V block), 1: collimator scatter (variant 0/1).
+long lane_scatter_pairs(const float* coll, const float* det, const float* img, const float* phys,
+                        const float* pe, const uint32_t* bitmap, int kind, int variant, int rot,
+                        int sx, int sy, int sz, int exact, const int* d, const int* v, int n,
+                        double* out) {
+    Ctx c;
+    if (!make_ctx(coll, det, img, phys, c)) return -1;
+    long fb = 0;
+    for (int i = 0; i < n; ++i)
+        out[i] = exact ? scatter_element<double>(c, pe, bitmap, kind, variant, d[i], v[i], rot, sx,
+                                                 sy, sz, &fb)
+                       : scatter_element<float>(c, pe, bitmap, kind, variant, d[i], v[i], rot, sx,
+                                                sy, sz, &fb);
     return fb;
 }
 

@@ -1,0 +1,133 @@
+"""GPU parity of the geometry bitmaps and the primary-Compton scatter
+matrices against the FP64 oracle.  Mirrors the three scatter() launch
+wrappers (ScatterGen_RayTracing_CircularHole/scatter.cu:2013-2378,
+ScatterGen_Collimator/scatter.cu:1511-1747, ScatterGen_Crystal/scatter.cu:1528-1738)."""
+import numpy as np
+import pytest
+
+from conftest import TOL_FP32, TOL_FP64, assert_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def test_bitmaps_cfg1_bit_exact(smg_mod, oracle_mod, params_mod):
+    p = params_mod.config(1)
+    o = oracle_mod.Oracle(p)
+    with smg_mod.open_context(p) as ctx:
+        bc = ctx.bitmap_crystal()
+        bl = ctx.bitmap_collimator()
+    assert np.array_equal(bc, o.bitmap_crystal())
+    assert np.array_equal(bl, o.bitmap_collimator())
+
+
+def test_bitmaps_ragged_bit_exact(smg_mod, oracle_mod, params_mod):
+    """N = 37 bars (words straddle pair rows), 5 layer-0 holes."""
+    p = params_mod.config(1)
+    p.det[0] = 37
+    p.coll[10] = 5
+    o = oracle_mod.Oracle(p)
+    with smg_mod.open_context(p) as ctx:
+        assert np.array_equal(ctx.bitmap_crystal(), o.bitmap_crystal())
+        assert np.array_equal(ctx.bitmap_collimator(), o.bitmap_collimator())
+
+
+def test_bitmaps_cfg2_sampled_bits(smg_mod, oracle_mod, params_mod):
+    """2048 bars, 400 holes: 537 MB and 210 MB bitmaps on the GPU, a seeded
+    sample of bits against the oracle's predicate."""
+    p = params_mod.config(2)
+    o = oracle_mod.Oracle(p)
+    N = o.n_det
+    rng = np.random.default_rng(4)
+    with smg_mod.open_context(p) as ctx:
+        bc = ctx.bitmap_crystal()
+        bl = ctx.bitmap_collimator()
+    # random triples plus triples near each other (where bits are set)
+    i = rng.integers(0, N, 4000)
+    j = np.clip(i + rng.integers(1, 200, 4000), 0, N - 1)
+    k = np.clip(i + rng.integers(-100, 300, 4000), 0, N - 1)
+    keep = (i < j) & (k != i) & (k != j)
+    n_set = 0
+    for a, b, c in zip(i[keep], j[keep], k[keep]):
+        pair = a * N - a * (a + 1) // 2 + b
+        bit = pair * N + c
+        got = (int(bc[bit // 32]) >> (bit % 32)) & 1
+        assert got == o.crystal_pred(int(a), int(b), int(c)), (a, b, c)
+        n_set += got
+    assert n_set > 10
+    h = rng.integers(0, o.n_holes_l0, 3000)
+    j = rng.integers(0, N, 3000)
+    k = np.clip(j + rng.integers(-80, 80, 3000), 0, N - 1)
+    for a, b, c in zip(h, j, k):
+        bit = (int(a) * N + int(b)) * N + int(c)
+        got = (int(bl[bit // 32]) >> (bit % 32)) & 1
+        assert got == o.coll_pred(int(a), int(b), int(c)), (a, b, c)
+
+
+def _sample(o, n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, o.n_det, n), rng.integers(0, o.n_vox, n)
+
+
+@pytest.mark.parametrize("variant", ["combined", "standalone"])
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
+def test_collimator_scatter_cfg3(smg_mod, oracle_mod, params_mod, variant, prec, tol):
+    """Config 3 geometry (32x32x16 image, 64 bars, 16 layer-0 holes)."""
+    p = params_mod.config(3)
+    o = oracle_mod.Oracle(p)
+    d, v = _sample(o, 3000, 5)
+    with smg_mod.open_context(p) as ctx:
+        got, st = ctx.scatter_collimator(0, variant, prec)
+        bm = ctx.bitmap_collimator()
+    ref = o.collimator_scatter_pairs(bm, d, v, variant=0 if variant == "combined" else 1)
+    assert (ref > 0).mean() > 0.5
+    assert_parity(got[d, v], ref, tol, f"collimator scatter {variant} {prec}")
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
+def test_crystal_scatter_cfg3(smg_mod, oracle_mod, params_mod, prec, tol):
+    """Crystal scatter driven by the GPU PE block of the same geometry."""
+    p = params_mod.config(3)
+    o = oracle_mod.Oracle(p)
+    d, v = _sample(o, 3000, 6)
+    with smg_mod.open_context(p) as ctx:
+        ctx.set_pe_subsamples(4, 4, 4)
+        pe, _ = ctx.pe_forward(0, "fp32")
+        got, st = ctx.scatter_crystal(pe, 0, prec)
+        bm = ctx.bitmap_crystal()
+    ref = o.crystal_scatter_pairs(pe, bm, d, v)
+    assert (ref > 0).mean() > 0.05
+    assert_parity(got[d, v], ref, tol, f"crystal scatter {prec}")
+
+
+def test_crystal_scatter_subcells_and_loaded_bitmap(smg_mod, oracle_mod, params_mod):
+    """2x2x2 detection sub-cells, a bitmap installed from a cache instead of
+    computed, and a random PE block (the scatter is linear in it)."""
+    p = params_mod.config(1)
+    o = oracle_mod.Oracle(p)
+    rng = np.random.default_rng(8)
+    pe = rng.uniform(0, 1e-9, (o.n_det, o.n_vox)).astype(np.float32)
+    bm = o.bitmap_crystal()
+    d, v = _sample(o, 600, 9)
+    with smg_mod.open_context(p) as ctx:
+        ctx.set_scatter_subsamples(2, 2, 2)
+        ctx.load_bitmap_crystal(bm)
+        got, _ = ctx.scatter_crystal(pe, 0, "fp32")
+    ref = o.crystal_scatter_pairs(pe, bm, d, v, sub=(2, 2, 2))
+    assert_parity(got[d, v], ref, TOL_FP32, "crystal scatter 2x2x2")
+
+
+def test_scatter_deterministic(smg_mod, params_mod):
+    p = params_mod.config(1)
+    with smg_mod.open_context(p) as ctx:
+        a, _ = ctx.scatter_collimator(0, "combined", "fp32")
+        b, _ = ctx.scatter_collimator(0, "combined", "fp32")
+    assert np.array_equal(a, b)
+
+
+def test_compton_off_gives_zero(smg_mod, params_mod):
+    """Phys[0] != 1 disables the scatter kernels (scatter.cu:1226)."""
+    p = params_mod.config(1)
+    p.phys[0] = 0
+    with smg_mod.open_context(p) as ctx:
+        a, _ = ctx.scatter_collimator(0, "combined", "fp32")
+    assert np.all(a == 0)

@@ -1,0 +1,50 @@
+"""The GPU lane arithmetic built for the host (tests/lane_harness.cpp) against
+the FP64 oracle: the FP32 path with its FP64 re-computation of flagged rays
+must stay within 1e-5, the FP64 path within float output rounding."""
+import numpy as np
+import pytest
+
+from conftest import TOL_FP32, TOL_FP64, assert_parity
+
+
+def lane_pe(L, p, d, v, sub, exact, rot=0):
+    out = np.zeros(len(d))
+    fb = L.lane_pe_pairs(p.coll, p.det, p.img, p.phys, rot, *sub, exact,
+                         np.ascontiguousarray(d, np.int32), np.ascontiguousarray(v, np.int32),
+                         len(d), out)
+    assert fb >= 0
+    return out, fb
+
+
+@pytest.mark.parametrize("cfg,n", [(1, 3000), (2, 800)])
+@pytest.mark.parametrize("exact", [0, 1])
+def test_lane_pe(lane_lib, oracle_mod, params_mod, cfg, n, exact):
+    p = params_mod.config(cfg)
+    o = oracle_mod.Oracle(p)
+    sub = p.meta["subsamples"]
+    rng = np.random.default_rng(20 + cfg)
+    d = rng.integers(0, o.n_det, n)
+    v = rng.integers(0, o.n_vox, n)
+    got, fb = lane_pe(lane_lib, p, d, v, sub, exact)
+    assert_parity(got, o.pe_pairs(d, v, sub=sub), TOL_FP64 if exact else TOL_FP32,
+                  f"lane PE cfg{cfg} exact={exact} fallbacks={fb}")
+
+
+@pytest.mark.parametrize("kind,variant", [(0, 0), (1, 0), (1, 1)])
+@pytest.mark.parametrize("exact", [0, 1])
+def test_lane_scatter(lane_lib, oracle_mod, params_mod, kind, variant, exact):
+    p = params_mod.config(3)
+    o = oracle_mod.Oracle(p)
+    rng = np.random.default_rng(30 + kind)
+    pe = rng.uniform(0, 1e-9, (o.n_det, o.n_vox)).astype(np.float32)
+    bm = o.bitmap_crystal() if kind == 0 else o.bitmap_collimator()
+    d = rng.integers(0, o.n_det, 300).astype(np.int32)
+    v = rng.integers(0, o.n_vox, 300).astype(np.int32)
+    out = np.zeros(300)
+    fb = lane_lib.lane_scatter_pairs(p.coll, p.det, p.img, p.phys, pe, bm, kind, variant, 0,
+                                     1, 1, 1, exact, d, v, 300, out)
+    ref = (o.crystal_scatter_pairs(pe, bm, d, v) if kind == 0
+           else o.collimator_scatter_pairs(bm, d, v, variant=variant))
+    assert (ref > 0).any()
+    assert_parity(out, ref, TOL_FP64 if exact else TOL_FP32,
+                  f"lane scatter kind={kind} variant={variant} exact={exact} fallbacks={fb}")

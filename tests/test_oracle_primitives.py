@@ -1,0 +1,247 @@
+"""Pins the FP64 oracle's geometry and physics primitives against closed
+forms, brute force and library routines (the reference ships no tests or
+golden data, so these are what fixes the oracle).
+
+References restated by the oracle:
+  length_box_ray          PEGen_RayTracing_CircularHole/PESysMatGen.cu:20-312
+  length_box_ray_inside   PESysMatGen.cu:314-686
+  length_cylinder_ray     PESysMatGen.cu:688-799
+  KN / KN sum / cone angle / E' / window   ScatterGen_RayTracing_CircularHole/scatter.cu:961-1038
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import special
+
+EPS = float(np.float32(0.001))
+
+
+def clip_chord(a, b, lo, hi):
+    """Independent chord of segment a->b in a box: Liang-Barsky clipping of
+    the parameter s in [0, 1], returning (chord, s_in, s_out) or None."""
+    a, b, lo, hi = (np.asarray(x, float) for x in (a, b, lo, hi))
+    d = b - a
+    s0, s1 = -np.inf, np.inf
+    for k in range(3):
+        if d[k] == 0.0:
+            if a[k] < lo[k] or a[k] > hi[k]:
+                return None
+            continue
+        t1, t2 = (lo[k] - a[k]) / d[k], (hi[k] - a[k]) / d[k]
+        s0, s1 = max(s0, min(t1, t2)), min(s1, max(t1, t2))
+    if s0 > s1:
+        return None
+    L = np.linalg.norm(d)
+    return (s1 - s0) * L, s0, s1
+
+
+def test_box_ray_generic_random(oracle_mod):
+    """Generic rays that fully cross a box: the chord equals the clipped
+    parameter interval times the segment length."""
+    rng = np.random.default_rng(1)
+    checked = 0
+    for _ in range(3000):
+        lo = rng.uniform(-5, 0, 3)
+        hi = lo + rng.uniform(0.5, 5, 3)
+        c = rng.uniform(lo - 0.5, hi + 0.5)  # aim through (or near) the box
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        a = c - u * rng.uniform(1, 30)
+        b = c + u * rng.uniform(1, 30)
+        if np.min(np.abs(b - a)) < 0.01:
+            continue
+        r = clip_chord(a, b, lo, hi)
+        got = oracle_mod.box_ray(a, b, lo, hi)
+        if r is None or r[1] <= 0.0 or r[2] >= 1.0:  # missed, or an end point inside
+            assert got == 0.0
+            continue
+        chord = r[0]
+        L = np.linalg.norm(b - a)
+        if chord < EPS or r[1] * L <= EPS:
+            continue  # threshold cases are pinned separately below
+        assert got == pytest.approx(chord, rel=1e-12, abs=1e-12)
+        checked += 1
+    assert checked > 300
+
+
+def test_box_ray_axis_rays_exact_extent(oracle_mod):
+    """A segment along one axis (other components below eps) through the
+    whole box gives exactly the box extent (PESysMatGen.cu:35-66)."""
+    lo, hi = [-1.0, -2.0, -0.5], [1.0, 3.0, 0.5]
+    assert oracle_mod.box_ray([0.2, -9, 0.1], [0.2, 9, 0.1], lo, hi) == 5.0
+    assert oracle_mod.box_ray([-9, 0.0, 0.2], [9, 0.0005, 0.2], lo, hi) == 2.0
+    assert oracle_mod.box_ray([0.0, 0.0, 7], [0.0, 0.0, -7], lo, hi) == 1.0
+    # the segment must run past both faces
+    assert oracle_mod.box_ray([0.2, -9, 0.1], [0.2, 1.0, 0.1], lo, hi) == 0.0
+    # outside the cross-section
+    assert oracle_mod.box_ray([1.5, -9, 0.1], [1.5, 9, 0.1], lo, hi) == 0.0
+
+
+def test_box_ray_one_flat_component(oracle_mod):
+    """One component below eps: the 2-D chord, and 0 unless the start lies
+    strictly between that axis's faces (PESysMatGen.cu:69-123)."""
+    lo, hi = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]
+    a, b = [0.3, -5.0, -5.0], [0.3, 5.0, 5.0]
+    assert oracle_mod.box_ray(a, b, lo, hi) == pytest.approx(2.0 * math.sqrt(2.0), rel=1e-14)
+    assert oracle_mod.box_ray([1.0, -5, -5], [1.0, 5, 5], lo, hi) == 0.0  # on the face
+    # the tiny x component is ignored: same chord as the flat ray
+    assert oracle_mod.box_ray([0.3, -5, -5], [0.3005, 5, 5], lo, hi) == pytest.approx(
+        2.0 * math.sqrt(2.0), rel=1e-14)
+
+
+def test_box_ray_segment_ends_inside_is_zero(oracle_mod):
+    """Both slab parameters must lie in (eps, |b-a|): starting or ending inside
+    the box gives 0 (PESysMatGen.cu:297-304)."""
+    lo, hi = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]
+    assert oracle_mod.box_ray([0.1, 0.2, 0.3], [5, 7, 9], lo, hi) == 0.0
+    assert oracle_mod.box_ray([-5, -7, -9], [0.1, 0.2, 0.3], lo, hi) == 0.0
+    assert oracle_mod.box_ray([-5, -7, -9], [5, 7, 9], lo, hi) > 0.0
+
+
+def test_box_ray_short_chord_dropped(oracle_mod):
+    """A chord below eps = 0.001 counts as 0 (PESysMatGen.cu:297-298)."""
+    lo, hi = [0.0, 0.0, 0.0], [1.0, 1.0, 1.0]
+    # clip the corner: x + y crossing near (1, 1)
+    a = [-10.0, 1.0 - 0.0004 - 10.0, 0.5]
+    b = [10.0, 1.0 - 0.0004 + 10.0, 0.6]
+    r = clip_chord(a, b, lo, hi)
+    assert r is not None and r[0] < EPS
+    assert oracle_mod.box_ray(a, b, lo, hi) == 0.0
+
+
+def test_box_ray_inside_clips_at_end_point(oracle_mod):
+    """Segment ending inside the box: chord from the entry face to the end
+    point (PESysMatGen.cu:415-434)."""
+    rng = np.random.default_rng(2)
+    lo, hi = np.array([-1.0, -5.0, -1.0]), np.array([1.0, 5.0, 1.0])
+    for _ in range(500):
+        b = rng.uniform(lo * 0.9, hi * 0.9)
+        a = b + rng.uniform(-30, 30, 3)
+        a[1] = b[1] - rng.uniform(20, 40)
+        if np.min(np.abs(b - a)) < 0.01:
+            continue
+        r = clip_chord(a, b, lo, hi)
+        assert r is not None and r[2] == pytest.approx(1.0) or r[2] > 1.0
+        expect = (1.0 - r[1]) * np.linalg.norm(b - a)
+        assert oracle_mod.box_ray_inside(a, b, lo, hi) == pytest.approx(expect, rel=1e-11)
+    # |dy| < 0.001 gives 0 (PESysMatGen.cu:321-324)
+    assert oracle_mod.box_ray_inside([-5, 0.0, 0.2], [0.5, 0.0005, 0.2], lo, hi) == 0.0
+    # along y with x, z inside: end point minus the entry face
+    assert oracle_mod.box_ray_inside([0.2, -20, 0.1], [0.2, 1.5, 0.1], lo, hi) == 6.5
+
+
+def cyl_chord_brute(a, b, xc, y1, y2, zc, r, n=200001):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    s = np.linspace(0.0, 1.0, n)
+    pts = a[None, :] + s[:, None] * (b - a)[None, :]
+    inside = ((pts[:, 0] - xc) ** 2 + (pts[:, 2] - zc) ** 2 <= r * r) & (pts[:, 1] >= y1) & (pts[:, 1] <= y2)
+    return inside.mean() * np.linalg.norm(b - a), np.linalg.norm(b - a) / (n - 1)
+
+
+def test_cylinder_through_both_caps(oracle_mod):
+    """A ray crossing both end discs: chord = (y2 - y1)/|k_y|."""
+    a, b = np.array([0.1, -20.0, -0.2]), np.array([0.5, 40.0, 0.3])
+    k = (b - a) / np.linalg.norm(b - a)
+    got = oracle_mod.cylinder_ray(a, b, 0.3, 10.0, 15.0, 0.05, 0.5)
+    assert got == pytest.approx(5.0 / k[1], rel=1e-13)
+
+
+def test_cylinder_misses_and_tangent(oracle_mod):
+    a, b = [3.0, -20.0, 0.0], [3.0, 40.0, 0.0]
+    assert oracle_mod.cylinder_ray(a, b, 0.0, 10.0, 15.0, 0.0, 0.5) == 0.0
+    # just inside tangency: discriminant below 1e-5 gives 0 (PESysMatGen.cu:745)
+    a, b = [0.5 - 1e-7, -20.0, -5.0], [0.5 - 1e-7, 40.0, 5.0]
+    assert oracle_mod.cylinder_ray(a, b, 0.0, 10.0, 15.0, 0.0, 0.5) == 0.0
+
+
+def test_cylinder_wall_crossings_brute_force(oracle_mod):
+    """Oblique rays entering or leaving through the wall, against dense
+    sampling of the segment."""
+    rng = np.random.default_rng(3)
+    n_wall = 0
+    for _ in range(150):
+        xc, zc, r = rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.3, 1.0)
+        a = np.array([xc + rng.uniform(-3, 3), -10.0, zc + rng.uniform(-3, 3)])
+        b = np.array([xc + rng.uniform(-3, 3), 30.0, zc + rng.uniform(-3, 3)])
+        got = oracle_mod.cylinder_ray(a, b, xc, 8.0, 14.0, zc, r)
+        ref, step = cyl_chord_brute(a, b, xc, 8.0, 14.0, zc, r)
+        assert got == pytest.approx(ref, abs=3 * step + 1e-9)
+        n_wall += 0.0 < got < 6.0 * np.linalg.norm(b - a) / 40.0 - 1e-6
+    assert n_wall > 5
+
+
+def test_kn_textbook(oracle_mod):
+    """Factorised KN of scatter.cu:962-977 equals the textbook form
+    P^2 (P + 1/P - sin^2 theta), P = E'/E0, i.e. 2/r_e^2 dsigma/dOmega."""
+    for e0 in (70.0, 140.5, 364.5, 511.0):
+        for th in np.linspace(0.0, math.pi, 37):
+            P = 1.0 / (1.0 + e0 / 511.0 * (1.0 - math.cos(th)))
+            text = P * P * (P + 1.0 / P - math.sin(th) ** 2)
+            assert oracle_mod.kn(th, e0) == pytest.approx(text, rel=1e-13)
+
+
+def test_kn_normaliser_golden(oracle_mod):
+    """The [0, pi] normaliser with the reference's float step and end
+    (0.01f, (float)M_PI; scatter.cu:1169) against an mpmath evaluation."""
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 40
+    step = mp.mpf(float(np.float32(0.01)))
+    hi = mp.mpf(float(np.float32(math.pi)))
+    n = int(hi / step)
+    e0 = mp.mpf(140.5)
+    total = mp.mpf(0)
+    for i in range(n + 1):
+        th = i * step
+        P = 1 / (1 + e0 / 511 * (1 - mp.cos(th)))
+        total += P * P * (P + 1 / P - mp.sin(th) ** 2) * mp.sin(th) * step
+    got = oracle_mod.kn_sum(140.5, 0.0, float(np.float32(math.pi)), float(np.float32(0.01)))
+    assert got == pytest.approx(float(total), rel=1e-13)
+    # the rectangle sum approximates the closed-form integral of KN sin
+    a = 140.5 / 511.0
+    closed = ((1 + a) / a ** 2 * (2 * (1 + a) / (1 + 2 * a) - math.log(1 + 2 * a) / a)
+              + math.log(1 + 2 * a) / (2 * a) - (1 + 3 * a) / (1 + 2 * a) ** 2)
+    assert got == pytest.approx(2 * closed, rel=2e-3)
+
+
+def test_kn_sum_step_count(oracle_mod):
+    """Inclusive rule with n = (int)((hi-lo)/step): n + 1 terms."""
+    e0, step = 140.5, 0.01
+    lo, hi = 0.3, 0.3 + 5 * step + 0.004
+    expect = sum(oracle_mod.kn(lo + i * step, e0) * math.sin(lo + i * step) * step for i in range(6))
+    assert oracle_mod.kn_sum(e0, lo, hi, step) == pytest.approx(expect, rel=1e-14)
+    # an empty range still counts the first term
+    assert oracle_mod.kn_sum(e0, lo, lo, step) == pytest.approx(
+        oracle_mod.kn(lo, e0) * math.sin(lo) * step, rel=1e-14)
+
+
+def test_scatter_energy_limits(oracle_mod):
+    e0 = 140.5
+    assert oracle_mod.scatter_energy(0.0, e0) == e0
+    assert oracle_mod.scatter_energy(math.pi, e0) == pytest.approx(e0 / (1 + 2 * e0 / 511.0), rel=1e-15)
+    # Compton formula in wavelength form: 1/E' - 1/E0 = (1 - cos)/511
+    th = 1.1
+    assert 1 / oracle_mod.scatter_energy(th, e0) - 1 / e0 == pytest.approx((1 - math.cos(th)) / 511.0, rel=1e-12)
+
+
+def test_cone_angle(oracle_mod):
+    assert oracle_mod.cone_angle([0, 0, 0], [0, 1, 0], [0, 2, 0]) == 0.0
+    assert oracle_mod.cone_angle([0, 0, 0], [0, 1, 0], [1, 1, 0]) == pytest.approx(math.pi / 2)
+    assert oracle_mod.cone_angle([0, 0, 0], [0, 1, 0], [0, 0, 0]) == pytest.approx(math.pi)
+    rng = np.random.default_rng(4)
+    for _ in range(100):
+        s, a, d = rng.normal(size=(3, 3))
+        u, w = a - s, d - a
+        ref = math.atan2(np.linalg.norm(np.cross(u, w)), u @ w)
+        assert oracle_mod.cone_angle(s, a, d) == pytest.approx(ref, abs=1e-12)
+
+
+def test_gauss_window(oracle_mod):
+    """0.5 [erf(z2) - erf(z1)], sigma = res/2.35482 * E (scatter.cu:1025-1033),
+    against scipy's normal CDF."""
+    fwhm = float(np.float32(2.35482))
+    for e in (110.0, 126.0, 140.5, 150.0):
+        for res in (0.05, 0.1, 0.2):
+            sigma = res / fwhm * e
+            ref = special.ndtr((154.55 - e) / sigma) - special.ndtr((126.45 - e) / sigma)
+            assert oracle_mod.gauss_window(e, res, 126.45, 154.55) == pytest.approx(ref, rel=1e-12, abs=1e-15)
