@@ -8,6 +8,7 @@
 // exiting the process.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -576,6 +577,67 @@ smg_status smg_scatter_collimator(smg_ctx* c, int rotation, smg_coll_variant var
     if (variant != SMG_COLL_COMBINED && variant != SMG_COLL_STANDALONE)
         return fail(c, SMG_ERR_BAD_PARAM, "unknown collimator variant %d", (int)variant);
     return scatter_entry(c, true, rotation, (int)variant, prec, nullptr, 0, dst, dst_is_device, stats);
+}
+
+smg_status smg_pe_backward(smg_ctx* c, int rotation, const float* w, int w_is_device, double* grad,
+                           double* loss, smg_stats* stats) {
+    double t_start = now_ms();
+    smg_status s = check_ready(c, rotation);
+    if (s != SMG_OK) return s;
+    if (!w || !grad) return fail(c, SMG_ERR_BAD_PARAM, "w and grad must not be NULL");
+    const GeomDesc& g = c->hg.g;
+    const size_t n_el = (size_t)g.n_det * (size_t)g.n_vox;
+    PeLaunch A;
+    A.geom = device_geom(c);
+    A.dst = nullptr;
+    A.counters = c->counters.ptr;
+    double phi = (double)rotation * g.dphi;
+    A.cphi = std::cos(phi);
+    A.sphi = std::sin(phi);
+    for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
+    pick_tile(g, A.tile, A.tiles);
+    const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    PeGradScratch S;
+    // enough waves to fill the GPU: about 8192 (bar, tile group) pairs
+    S.groups = (int)std::max<int64_t>(1, std::min<int64_t>(n_tiles, (8192 + g.n_det - 1) / g.n_det));
+    S.stride = (int64_t)g.n_det + g.n_layers + 2;
+    if ((size_t)g.n_det * sizeof(double) > 64 * 1024)
+        return fail(c, SMG_ERR_CAPACITY, "adjoint LDS row of %d bars exceeds 64 KiB", g.n_det);
+    SMG_HIP(c, c->scratch.reserve((size_t)g.n_det * S.groups * S.stride));
+    S.rows = c->scratch.ptr;
+    const int n_out = g.n_layers + 2 * g.n_det + 1;
+    SMG_HIP(c, c->grad.reserve((size_t)n_out));
+    if (w_is_device) {
+        S.w = w;
+    } else {
+        SMG_HIP(c, c->in.reserve(n_el));
+        SMG_HIP(c, hipMemcpyAsync(c->in.ptr, w, n_el * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        S.w = c->in.ptr;
+    }
+    SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    SMG_HIP(c, launch_pe_backward(A, S, false, c->grad.ptr, c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    std::vector<double> out((size_t)n_out);
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    SMG_HIP(c, hipMemcpyAsync(out.data(), c->grad.ptr, n_out * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    SMG_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (cnt[1])
+        return fail(c, SMG_ERR_CAPACITY, "%llu waves exceeded the candidate capacity", cnt[1]);
+    std::memcpy(grad, out.data(), (size_t)(n_out - 1) * sizeof(double));
+    if (loss) *loss = out[(size_t)n_out - 1];
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t_start;
+        stats->elements = n_el;
+        stats->rays = n_el * (uint64_t)(c->pe_sub[0] * c->pe_sub[1] * c->pe_sub[2]);
+        stats->fp64_fallbacks = cnt[0];
+        stats->overflow_waves = 0;
+    }
+    return SMG_OK;
 }
 
 }  // extern "C"

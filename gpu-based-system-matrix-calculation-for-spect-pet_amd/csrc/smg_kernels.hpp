@@ -27,6 +27,18 @@ struct PeLaunch {
 
 hipError_t launch_pe_forward(const PeLaunch& a, bool exact, hipStream_t stream);
 
+// Adjoint scratch: one row of `stride` doubles per (bar d, tile group):
+// [n_det mu_total partials][n_layers layer partials][mu_pe of d][loss].
+struct PeGradScratch {
+    double* rows;
+    int64_t stride;
+    int groups;
+    const float* w;   // [N][V] weights of L = sum w A
+};
+
+hipError_t launch_pe_backward(const PeLaunch& a, const PeGradScratch& s, bool exact, double* grad,
+                              hipStream_t stream);
+
 struct BitmapLaunch {
     const Sphere<float>* spheres_f;   // bars
     const Sphere<double>* spheres_d;

@@ -188,6 +188,56 @@ SMG_FN void make_cell_ctx(const Bar<double>& bd, int a, int b, int c, int nsx, i
 //   int nbars() const; int bar(int j) const; bool bar_on(int j) const;
 //   int nholes() const; int hole(int j) const; bool hole_on(int j) const;
 
+// Chord of the ray voxel p -> sub-cell point q through another bar k
+// (PESysMatGen.cu:1290-1325).  FP32: slab parameters relative to the point of
+// the ray closest to the bar centre, placed in FP64; FP64: the reference form.
+template <class T>
+SMG_FN T other_bar_chord(const PeGeom& G, int k, const double p[3], const double D[3], double dd,
+                         const double q[3], const SegDir<T>& sw, T tol_d, bool& near) {
+    const Bar<double>& bk = G.bars_d[k];
+    if constexpr (std::is_same<T, float>::value) {
+        const Bar<float>& bf = G.bars_f[k];
+        double rel[3] = {bk.cx - p[0], bk.cy - p[1], bk.cz - p[2]};
+        double proj = rel[0] * D[0] + rel[1] * D[1] + rel[2] * D[2];
+        double inv_dd = 1.0 / dd;
+        double t0 = proj * inv_dd;
+        double s0 = t0 * inv_dd;
+        double o[3] = {D[0] * s0 - rel[0], D[1] * s0 - rel[1], D[2] * s0 - rel[2]};
+        float lo[3] = {-0.5f * bf.w, -0.5f * bf.t, -0.5f * bf.h};
+        float hi[3] = {0.5f * bf.w, 0.5f * bf.t, 0.5f * bf.h};
+        float lim_hi = float(dd - t0), lim_lo = float(double(kSlabEps<float>) - t0);
+        if (bk.sb == 0.0 && bk.cb == 1.0) {
+            float of[3] = {float(o[0]), float(o[1]), float(o[2])};
+            return seg_box_local(sw, of, lim_lo, lim_hi, lo, hi, near);
+        }
+        double cb = bk.cb, sb = -bk.sb;
+        float of[3] = {float(o[0] * cb - o[2] * sb), float(o[1]), float(o[0] * sb + o[2] * cb)};
+        SegDir<float> sr;
+        segdir_init<float>(sr, float(D[0] * cb - D[2] * sb), float(D[1]), float(D[0] * sb + D[2] * cb),
+                           tol_d, near);
+        return seg_box_local(sr, of, lim_lo, lim_hi, lo, hi, near);
+    } else {
+        double a[3], b[3];
+        to_frame(bk, p, a);
+        to_frame(bk, q, b);
+        double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
+        double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
+        SegDir<double> sr;
+        segdir_init<double>(sr, b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.0, near);
+        return seg_box<double>(sr, a, lo, hi, near);
+    }
+}
+
+// World-frame direction data of the ray voxel -> sub-cell point.
+template <class T>
+SMG_FN void ray_dir(const double p[3], const double q[3], double D[3], double& dd, SegDir<T>& sw,
+                    T& tol_d, bool& near) {
+    for (int k = 0; k < 3; ++k) D[k] = q[k] - p[k];
+    dd = sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+    tol_d = T(4) * kUnit<T> * (tabs(T(D[0])) + tabs(T(D[1])) + tabs(T(D[2])));
+    segdir_init<T>(sw, T(D[0]), T(D[1]), T(D[2]), tol_d, near);
+}
+
 template <class T, class Cands>
 SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[3],
                      const CellCtx& cc, const Cands& C, bool& near, double* dl_layer = nullptr,
@@ -266,41 +316,9 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
             continue;
         }
         int k = C.bar(j);
-        const Bar<double>& bk = G.bars_d[k];
-        T l;
-        if constexpr (std::is_same<T, float>::value) {
-            const Bar<float>& bf = G.bars_f[k];
-            // local origin: the point of the ray closest to the bar centre
-            double rel[3] = {bk.cx - p[0], bk.cy - p[1], bk.cz - p[2]};
-            double t0 = (rel[0] * D[0] + rel[1] * D[1] + rel[2] * D[2]) / dd;
-            double s0 = t0 / dd;
-            double o[3] = {D[0] * s0 - rel[0], D[1] * s0 - rel[1], D[2] * s0 - rel[2]};
-            float lo[3] = {-0.5f * bf.w, -0.5f * bf.t, -0.5f * bf.h};
-            float hi[3] = {0.5f * bf.w, 0.5f * bf.t, 0.5f * bf.h};
-            float lim_hi = float(dd - t0), lim_lo = float(double(kSlabEps<float>) - t0);
-            if (bk.sb == 0.0 && bk.cb == 1.0) {
-                float of[3] = {float(o[0]), float(o[1]), float(o[2])};
-                l = seg_box_local(sw, of, lim_lo, lim_hi, lo, hi, near);
-            } else {
-                double cb = bk.cb, sb = -bk.sb;
-                float of[3] = {float(o[0] * cb - o[2] * sb), float(o[1]), float(o[0] * sb + o[2] * cb)};
-                SegDir<float> sr;
-                segdir_init<float>(sr, float(D[0] * cb - D[2] * sb), float(D[1]),
-                                   float(D[0] * sb + D[2] * cb), tol_d, near);
-                l = seg_box_local(sr, of, lim_lo, lim_hi, lo, hi, near);
-            }
-        } else {
-            double a[3], b[3];
-            to_frame(bk, p, a);
-            to_frame(bk, cc.q, b);
-            double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
-            double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
-            SegDir<double> sr;
-            segdir_init<double>(sr, b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.0, near);
-            l = seg_box<double>(sr, a, lo, hi, near);
-        }
+        T l = other_bar_chord<T>(G, k, p, D, dd, cc.q, sw, tol_d, near);
         if (dl_bar) dl_bar[j] = double(l);
-        att += double(l) * bk.mut;
+        att += double(l) * G.bars_d[k].mut;
     }
 
     // own bar from its surface to the sub-cell (PESysMatGen.cu:1329-1346), and the

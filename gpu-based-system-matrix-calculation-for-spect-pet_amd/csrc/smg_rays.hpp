@@ -49,6 +49,13 @@ SMG_FN T tmin2(T a, T b) {
     return a <= b ? a : b;
 }
 
+// Element k of a 3-vector by selects: indexing with a run-time k would put
+// the whole array in scratch memory.
+template <class T>
+SMG_FN T at3(const T v[3], int k) {
+    return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]);
+}
+
 // Comparisons that flag a decision taken within `tol` of its threshold.
 template <class T>
 SMG_FN bool g_lt(T x, T y, T tol, bool& near) {
@@ -163,23 +170,27 @@ SMG_FN T seg_box(const SegDir<T>& s, const T a[3], const T lo[3], const T hi[3],
         int k = !s.small[0] ? 0 : (!s.small[1] ? 1 : 2);
         int i = k == 0 ? 1 : 0;
         int j = k == 2 ? 1 : 2;
-        T b_k = a[k] + s.D[k];
-        T ti = T(4) * kUnit<T> * (tabs(a[i]) + tabs(lo[i]) + tabs(hi[i])) + T(1e-7);
-        T tj = T(4) * kUnit<T> * (tabs(a[j]) + tabs(lo[j]) + tabs(hi[j])) + T(1e-7);
-        T tk = T(4) * kUnit<T> * (tabs(a[k]) + tabs(b_k) + tabs(lo[k]) + tabs(hi[k])) + T(1e-7);
-        bool in = g_ge(a[i], lo[i], ti, near) && g_le(a[i], hi[i], ti, near) &&
-                  g_ge(a[j], lo[j], tj, near) && g_le(a[j], hi[j], tj, near);
+        const T ai = at3(a, i), aj = at3(a, j), ak = at3(a, k);
+        const T loi = at3(lo, i), loj = at3(lo, j), lok = at3(lo, k);
+        const T hii = at3(hi, i), hij = at3(hi, j), hik = at3(hi, k);
+        T b_k = ak + at3(s.D, k);
+        T ti = T(4) * kUnit<T> * (tabs(ai) + tabs(loi) + tabs(hii)) + T(1e-7);
+        T tj = T(4) * kUnit<T> * (tabs(aj) + tabs(loj) + tabs(hij)) + T(1e-7);
+        T tk = T(4) * kUnit<T> * (tabs(ak) + tabs(b_k) + tabs(lok) + tabs(hik)) + T(1e-7);
+        bool in = g_ge(ai, loi, ti, near) && g_le(ai, hii, ti, near) && g_ge(aj, loj, tj, near) &&
+                  g_le(aj, hij, tj, near);
         if (!in) return T(0);
-        bool span = (g_le(a[k], lo[k], tk, near) && g_ge(b_k, hi[k], tk, near)) ||
-                    (g_le(b_k, lo[k], tk, near) && g_ge(a[k], hi[k], tk, near));
-        return span ? tabs(hi[k] - lo[k]) : T(0);
+        bool span = (g_le(ak, lok, tk, near) && g_ge(b_k, hik, tk, near)) ||
+                    (g_le(b_k, lok, tk, near) && g_ge(ak, hik, tk, near));
+        return span ? tabs(hik - lok) : T(0);
     }
     if (s.ns == 1) {
         // the start must lie strictly between the faces of the small axis
         // (PESysMatGen.cu:69-73, :125-128, :180-183)
         int k = s.small[0] ? 0 : (s.small[1] ? 1 : 2);
-        T tk = T(4) * kUnit<T> * (tabs(a[k]) + tabs(lo[k]) + tabs(hi[k])) + T(1e-7);
-        if (g_ge(a[k], hi[k], tk, near) || g_le(a[k], lo[k], tk, near)) return T(0);
+        const T ak = at3(a, k), lok = at3(lo, k), hik = at3(hi, k);
+        T tk = T(4) * kUnit<T> * (tabs(ak) + tabs(lok) + tabs(hik)) + T(1e-7);
+        if (g_ge(ak, hik, tk, near) || g_le(ak, lok, tk, near)) return T(0);
     }
     T tmin, tmax, emin, emax;
     slab(s, a, lo, hi, tmin, tmax, emin, emax);
@@ -208,20 +219,24 @@ SMG_FN T seg_box_inside(const SegDir<T>& s, const T a[3], const T lo[3], const T
         int k = !s.small[0] ? 0 : (!s.small[1] ? 1 : 2);
         int i = k == 0 ? 1 : 0;
         int j = k == 2 ? 1 : 2;
-        T b_k = a[k] + s.D[k];
-        T ti = T(4) * kUnit<T> * (tabs(a[i]) + tabs(lo[i]) + tabs(hi[i])) + T(1e-7);
-        T tj = T(4) * kUnit<T> * (tabs(a[j]) + tabs(lo[j]) + tabs(hi[j])) + T(1e-7);
-        bool in = g_ge(a[i], lo[i], ti, near) && g_le(a[i], hi[i], ti, near) &&
-                  g_ge(a[j], lo[j], tj, near) && g_le(a[j], hi[j], tj, near);
+        const T ai = at3(a, i), aj = at3(a, j), ak = at3(a, k);
+        const T loi = at3(lo, i), loj = at3(lo, j), lok = at3(lo, k);
+        const T hii = at3(hi, i), hij = at3(hi, j), hik = at3(hi, k);
+        T b_k = ak + at3(s.D, k);
+        T ti = T(4) * kUnit<T> * (tabs(ai) + tabs(loi) + tabs(hii)) + T(1e-7);
+        T tj = T(4) * kUnit<T> * (tabs(aj) + tabs(loj) + tabs(hij)) + T(1e-7);
+        bool in = g_ge(ai, loi, ti, near) && g_le(ai, hii, ti, near) && g_ge(aj, loj, tj, near) &&
+                  g_le(aj, hij, tj, near);
         if (!in) return T(0);
-        if (a[k] < b_k) return b_k - lo[k];
-        if (a[k] > b_k) return hi[k] - b_k;
+        if (ak < b_k) return b_k - lok;
+        if (ak > b_k) return hik - b_k;
         return T(0);
     }
     if (s.ns == 1) {
         int k = s.small[0] ? 0 : (s.small[1] ? 1 : 2);
-        T tk = T(4) * kUnit<T> * (tabs(a[k]) + tabs(lo[k]) + tabs(hi[k])) + T(1e-7);
-        if (g_ge(a[k], hi[k], tk, near) || g_le(a[k], lo[k], tk, near)) return T(0);
+        const T ak = at3(a, k), lok = at3(lo, k), hik = at3(hi, k);
+        T tk = T(4) * kUnit<T> * (tabs(ak) + tabs(lok) + tabs(hik)) + T(1e-7);
+        if (g_ge(ak, hik, tk, near) || g_le(ak, lok, tk, near)) return T(0);
     }
     T tmin, tmax, emin, emax;
     slab(s, a, lo, hi, tmin, tmax, emin, emax);

@@ -174,6 +174,7 @@ class Context:
                                              _fptr(arrs[1]), arrs[1].size, _fptr(arrs[2]),
                                              arrs[2].size, _fptr(arrs[3]), arrs[3].size))
         self.dims = self.get_dims()
+        self._n_layers = p.n_layers
         return self.dims
 
     def get_dims(self) -> Dims:
@@ -207,8 +208,9 @@ class Context:
         return out, st.as_dict()
 
     def pe_backward(self, w: np.ndarray, rotation: int = 0) -> tuple[float, np.ndarray, dict]:
+        """Loss sum(w * A) and its gradient: [layer mu][bar mu_total][bar mu_pe]."""
         w = np.ascontiguousarray(w, dtype=np.float32).reshape(self.dims.n_det, self.dims.n_vox)
-        n_par = self._n_layers_hint() + 2 * self.dims.n_det
+        n_par = self._n_layers + 2 * self.dims.n_det
         grad = np.zeros(n_par, np.float64)
         loss = C.c_double()
         st = Stats()
@@ -216,9 +218,6 @@ class Context:
                                             grad.ctypes.data_as(C.POINTER(C.c_double)),
                                             C.byref(loss), C.byref(st)))
         return loss.value, grad, st.as_dict()
-
-    def _n_layers_hint(self) -> int:
-        return self._n_layers
 
     # ---- bitmaps -----------------------------------------------------------------
     def bitmap_crystal(self) -> np.ndarray:
@@ -268,5 +267,4 @@ def open_context(p, device: int = 0) -> Context:
     """Context with the geometry of ``p`` (a :class:`params.Params`)."""
     ctx = Context(device)
     ctx.set_geometry(p)
-    ctx._n_layers = p.n_layers
     return ctx
