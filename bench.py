@@ -8,6 +8,12 @@ matrix elements, 7.25e9 sub-cell rays.  The geometry and the output buffer
 (1 GiB) are resident on the GPU before the timed region; the output stays on
 the GPU.  Prints one JSON line.
 
+Unless --no-extras is given, the line also carries (under "extras") the
+all-FP64 time of the same step, the primary-Compton scatter throughputs with
+the reference's scatter points (crystal scatter on 32x32x1 voxels x 2048 bars,
+collimator scatter on 32x32x16 voxels x 64 bars), and the forward and
+adjoint (dL/dmu) times on the 32x32x1-voxel, 64-bar geometry.
+
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision fp32|fp64]
 """
 from __future__ import annotations
@@ -80,6 +86,77 @@ def ops_per_ray(p, sub):
     return o.pe_ops_per_ray(d, v, sub=sub)
 
 
+def timed(fn, reps):
+    """Median kernel time (ms) of reps calls of fn() -> (result, stats)."""
+    ks = sorted(fn()[1]["kernel_ms"] for _ in range(reps))
+    return ks[len(ks) // 2]
+
+
+def extras(smg, params, pe_ctx, out, precision):
+    """Secondary measurements: FP64 vs FP32 on the headline step, scatter
+    throughputs, adjoint cost."""
+    import numpy as np
+
+    ex = {}
+    other = "fp64" if precision == "fp32" else "fp32"
+    pe_ctx.pe_forward(0, other, device_ptr=out.data_ptr())
+    ms_other = timed(lambda: pe_ctx.pe_forward(0, other, device_ptr=out.data_ptr()), 1)
+    ms_this = timed(lambda: pe_ctx.pe_forward(0, precision, device_ptr=out.data_ptr()), 1)
+    ex[f"pe_{other}_kernel_ms"] = round(ms_other, 2)
+    ex["pe_fp64_over_fp32_time"] = round((ms_other / ms_this) if other == "fp64" else (ms_this / ms_other), 3)
+    log(f"PE {other}: {ms_other:.1f} ms")
+
+    # crystal scatter (ScatterGen_Crystal path): 32x32x1 voxels, 2048 bars, 2047 scatter points
+    p4 = params.config(4)
+    with smg.open_context(p4) as c:
+        c.set_pe_subsamples(*p4.meta["pe_subsamples"])
+        pe, _ = c.pe_forward(0, "fp32")
+        c.set_scatter_subsamples(*p4.meta["subsamples"])
+        c.scatter_crystal(pe, 0, "fp32")  # builds the bitmap, warms up
+        for prec in ("fp32", "fp64"):
+            _, st = c.scatter_crystal(pe, 0, prec)
+            ms = timed(lambda: c.scatter_crystal(pe, 0, prec), 3 if prec == "fp32" else 1)
+            ex[f"crystal_scatter_{prec}"] = {
+                "elements_per_s": round(st["elements"] / ms * 1e3, 1),
+                "pairs_per_s": round(st["rays"] / ms * 1e3, 1),
+                "kernel_ms": round(ms, 3), "fp64_fallbacks": st["fp64_fallbacks"]}
+            log(f"crystal scatter {prec}: {ms:.2f} ms")
+        ex["crystal_scatter_workload"] = ("32x32x1 voxels x 2048 bars, scatter points at the "
+                                          "2047 other bar centres (reference semantics), 1x1x1 cells")
+
+    # collimator scatter (combined variant): 32x32x16 voxels, 64 bars, 16 hole centres
+    p3 = params.config(3)
+    with smg.open_context(p3) as c:
+        c.set_scatter_subsamples(*p3.meta["subsamples"])
+        c.scatter_collimator(0, "combined", "fp32")
+        for prec in ("fp32", "fp64"):
+            _, st = c.scatter_collimator(0, "combined", prec)
+            ms = timed(lambda: c.scatter_collimator(0, "combined", prec), 3)
+            ex[f"collimator_scatter_{prec}"] = {
+                "elements_per_s": round(st["elements"] / ms * 1e3, 1),
+                "pairs_per_s": round(st["rays"] / ms * 1e3, 1),
+                "kernel_ms": round(ms, 3), "fp64_fallbacks": st["fp64_fallbacks"]}
+            log(f"collimator scatter {prec}: {ms:.2f} ms")
+        ex["collimator_scatter_workload"] = ("32x32x16 voxels x 64 bars, scatter points at the 16 "
+                                             "hole centres (reference semantics), 1x1x1 cells")
+
+    # adjoint: forward + dL/dmu on the 32x32x1-voxel, 64-bar geometry, 4x4x4 sub-cells
+    p5 = params.config(5)
+    w = np.random.default_rng(2).uniform(0, 1, (p5.n_det, p5.n_vox)).astype(np.float32)
+    with smg.open_context(p5) as c:
+        c.set_pe_subsamples(*p5.meta["subsamples"])
+        c.pe_forward(0, "fp32")
+        c.pe_backward(w)
+        f_ms = timed(lambda: c.pe_forward(0, "fp32"), 5)
+        b_ms = timed(lambda: c.pe_backward(w)[1:], 5)
+    ex["adjoint"] = {"forward_kernel_ms": round(f_ms, 3), "backward_kernel_ms": round(b_ms, 3),
+                     "fwd_plus_grad_ms": round(f_ms + b_ms, 3),
+                     "backward_over_forward": round(b_ms / f_ms, 2),
+                     "workload": "32x32x1 voxels x 64 bars, 4x4x4 sub-cells, 1 + 128 mu"}
+    log(f"adjoint: forward {f_ms:.3f} ms, backward {b_ms:.3f} ms")
+    return ex
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,10 +165,12 @@ def main():
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="host time budget of the CPU-baseline sample (0 to skip)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the FP64, scatter and adjoint measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by detector rows, see DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU run would give each GPU its own detector rows)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -141,6 +220,7 @@ def main():
         "rays_per_launch": rays,
         "write_GBps": round(n_el * 4 / avg_kernel_s / 1e9, 2),
     }
+    ex = None if args.no_extras else extras(smg, params, ctx, out, args.precision)
     cpu = cpu_baseline(p, sub, args.cpu_seconds) if args.cpu_seconds > 0 else None
     line = {
         "metric": METRIC,
@@ -166,6 +246,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "extras": ex,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -5,13 +5,16 @@
 // and holes for culling (PEGen_RayTracing_CircularHole/PESysMatGen.cu:1097-1186)
 // before its sub-cell loop (:1193-1362).  Here the 64 lanes of a wave share
 // detector d, so every sub-cell point is wave-uniform and:
-//   1. the wave scans the bars and holes once, 64 at a time, with a
-//      conservative test against the tile's bounding sphere; each survivor is
+//   1. the wave scans the bars and holes once, 64 at a time, keeping those
+//      whose box a segment from bar d to the tile's bounding ball can meet
+//      (any other has chord 0 for every ray of the wave, so whether the
+//      reference would cull it does not change the result); each survivor is
 //      then tested with the reference culling predicate per lane and stored
 //      in LDS with the 64-bit ballot of the lanes that keep it;
-//   2. for each sub-cell, the candidates are narrowed to those whose bounding
-//      sphere can meet the cone from the sub-cell centre to the tile (a bar
-//      outside that cone has chord 0 for every lane, so dropping it is exact);
+//   2. for each sub-cell, the candidates are narrowed to those whose box can
+//      meet a segment from the sub-cell centre to the tile's bounding ball (a
+//      bar outside that cone has chord 0 for every lane, so dropping it is
+//      exact);
 //   3. every lane traces its own ray against the narrowed lists (wave-uniform
 //      loops, per-lane masks) and accumulates the sub-cell terms with a
 //      two-sum compensated sum;
@@ -24,6 +27,9 @@
 namespace smg {
 
 namespace {
+
+// Capacity of the per-wave queue of sub-cells to redo in FP64.
+constexpr int kRedoCap = 64;
 
 // Candidate lists of one wave, in LDS.
 struct WaveLists {
@@ -40,68 +46,17 @@ struct LdsCands {
     const WaveLists* L;
     int nb, nh, lane;
     __device__ int nbars() const { return nb; }
-    __device__ int bar(int j) const { return L->bar_idx[L->bar_sel[j]]; }
+    // the index is wave-uniform: readfirstlane lets the record loads go
+    // through the scalar unit
+    __device__ int bar(int j) const { return __builtin_amdgcn_readfirstlane(L->bar_idx[L->bar_sel[j]]); }
     __device__ bool bar_on(int j) const { return (L->bar_mask[L->bar_sel[j]] >> lane) & 1ull; }
     __device__ int nholes() const { return nh; }
-    __device__ int hole(int j) const { return L->hole_idx[L->hole_sel[j]]; }
+    __device__ int hole(int j) const { return __builtin_amdgcn_readfirstlane(L->hole_idx[L->hole_sel[j]]); }
     __device__ bool hole_on(int j) const { return (L->hole_mask[L->hole_sel[j]] >> lane) & 1ull; }
 };
 
 __device__ __forceinline__ int lanes_below(uint64_t m, int lane) {
     return __popcll(m & ((1ull << lane) - 1ull));
-}
-
-// Conservative superset of the reference culling test for any voxel in the
-// ball (tc, rho): the angle between the directions to a fixed point seen from
-// two points of the ball differs by at most asin(rho/D) <= rho/sqrt(D^2-rho^2),
-// and the thresholds grow at most to r/(D - rho).
-__device__ __forceinline__ bool tile_may_keep(const double tc[3], double rho, const double ud[3],
-                                              double Dd, double r_self, double cx, double cy,
-                                              double cz, double r_other) {
-    double dx = cx - tc[0], dy = cy - tc[1], dz = cz - tc[2];
-    double D = sqrt(dx * dx + dy * dy + dz * dz);
-    if (D <= 1.01 * rho || Dd <= 1.01 * rho) return true;
-    double ux = dx / D - ud[0], uy = dy / D - ud[1], uz = dz / D - ud[2];
-    double s = sqrt(ux * ux + uy * uy + uz * uz);
-    double e = rho / sqrt(D * D - rho * rho) + rho / sqrt(Dd * Dd - rho * rho);
-    double lim = r_other / (D - rho) + r_self / (Dd - rho);
-    return s - e <= lim + 1e-4;
-}
-
-// Can any ray from a point of the ball (tc, rho) to the apex q touch the
-// sphere (c, R)?  The rays lie in the cone of half-angle asin(rho/Lt) around
-// q -> tc, cut at Lt + rho.
-struct Cone {
-    float q[3], u[3];
-    float Lt, rho, tana, seca;
-    bool all;
-};
-
-__device__ __forceinline__ void make_cone(Cone& k, const double q[3], const double tc[3], double rho) {
-    double ux = tc[0] - q[0], uy = tc[1] - q[1], uz = tc[2] - q[2];
-    double Lt = sqrt(ux * ux + uy * uy + uz * uz);
-    k.all = Lt <= 1.01 * rho;
-    double sa = k.all ? 0.0 : rho / Lt;
-    double ca = sqrt(1.0 - sa * sa);
-    for (int i = 0; i < 3; ++i) k.q[i] = float(q[i]);
-    k.u[0] = float(ux / Lt);
-    k.u[1] = float(uy / Lt);
-    k.u[2] = float(uz / Lt);
-    k.Lt = float(Lt);
-    k.rho = float(rho);
-    k.tana = float(sa / ca);
-    k.seca = float(1.0 / ca);
-}
-
-__device__ __forceinline__ bool cone_hits(const Cone& k, float cx, float cy, float cz, float R) {
-    if (k.all) return true;
-    float wx = cx - k.q[0], wy = cy - k.q[1], wz = cz - k.q[2];
-    float proj = wx * k.u[0] + wy * k.u[1] + wz * k.u[2];
-    float w2 = wx * wx + wy * wy + wz * wz;
-    float perp = sqrtf(fmaxf(w2 - proj * proj, 0.0f));
-    float m = 2e-3f + 1e-5f * sqrtf(w2);
-    return proj >= -R - m && proj <= k.Lt + k.rho + R + m &&
-           perp <= R * k.seca + fmaxf(proj, 0.0f) * k.tana + m;
 }
 
 // Lane and tile state of one wave: voxel of the lane, tile bounding sphere
@@ -114,7 +69,8 @@ struct WaveSetup {
     float pf[3];
     SelfCull sc;
     float udf[3], critf;
-    double tc[3], rho, ud_t[3], Dd_t, rself;
+    double tc[3], rho;
+    float tcf[3], rhof;
 };
 
 __device__ __forceinline__ void wave_setup(const PeLaunch& A, int d, int tile, int lane, WaveSetup& W) {
@@ -137,40 +93,30 @@ __device__ __forceinline__ void wave_setup(const PeLaunch& A, int d, int tile, i
     self_cull(bd, W.p, W.sc);
     for (int k = 0; k < 3; ++k) W.udf[k] = float(W.sc.ud[k]);
     W.critf = float(W.sc.crit);
-    // tile bounding sphere (the rotation about z preserves distances)
-    const int ex = min(A.tile[0], g.nx - ix0), ey = min(A.tile[1], g.ny - iy0),
-              ez = min(A.tile[2], g.nz - iz0);
-    double x = (ix0 + 0.5 * (ex - 1) - g.nx / 2.0 + 0.5) * g.wx + g.sx;
-    double y = (iy0 + 0.5 * (ey - 1) - g.ny / 2.0 + 0.5) * g.wy + g.sy;
-    double z = (iz0 + 0.5 * (ez - 1) - g.nz / 2.0 + 0.5) * g.wz + g.sz;
-    W.tc[0] = x * A.cphi - y * A.sphi;
-    W.tc[1] = x * A.sphi + y * A.cphi;
-    W.tc[2] = z;
-    W.rho = 0.5 * sqrt(((ex - 1) * g.wx) * ((ex - 1) * g.wx) + ((ey - 1) * g.wy) * ((ey - 1) * g.wy) +
-                       ((ez - 1) * g.wz) * ((ez - 1) * g.wz)) + 1e-6;
-    double dx = bd.cx - W.tc[0], dy = bd.cy - W.tc[1], dz = bd.cz - W.tc[2];
-    W.Dd_t = sqrt(dx * dx + dy * dy + dz * dz);
-    W.ud_t[0] = dx / W.Dd_t;
-    W.ud_t[1] = dy / W.Dd_t;
-    W.ud_t[2] = dz / W.Dd_t;
-    W.rself = 0.5 * bd.rdiag;
+    tile_ball(g, ix0, iy0, iz0, A.tile, A.cphi, A.sphi, W.tc, W.rho);
+    tile_ball_f(W.tc, W.rho, W.tcf, W.rhof);
 }
 
-// Step 1: candidate bars and holes of the tile, each with the ballot of the
-// lanes whose reference culling test keeps it.  Returns false on overflow.
+// Step 1: candidate bars and holes of the tile: those whose box a segment from
+// bar d to the tile can meet (bar_tile_may_hit; any other has chord 0 for
+// every ray of the tile, so its culling decision does not matter), each with
+// the ballot of the lanes whose reference culling test keeps it.  Returns
+// false on overflow.
 template <bool kExact>
 __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveSetup& W,
                                  WaveLists& L, int& nb, int& nh) {
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;
+    const Bar<float>& bdf = G.bars_f[d];
     nb = 0;
     nh = 0;
     for (int base = 0; base < g.n_det; base += 64) {
         int k = base + lane;
         bool pass = false;
         if (k < g.n_det && k != d) {
-            const Bar<double>& b = G.bars_d[k];
-            pass = tile_may_keep(W.tc, W.rho, W.ud_t, W.Dd_t, W.rself, b.cx, b.cy, b.cz, 0.5 * b.rdiag);
+            const Bar<float>& b = G.bars_f[k];
+            pass = bar_tile_may_hit(bdf, W.tcf, W.rhof, b.cx, b.cy, b.cz, b.cb, b.sb, 0.5f * b.w,
+                                    0.5f * b.t, 0.5f * b.h);
         }
         uint64_t bal = __ballot(pass);
         while (bal) {
@@ -192,8 +138,9 @@ __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveS
         int h = base + lane;
         bool pass = false;
         if (h < g.n_holes) {
-            const Hole<double>& o = G.holes_d[h];
-            pass = tile_may_keep(W.tc, W.rho, W.ud_t, W.Dd_t, W.rself, o.xc, o.ymid, o.zc, o.crit);
+            const Hole<float>& o = G.holes_f[h];
+            pass = bar_tile_may_hit(bdf, W.tcf, W.rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f, o.r,
+                                    0.5f * tabs(o.y2 - o.y1), o.r);
         }
         uint64_t bal = __ballot(pass);
         while (bal) {
@@ -214,13 +161,12 @@ __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveS
     return nb <= kCapBars && nh <= kCapHoles;
 }
 
-// Step 2: candidates whose bounding sphere can meet the cone from the
-// sub-cell point to the tile; writes their list indices to bar_sel/hole_sel.
+// Step 2: candidates whose box can meet a segment from the sub-cell point to
+// the tile (box_meets_cone); writes their list indices to bar_sel/hole_sel.
 __device__ void narrow_candidates(const PeLaunch& A, const CellCtx& cc, const WaveSetup& W,
                                   int lane, WaveLists& L, int nb, int nh, int& nsb, int& nsh) {
     const PeGeom& G = A.geom;
-    Cone cone;
-    make_cone(cone, cc.q, W.tc, W.rho);
+    const float q[3] = {float(cc.q[0]), float(cc.q[1]), float(cc.q[2])};
     nsb = 0;
     nsh = 0;
     for (int base = 0; base < nb; base += 64) {
@@ -228,7 +174,8 @@ __device__ void narrow_candidates(const PeLaunch& A, const CellCtx& cc, const Wa
         bool pass = false;
         if (j < nb) {
             const Bar<float>& bf = G.bars_f[L.bar_idx[j]];
-            pass = cone_hits(cone, bf.cx, bf.cy, bf.cz, 0.5f * bf.rdiag);
+            pass = box_meets_cone(q, W.tcf, W.rhof, bf.cx, bf.cy, bf.cz, bf.cb, bf.sb, 0.5f * bf.w,
+                                  0.5f * bf.t, 0.5f * bf.h);
         }
         uint64_t bal = __ballot(pass);
         if (pass) L.bar_sel[nsb + lanes_below(bal, lane)] = (uint16_t)j;
@@ -239,7 +186,8 @@ __device__ void narrow_candidates(const PeLaunch& A, const CellCtx& cc, const Wa
         bool pass = false;
         if (j < nh) {
             const Hole<float>& o = G.holes_f[L.hole_idx[j]];
-            pass = cone_hits(cone, o.xc, o.ymid, o.zc, o.crit);
+            pass = box_meets_cone(q, W.tcf, W.rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f, o.r,
+                                  0.5f * tabs(o.y2 - o.y1), o.r);
         }
         uint64_t bal = __ballot(pass);
         if (pass) L.hole_sel[nsh + lanes_below(bal, lane)] = (uint16_t)j;
@@ -276,29 +224,59 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
     using T = typename std::conditional<kExact, double, float>::type;
     KahanSum<T> acc;
     unsigned fallbacks = 0;
-    for (int c = 0; c < A.sub[2]; ++c)
-        for (int a = 0; a < A.sub[0]; ++a)
-            for (int b = 0; b < A.sub[1]; ++b) {
+    // Sub-cells in the reference's loop order (z outer, then x, then y;
+    // PESysMatGen.cu:1193-1197).  FP32 path: a ray whose decisions came
+    // within tolerance of a threshold is not added; its sub-cell goes to a
+    // per-wave queue with the ballot of such lanes, and the queue is redone
+    // in FP64 when full and at the end.  Keeping the FP64 ray out of the FP32
+    // loop keeps the loop's register use low.
+    __shared__ uint16_t redo_cell[kRedoCap];
+    __shared__ uint64_t redo_mask[kRedoCap];
+    const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
+    int cell = 0;
+    while (cell < n_cells) {
+        int n_redo = 0;
+        for (; cell < n_cells && n_redo < kRedoCap; ++cell) {
+            CellCtx cc;
+            make_cell_ctx_linear(bd, cell, A.sub, cc);
+            int nsb, nsh;
+            narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
+            __syncthreads();
+            bool near = false;
+            if (W.valid) {
+                LdsCands C{&L, nsb, nsh, lane};
+                T val = pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near);
+                if (!near) acc.add(val);
+            }
+            if constexpr (!kExact) {
+                const uint64_t m = __ballot(near);
+                if (m) {
+                    if (lane == 0) {
+                        redo_cell[n_redo] = (uint16_t)cell;
+                        redo_mask[n_redo] = m;
+                    }
+                    ++n_redo;
+                }
+            }
+            __syncthreads();
+        }
+        if constexpr (!kExact) {
+            for (int i = 0; i < n_redo; ++i) {
                 CellCtx cc;
-                make_cell_ctx(bd, a, b, c, A.sub[0], A.sub[1], A.sub[2], cc);
+                make_cell_ctx_linear(bd, redo_cell[i], A.sub, cc);
                 int nsb, nsh;
                 narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
                 __syncthreads();
-                if (W.valid) {
+                if ((redo_mask[i] >> lane) & 1ull) {
                     LdsCands C{&L, nsb, nsh, lane};
-                    bool near = false;
-                    T val = pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near);
-                    if constexpr (!kExact) {
-                        if (near) {
-                            bool n2 = false;
-                            val = (T)pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2);
-                            ++fallbacks;
-                        }
-                    }
-                    acc.add(val);
+                    bool n2 = false;
+                    acc.add((T)pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2));
+                    ++fallbacks;
                 }
                 __syncthreads();
             }
+        }
+    }
     if (W.valid) out[W.v] = float(acc.value());
     if constexpr (!kExact) {
         unsigned tot = fallbacks;
@@ -328,10 +306,73 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
+// Per-wave adjoint accumulators: per-lane sums for bar d's own mu and the
+// loss, wave sums (in LDS, added by lane 0 in a fixed order) for the layers
+// and, per tile, for each candidate bar.
+struct AdjAcc {
+    double own_mut = 0.0, own_mupe = 0.0, loss = 0.0;
+};
+
+// Adjoint contributions of one sub-cell for the lanes with `active` set.  In
+// FP32 a lane whose ray came within tolerance of a threshold contributes
+// nothing and is reported through the return value (the caller redoes it in
+// FP64).  Every lane of the wave takes part in the wave sums.
+template <class T>
+__device__ bool adjoint_cell(const PeGeom& G, int d, int lane, const WaveSetup& W,
+                             const WaveLists& L, const LdsCands& C, const CellCtx& cc,
+                             bool active, double wv, AdjAcc& acc, double* lay_acc,
+                             double* cand_acc) {
+    const GeomDesc& g = G.g;
+    const Bar<double>& bd = G.bars_d[d];
+    const double mut = bd.mut, mupe = bd.mupe;
+    double term = 0.0, dlay[kMaxLayers], aux[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < kMaxLayers; ++m) dlay[m] = 0.0;
+    bool near = false;
+    if (active) {
+        term = (double)pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near, dlay, aux);
+        if (kGuarded<T> && near) active = false;
+    }
+    const double cw = active ? wv * term : 0.0;
+    if (active) {
+        const double la = aux[2], ea = aux[3], effom = aux[0];
+        acc.loss += cw;
+        acc.own_mut += -cw * aux[1] + wv * effom * mupe * (la * ea / mut - (1.0 - ea) / (mut * mut));
+        acc.own_mupe += wv * effom * (1.0 - ea) / mut;
+    }
+#pragma unroll
+    for (int m = 0; m < kMaxLayers; ++m) {
+        if (m < g.n_layers) {
+            const double x = wave_sum(-cw * dlay[m]);
+            if (lane == 0) lay_acc[m] += x;
+        }
+    }
+    // chords through the other bars, recomputed
+    double D[3], dd;
+    SegDir<T> sw;
+    T tol_d;
+    bool nd = false;
+    ray_dir<T>(W.p, cc.q, D, dd, sw, tol_d, nd);
+    const double inv_dd = 1.0 / dd;
+    for (int j = 0; j < C.nbars(); ++j) {
+        double x = 0.0;
+        if (active && cw != 0.0 && C.bar_on(j)) {
+            bool n3 = false;
+            x = -cw * (double)other_bar_chord<T>(G, C.bar(j), W.p, D, dd, inv_dd, cc.q, sw, tol_d, n3);
+        }
+        x = wave_sum(x);
+        if (lane == 0) cand_acc[L.bar_sel[j]] += x;
+    }
+    return near;
+}
+
 template <bool kExact>
 __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScratch S) {
     __shared__ WaveLists L;
     __shared__ double cand_acc[kCapBars];
+    __shared__ double lay_acc[kMaxLayers];
+    __shared__ uint16_t redo_cell[kRedoCap];
+    __shared__ uint64_t redo_mask[kRedoCap];
     extern __shared__ double row[];  // n_det partials of the bars' mu_total
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;
@@ -342,15 +383,14 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
     const Bar<double>& bd = G.bars_d[d];
     double* out_row = S.rows + (int64_t)blockIdx.x * S.stride;
     for (int k = lane; k < g.n_det; k += 64) row[k] = 0.0;
-    double lay[kMaxLayers];
-    for (int m = 0; m < kMaxLayers; ++m) lay[m] = 0.0;
-    double own_mut = 0.0, own_mupe = 0.0, loss = 0.0;
+    if (lane < kMaxLayers) lay_acc[lane] = 0.0;
+    AdjAcc acc;
     unsigned fallbacks = 0;
     bool overflow = false;
     __syncthreads();
     if (bd.mut > (double)0.01f) {
         using T = typename std::conditional<kExact, double, float>::type;
-        const double mut = bd.mut, mupe = bd.mupe;
+        const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
         for (int tile = grp; tile < n_tiles; tile += S.groups) {
             WaveSetup W;
             wave_setup(A, d, tile, lane, W);
@@ -362,73 +402,60 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
             for (int j = lane; j < nb; j += 64) cand_acc[j] = 0.0;
             const double wv = W.valid ? (double)S.w[(int64_t)d * g.n_vox + W.v] : 0.0;
             __syncthreads();
-            for (int c = 0; c < A.sub[2]; ++c)
-                for (int a = 0; a < A.sub[0]; ++a)
-                    for (int b = 0; b < A.sub[1]; ++b) {
+            // sub-cells in the reference's order; FP32 rays near a threshold
+            // are queued and redone in FP64 (as in the forward kernel)
+            int cell = 0;
+            while (cell < n_cells) {
+                int n_redo = 0;
+                for (; cell < n_cells && n_redo < kRedoCap; ++cell) {
+                    CellCtx cc;
+                    make_cell_ctx_linear(bd, cell, A.sub, cc);
+                    int nsb, nsh;
+                    narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
+                    __syncthreads();
+                    LdsCands C{&L, nsb, nsh, lane};
+                    const bool near = adjoint_cell<T>(G, d, lane, W, L, C, cc, W.valid, wv, acc,
+                                                      lay_acc, cand_acc);
+                    if constexpr (!kExact) {
+                        const uint64_t m = __ballot(near);
+                        if (m) {
+                            if (lane == 0) {
+                                redo_cell[n_redo] = (uint16_t)cell;
+                                redo_mask[n_redo] = m;
+                            }
+                            ++n_redo;
+                        }
+                    }
+                    __syncthreads();
+                }
+                if constexpr (!kExact) {
+                    for (int i = 0; i < n_redo; ++i) {
                         CellCtx cc;
-                        make_cell_ctx(bd, a, b, c, A.sub[0], A.sub[1], A.sub[2], cc);
+                        make_cell_ctx_linear(bd, redo_cell[i], A.sub, cc);
                         int nsb, nsh;
                         narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
                         __syncthreads();
                         LdsCands C{&L, nsb, nsh, lane};
-                        double term = 0.0, dlay[kMaxLayers], aux[4] = {0, 0, 0, 0};
-                        for (int m = 0; m < kMaxLayers; ++m) dlay[m] = 0.0;
-                        bool use64 = kExact;
-                        if (W.valid) {
-                            bool near = false;
-                            term = (double)pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near, dlay, nullptr, aux);
-                            if constexpr (!kExact) {
-                                if (near) {
-                                    bool n2 = false;
-                                    term = pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2, dlay, nullptr, aux);
-                                    use64 = true;
-                                    ++fallbacks;
-                                }
-                            }
-                        }
-                        const double cw = wv * term;
-                        loss += cw;
-                        for (int m = 0; m < kMaxLayers; ++m) lay[m] -= cw * dlay[m];
-                        const double la = aux[2], ea = aux[3], effom = aux[0];
-                        own_mut += -cw * aux[1] + wv * effom * mupe * (la * ea / mut - (1.0 - ea) / (mut * mut));
-                        own_mupe += wv * effom * (1.0 - ea) / mut;
-                        // chords through the other bars, recomputed
-                        double D[3], dd;
-                        SegDir<T> sw;
-                        T tol_d;
-                        bool nd = false;
-                        ray_dir<T>(W.p, cc.q, D, dd, sw, tol_d, nd);
-                        SegDir<double> sw64;
-                        double tol64;
-                        ray_dir<double>(W.p, cc.q, D, dd, sw64, tol64, nd);
-                        for (int j = 0; j < nsb; ++j) {
-                            double x = 0.0;
-                            if (W.valid && C.bar_on(j) && cw != 0.0) {
-                                bool n3 = false;
-                                int k = C.bar(j);
-                                double l = use64 ? other_bar_chord<double>(G, k, W.p, D, dd, cc.q, sw64, tol64, n3)
-                                                 : (double)other_bar_chord<T>(G, k, W.p, D, dd, cc.q, sw, tol_d, n3);
-                                x = -cw * l;
-                            }
-                            x = wave_sum(x);
-                            if (lane == 0) cand_acc[L.bar_sel[j]] += x;
-                        }
+                        const bool mine = (redo_mask[i] >> lane) & 1ull;
+                        adjoint_cell<double>(G, d, lane, W, L, C, cc, mine, wv, acc, lay_acc, cand_acc);
+                        fallbacks += mine;
                         __syncthreads();
                     }
+                }
+            }
             // fold this tile's candidate partials into the dense row (one
             // entry per candidate, each bar at most once per tile)
             for (int j = lane; j < nb; j += 64) row[L.bar_idx[j]] += cand_acc[j];
             __syncthreads();
         }
     }
-    for (int m = 0; m < kMaxLayers; ++m) lay[m] = wave_sum(lay[m]);
-    own_mut = wave_sum(own_mut);
-    own_mupe = wave_sum(own_mupe);
-    loss = wave_sum(loss);
+    const double own_mut = wave_sum(acc.own_mut);
+    const double own_mupe = wave_sum(acc.own_mupe);
+    const double loss = wave_sum(acc.loss);
     for (int k = lane; k < g.n_det; k += 64) out_row[k] = row[k];
     if (lane == 0) {
         out_row[d] += own_mut;
-        for (int m = 0; m < g.n_layers; ++m) out_row[g.n_det + m] = lay[m];
+        for (int m = 0; m < g.n_layers; ++m) out_row[g.n_det + m] = lay_acc[m];
         out_row[g.n_det + g.n_layers] = own_mupe;
         out_row[g.n_det + g.n_layers + 1] = loss;
         if (overflow) atomicAdd(&A.counters[1], 1ull);

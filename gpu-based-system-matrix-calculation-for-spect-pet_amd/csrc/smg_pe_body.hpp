@@ -146,6 +146,99 @@ SMG_FN bool cull_hole(const PeGeom& G, int h, const double p[3], const float pf[
     return cull_test<double>(p[0], p[1], p[2], o.xc, o.ymid, o.zc, o.crit, 0.0, sc.ud, sc.crit, near);
 }
 
+// ---- candidate narrowing for a tile of voxels ------------------------------------------
+
+// Bounding ball of the voxel centres of the tile with first voxel (ix0, iy0,
+// iz0) and extent tile[] (clipped at the image edge); the rotation about z
+// preserves distances.
+SMG_FN void tile_ball(const GeomDesc& g, int ix0, int iy0, int iz0, const int tile[3], double cphi,
+                      double sphi, double tc[3], double& rho) {
+    const int ex = tmin2(tile[0], g.nx - ix0), ey = tmin2(tile[1], g.ny - iy0),
+              ez = tmin2(tile[2], g.nz - iz0);
+    double x = (ix0 + 0.5 * (ex - 1) - g.nx / 2.0 + 0.5) * g.wx + g.sx;
+    double y = (iy0 + 0.5 * (ey - 1) - g.ny / 2.0 + 0.5) * g.wy + g.sy;
+    double z = (iz0 + 0.5 * (ez - 1) - g.nz / 2.0 + 0.5) * g.wz + g.sz;
+    tc[0] = x * cphi - y * sphi;
+    tc[1] = x * sphi + y * cphi;
+    tc[2] = z;
+    rho = 0.5 * sqrt(((ex - 1) * g.wx) * ((ex - 1) * g.wx) + ((ey - 1) * g.wy) * ((ey - 1) * g.wy) +
+                     ((ez - 1) * g.wz) * ((ez - 1) * g.wz)) + 1e-6;
+}
+
+// FP32 copy of the ball for box_meets_cone, radius rounded up.
+SMG_FN void tile_ball_f(const double tc[3], double rho, float tcf[3], float& rhof) {
+    for (int k = 0; k < 3; ++k) tcf[k] = float(tc[k]);
+    rhof = float(rho) * 1.0001f;
+}
+//
+// Can a segment from a point of the box (q, rq) to a point of the ball
+// (tc, rho) meet the box of half sizes (hx, hy, hz) centred at c and turned by
+// beta about y (cos and sin in cb, sb)?  rq holds the half sizes of an
+// axis-aligned box around q in the frame of the tested box.  Such a segment is
+//   (q + e1) + s (tc + e2 - q - e1),  s in [0, 1],  |e1_i| <= rq_i,  |e2| <= rho,
+// so per axis it lies within r_i(s) = (1 - s) rq_i + s rho of the axis point
+// q + s (tc - q): the box grown by r_i(s) must meet the axis segment.  Per axis
+// that is two linear inequalities in s; the test intersects their intervals
+// with [0, 1].  The margin m covers the FP32 rounding of the inputs and the
+// direction components below 0.001 that the chord functions treat as zero (a
+// segment then drifts less than 0.001 off the box along that axis).  A false
+// answer means every chord of every such segment through the box is zero.
+SMG_FN bool box_meets_frustum(const float q[3], const float rq[3], const float tc[3], float rho,
+                              float cx, float cy, float cz, float cb, float sb, float hx, float hy,
+                              float hz) {
+    float qx = q[0] - cx, qy = q[1] - cy, qz = q[2] - cz;
+    float tx = tc[0] - cx, ty = tc[1] - cy, tz = tc[2] - cz;
+    // into the box frame (rotation by -beta about y, as to_frame)
+    float a[3] = {qx * cb + qz * sb, qy, qz * cb - qx * sb};
+    float b[3] = {tx * cb + tz * sb, ty, tz * cb - tx * sb};
+    float h[3] = {hx, hy, hz};
+    float lo = 0.0f, hi = 1.0f;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float u = b[i] - a[i];
+        float dr = rho - rq[i];
+        float m = 2e-3f + 1e-5f * (tabs(a[i]) + tabs(b[i]) + h[i] + rho + rq[i]);
+        float c0 = u - dr, r0 = h[i] + m + rq[i] - a[i];   // a + s u <= h + r(s)
+        float c1 = -u - dr, r1 = h[i] + m + rq[i] + a[i];  // -(a + s u) <= h + r(s)
+        if (c0 > 0.0f) hi = tmin2(hi, r0 / c0);
+        else if (c0 < 0.0f) lo = tmax2(lo, r0 / c0);
+        else ok &= r0 >= 0.0f;
+        if (c1 > 0.0f) hi = tmin2(hi, r1 / c1);
+        else if (c1 < 0.0f) lo = tmax2(lo, r1 / c1);
+        else ok &= r1 >= 0.0f;
+    }
+    return ok && lo <= hi + 1e-5f;
+}
+
+// Segments from the point q (a sub-cell centre) to the tile ball.
+SMG_FN bool box_meets_cone(const float q[3], const float tc[3], float rho, float cx, float cy,
+                           float cz, float cb, float sb, float hx, float hy, float hz) {
+    const float rq[3] = {0.0f, 0.0f, 0.0f};
+    return box_meets_frustum(q, rq, tc, rho, cx, cy, cz, cb, sb, hx, hy, hz);
+}
+
+// Segments from any point of bar d to the tile ball, against a box turned by
+// beta_k (cos, sin in cbk, sbk): bar d's half sizes become the axis-aligned
+// extents of bar d seen in the frame of that box.
+SMG_FN bool bar_tile_may_hit(const Bar<float>& bd, const float tc[3], float rho, float cx,
+                             float cy, float cz, float cbk, float sbk, float hx, float hy, float hz) {
+    const float cd = tabs(bd.cb * cbk + bd.sb * sbk), sd = tabs(bd.sb * cbk - bd.cb * sbk);
+    const float hdx = 0.5f * bd.w * 1.0001f, hdy = 0.5f * bd.t * 1.0001f, hdz = 0.5f * bd.h * 1.0001f;
+    const float rq[3] = {cd * hdx + sd * hdz, hdy, sd * hdx + cd * hdz};
+    const float q[3] = {bd.cx, bd.cy, bd.cz};
+    return box_meets_frustum(q, rq, tc, rho, cx, cy, cz, cbk, sbk, hx, hy, hz);
+}
+
+// a[i] = v for a run-time i < N through unrolled selects: indexing a local
+// array with a run-time value would place it in scratch memory.
+template <int N>
+SMG_FN void store_at(double* a, int i, double v) {
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        if (k == i) a[k] = v;
+}
+
 // ---- exp(-x) for x >= 0 given in FP64, result in T ---------------------------------
 // FP32: e^-x = 2^-(x log2 e) split into integer and fraction in FP64, so the
 // rounding of a large x (a ray through 5 mm of tungsten has x ~ 20) does not
@@ -182,6 +275,13 @@ SMG_FN void make_cell_ctx(const Bar<double>& bd, int a, int b, int c, int nsx, i
     cc.area[2] = bd.w * bd.t / nsx / nsy;
 }
 
+// Sub-cell number `cell` in the reference's loop order: z outermost, then x,
+// then y (PESysMatGen.cu:1193-1197).
+SMG_FN void make_cell_ctx_linear(const Bar<double>& bd, int cell, const int sub[3], CellCtx& cc) {
+    const int b = cell % sub[1], a = (cell / sub[1]) % sub[0], c = cell / (sub[1] * sub[0]);
+    make_cell_ctx(bd, a, b, c, sub[0], sub[1], sub[2], cc);
+}
+
 // Candidate lists of one lane.  Provides the bars and holes the ray must be
 // tested against; on(j) tells whether candidate j is in this element's
 // reference culling set.
@@ -193,13 +293,13 @@ SMG_FN void make_cell_ctx(const Bar<double>& bd, int a, int b, int c, int nsx, i
 // the ray closest to the bar centre, placed in FP64; FP64: the reference form.
 template <class T>
 SMG_FN T other_bar_chord(const PeGeom& G, int k, const double p[3], const double D[3], double dd,
-                         const double q[3], const SegDir<T>& sw, T tol_d, bool& near) {
+                         double inv_dd, const double q[3], const SegDir<T>& sw, T tol_d,
+                         bool& near) {
     const Bar<double>& bk = G.bars_d[k];
     if constexpr (std::is_same<T, float>::value) {
         const Bar<float>& bf = G.bars_f[k];
         double rel[3] = {bk.cx - p[0], bk.cy - p[1], bk.cz - p[2]};
         double proj = rel[0] * D[0] + rel[1] * D[1] + rel[2] * D[2];
-        double inv_dd = 1.0 / dd;
         double t0 = proj * inv_dd;
         double s0 = t0 * inv_dd;
         double o[3] = {D[0] * s0 - rel[0], D[1] * s0 - rel[1], D[2] * s0 - rel[2]};
@@ -241,7 +341,7 @@ SMG_FN void ray_dir(const double p[3], const double q[3], double D[3], double& d
 template <class T, class Cands>
 SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[3],
                      const CellCtx& cc, const Cands& C, bool& near, double* dl_layer = nullptr,
-                     double* dl_bar = nullptr, double* aux = nullptr) {
+                     double* aux = nullptr) {
     const GeomDesc& g = G.g;
     const Bar<double>& bd = G.bars_d[d];
     double D[3] = {cc.q[0] - p[0], cc.q[1] - p[1], cc.q[2] - p[2]};
@@ -290,6 +390,8 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
     }
 
     // holes and plate material, FP64 (PESysMatGen.cu:1257-1286)
+    CylRay cr;
+    cyl_ray(D, dd, cr);
     for (int m = 0; m < g.n_layers; ++m) {
         const Layer<double>& L = G.layers_d[m];
         int first = g.ranges[m].first, last = first + g.ranges[m].count;
@@ -299,36 +401,28 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
             int h = C.hole(j);
             if (h < first || h >= last || !C.hole_on(j)) continue;
             const Hole<double>& o = G.holes_d[h];
-            double l = cyl_chord(p, D, dd, o.xc, o.y1, o.y2, o.zc, o.r);
+            double l = cyl_chord(p, cr, dd, o.xc, o.y1, o.y2, o.zc, o.r);
             sum += l;
             att += o.mu * l;
         }
         double rest = in_coll - sum;
         bool counted = rest >= (double)0.00001f;
         if (counted) att += L.mu * rest;
-        if (dl_layer) dl_layer[m] = counted ? rest : 0.0;
+        if (dl_layer) store_at<kMaxLayers>(dl_layer, m, counted ? rest : 0.0);
     }
 
     // other bars on the way (PESysMatGen.cu:1290-1325)
+    const double inv_dd = 1.0 / dd;
     for (int j = 0; j < C.nbars(); ++j) {
-        if (!C.bar_on(j)) {
-            if (dl_bar) dl_bar[j] = 0.0;
-            continue;
-        }
+        if (!C.bar_on(j)) continue;
         int k = C.bar(j);
-        T l = other_bar_chord<T>(G, k, p, D, dd, cc.q, sw, tol_d, near);
-        if (dl_bar) dl_bar[j] = double(l);
+        T l = other_bar_chord<T>(G, k, p, D, dd, inv_dd, cc.q, sw, tol_d, near);
         att += double(l) * G.bars_d[k].mut;
     }
 
     // own bar from its surface to the sub-cell (PESysMatGen.cu:1329-1346), and the
     // absorption chord along the ray extended 1000 mm (PESysMatGen.cu:1350-1357)
     double Ds[3] = {cc.qs[0] - ps[0], cc.qs[1] - ps[1], cc.qs[2] - ps[2]};
-    double ext[3], De[3];
-    for (int k = 0; k < 3; ++k) {
-        ext[k] = cc.qs[k] + 1000.0 * (cc.qs[k] - ps[k]) / dd;
-        De[k] = ext[k] - ps[k];
-    }
     T l1, l2, la;
     if constexpr (std::is_same<T, float>::value) {
         const Bar<float>& bf = G.bars_f[d];
@@ -344,15 +438,17 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
         float fdd = float(dd);
         l1 = seg_box_inside_local(ss, of, fdd, lim_lo, 0.0f, blo, bhi, near);
         l2 = seg_box_inside_local(ss, of, fdd, lim_lo, 0.0f, clo, chi, near);
-        SegDir<float> se = ss;
-        double le = sqrt(De[0] * De[0] + De[1] * De[1] + De[2] * De[2]);
-        se.len = float(le);
-        for (int k = 0; k < 3; ++k) {
-            se.D[k] = float(De[k]);
-            se.inv[k] = float(le / De[k]);
-        }
-        la = seg_box_local(se, of, lim_lo, float(le - dd), clo, chi, near);
+        // The extended segment runs along the same direction with length
+        // dd + 1000: its slab ratios len / D_k equal those of ss, any
+        // component below eps is flagged through ss, and its end lies 1000 mm
+        // past the cell, far beyond any slab parameter of the cell box.
+        la = seg_box_local(ss, of, lim_lo, 1000.0f, clo, chi, near);
     } else {
+        double ext[3], De[3];
+        for (int k = 0; k < 3; ++k) {
+            ext[k] = cc.qs[k] + 1000.0 * (cc.qs[k] - ps[k]) / dd;
+            De[k] = ext[k] - ps[k];
+        }
         double blo[3] = {-bd.w / 2.0, -bd.t / 2.0, -bd.h / 2.0};
         double bhi[3] = {bd.w / 2.0, bd.t / 2.0, bd.h / 2.0};
         SegDir<double> ss, se;

@@ -347,16 +347,30 @@ SMG_FN float seg_box_inside_local(const SegDir<float>& s, const float o[3], floa
     return chord;
 }
 
+// Per-ray part of the cylinder chord: the unit direction and the ratios the
+// reference forms for every hole (identical values, formed once per ray).
+struct CylRay {
+    double kx, ky, kz, sxk, szk, qa;
+};
+
+SMG_FN void cyl_ray(const double D[3], double len, CylRay& c) {
+    c.kx = D[0] / len;
+    c.ky = D[1] / len;
+    c.kz = D[2] / len;
+    c.sxk = c.kx / c.ky;
+    c.szk = c.kz / c.ky;
+    c.qa = c.kx * c.kx + c.kz * c.kz;
+}
+
 // Chord through the finite cylinder of radius r, axis parallel to y at
 // (xc, zc), spanning y1..y2; reference length_cylinder_ray
 // (PESysMatGen.cu:688-799).  Evaluated in FP64: rays through a hole run
 // almost along its axis, where kx^2 + kz^2 -> 0 and the quadratic's roots
 // lose every FP32 digit.
-SMG_FN double cyl_chord(const double a[3], const double D[3], double len, double xc,
-                                   double y1, double y2, double zc, double r) {
+SMG_FN double cyl_chord(const double a[3], const CylRay& c, double len, double xc, double y1,
+                        double y2, double zc, double r) {
     if (fabs(y1 - y2) < kCylDy) return 0.0;
-    double kx = D[0] / len, ky = D[1] / len, kz = D[2] / len;
-    double sxk = kx / ky, szk = kz / ky;
+    const double kx = c.kx, ky = c.ky, kz = c.kz, sxk = c.sxk, szk = c.szk;
     double xl = a[0] + sxk * (y1 - a[1]) - xc;
     double zl = a[2] + szk * (y1 - a[1]) - zc;
     double xr = a[0] + sxk * (y2 - a[1]) - xc;
@@ -372,7 +386,7 @@ SMG_FN double cyl_chord(const double a[3], const double D[3], double len, double
         return fabs(t_hi - t_lo);
     }
     double x0 = a[0] - xc, z0 = a[2] - zc;
-    double qa = kx * kx + kz * kz;
+    const double qa = c.qa;
     double cr = kx * z0 - kz * x0;
     double disc = qa * r2 - cr * cr;
     if (disc <= kCylDisc) return 0.0;

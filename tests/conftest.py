@@ -85,6 +85,8 @@ def lane_lib():
     up = np.ctypeslib.ndpointer(np.uint32, flags="C")
     L.lane_pe_pairs.restype = C.c_long
     L.lane_pe_pairs.argtypes = [fp, fp, fp, fp] + [C.c_int] * 5 + [ip, ip, C.c_int, dp]
+    L.lane_narrow_check.restype = C.c_int
+    L.lane_narrow_check.argtypes = [fp, fp, fp, fp] + [C.c_int] * 8 + [C.c_uint, dp]
     L.lane_scatter_pairs.restype = C.c_long
     L.lane_scatter_pairs.argtypes = [fp, fp, fp, fp, fp, up] + [C.c_int] * 7 + [ip, ip, C.c_int, dp]
     return L
@@ -96,3 +98,29 @@ def smg_mod():
     import smg
     smg.lib()
     return smg
+
+
+def small_geometry(params_mod, beta=0.0, layers=1, nx=13, ny=11, nz=3, dphi=0.0, n_rot=1):
+    """A small geometry exercising rotated bars, several collimator layers and
+    ragged voxel tiles (13 x 11 x 3 does not divide into 64-voxel tiles)."""
+    p = params_mod.Params.empty()
+    p.coll[0] = layers
+    rng = np.random.default_rng(11)
+    h = 0
+    for m in range(layers):
+        nh = 9
+        params_mod.set_layer(p, m, nh, 40.0, 3.0, 40.0, 6.0 * m, 3.65, 3.3, 0.35)
+        for i in range(nh):
+            x, z = rng.uniform(-6, 6, 2)
+            params_mod.set_hole(p, h, x, 6.0 * m - 1.5, 6.0 * m + 1.5, z, rng.uniform(0.4, 0.9))
+            h += 1
+    p.det[0] = 18
+    for k in range(18):
+        ix, iz = k % 6, k // 6
+        params_mod.set_bar(p, k, -5.0 + 2.0 * ix, 12.0 + 6.0 * (layers - 1) + (k % 2) * 0.5,
+                           -2.0 + 2.0 * iz,
+                           1.8, 6.0, 1.9, 0.25, 0.18, 0.07, 0.1, beta * (1 + 0.1 * k))
+    params_mod.set_image(p, nx, ny, nz, 0.8, 0.9, 1.1, n_rot=n_rot, dphi=dphi,
+                         shift=(0.3, -0.2, 0.1), fov2coll=25.0)
+    params_mod.set_physics(p)
+    return p

@@ -87,9 +87,147 @@ double element(const Ctx& c, int d, int v, int rot, int sx, int sy, int sz, long
     return (double)(float)acc.value();
 }
 
+// Candidate lists narrowed for one sub-cell: a wave-uniform list of bars and
+// holes with per-lane masks, as the GPU kernel holds them in LDS.
+struct MaskCands {
+    std::vector<int> b, h;
+    std::vector<uint64_t> bm, hm;
+    int lane = 0;
+    int nbars() const { return (int)b.size(); }
+    int bar(int j) const { return b[j]; }
+    bool bar_on(int j) const { return (bm[j] >> lane) & 1ull; }
+    int nholes() const { return (int)h.size(); }
+    int hole(int j) const { return h[j]; }
+    bool hole_on(int j) const { return (hm[j] >> lane) & 1ull; }
+};
+
 }  // namespace
 
 extern "C" {
+
+// Checks the wave-level candidate selection (bar_tile_may_hit per tile, then
+// box_meets_cone per sub-cell) on n_waves seeded (bar, tile) pairs: every lane's FP64 sub-cell term
+// with the narrowed lists must equal, bit for bit, the term with the lane's
+// full culling set.  stats[0] = mismatching (lane, sub-cell) terms,
+// stats[1] = terms checked, stats[2] = mean narrowed bars per sub-cell,
+// stats[3] = mean narrowed holes, stats[4] = mean bars in the full sets,
+// stats[5] = mean tile candidates (bars) kept by a culling test of some lane.
+int lane_narrow_check(const float* coll, const float* det, const float* img, const float* phys,
+                      int rot, int sx, int sy, int sz, int tx, int ty, int tz, int n_waves,
+                      unsigned seed, double* stats) {
+    Ctx c;
+    if (!make_ctx(coll, det, img, phys, c)) return -1;
+    const PeGeom& G = c.G;
+    const GeomDesc& g = G.g;
+    const int tile[3] = {tx, ty, tz};
+    const int nt[3] = {(g.nx + tx - 1) / tx, (g.ny + ty - 1) / ty, (g.nz + tz - 1) / tz};
+    const double phi = (double)rot * g.dphi, cphi = std::cos(phi), sphi = std::sin(phi);
+    uint64_t st = seed * 2654435761ull + 1;
+    auto next = [&]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return st;
+    };
+    double bad = 0, checked = 0, nsb = 0, nsh = 0, nfull = 0, cells = 0, ntile = 0, nwav = 0;
+    for (int w = 0; w < n_waves; ++w) {
+        int d = (int)(next() % g.n_det);
+        int t = (int)(next() % ((uint64_t)nt[0] * nt[1] * nt[2]));
+        int ix0 = (t % nt[0]) * tx, iy0 = ((t / nt[0]) % nt[1]) * ty, iz0 = (t / (nt[0] * nt[1])) * tz;
+        const Bar<double>& bd = G.bars_d[d];
+        if (!(bd.mut > (double)0.01f)) continue;
+        nwav += 1;
+        double tc[3], rho;
+        tile_ball(g, ix0, iy0, iz0, tile, cphi, sphi, tc, rho);
+        float tcf[3], rhof;
+        tile_ball_f(tc, rho, tcf, rhof);
+        // full culling sets (every bar and hole), and the kernel's tile
+        // candidates (bar_tile_may_hit first, then the culling test)
+        const Bar<float>& bdf = G.bars_f[d];
+        std::vector<char> tb(g.n_det, 0), th(g.n_holes, 0);
+        for (int k = 0; k < g.n_det; ++k) {
+            const Bar<float>& b = G.bars_f[k];
+            tb[k] = k != d && bar_tile_may_hit(bdf, tcf, rhof, b.cx, b.cy, b.cz, b.cb, b.sb,
+                                               0.5f * b.w, 0.5f * b.t, 0.5f * b.h);
+        }
+        for (int h = 0; h < g.n_holes; ++h) {
+            const Hole<float>& o = G.holes_f[h];
+            th[h] = bar_tile_may_hit(bdf, tcf, rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f, o.r,
+                                     0.5f * tabs(o.y2 - o.y1), o.r);
+        }
+        std::vector<uint64_t> bmask(g.n_det, 0), hmask(g.n_holes, 0);
+        double P[64][3], PS[64][3];
+        bool valid[64];
+        for (int l = 0; l < tx * ty * tz; ++l) {
+            int ix = ix0 + l % tx, iy = iy0 + (l / tx) % ty, iz = iz0 + l / (tx * ty);
+            valid[l] = ix < g.nx && iy < g.ny && iz < g.nz;
+            if (!valid[l]) continue;
+            voxel_position(g, ix, iy, iz, cphi, sphi, P[l]);
+            to_frame(bd, P[l], PS[l]);
+            SelfCull sc;
+            self_cull(bd, P[l], sc);
+            float pf[3] = {float(P[l][0]), float(P[l][1]), float(P[l][2])};
+            float udf[3] = {float(sc.ud[0]), float(sc.ud[1]), float(sc.ud[2])};
+            for (int k = 0; k < g.n_det; ++k)
+                if (k != d && cull_bar<true>(G, k, P[l], pf, sc, udf, float(sc.crit))) {
+                    bmask[k] |= 1ull << l;
+                    nfull += 1;
+                }
+            for (int h = 0; h < g.n_holes; ++h)
+                if (cull_hole<true>(G, h, P[l], pf, sc, udf, float(sc.crit))) hmask[h] |= 1ull << l;
+        }
+        for (int k = 0; k < g.n_det; ++k) ntile += tb[k] && bmask[k];
+        for (int cz = 0; cz < sz; ++cz)
+            for (int a = 0; a < sx; ++a)
+                for (int b = 0; b < sy; ++b) {
+                    CellCtx cc;
+                    make_cell_ctx(bd, a, b, cz, sx, sy, sz, cc);
+                    const float q[3] = {float(cc.q[0]), float(cc.q[1]), float(cc.q[2])};
+                    MaskCands full, nar;
+                    for (int k = 0; k < g.n_det; ++k) {
+                        if (!bmask[k]) continue;
+                        full.b.push_back(k);
+                        full.bm.push_back(bmask[k]);
+                        const Bar<float>& bf = G.bars_f[k];
+                        if (tb[k] && box_meets_cone(q, tcf, rhof, bf.cx, bf.cy, bf.cz, bf.cb, bf.sb, 0.5f * bf.w,
+                                           0.5f * bf.t, 0.5f * bf.h)) {
+                            nar.b.push_back(k);
+                            nar.bm.push_back(bmask[k]);
+                        }
+                    }
+                    for (int h = 0; h < g.n_holes; ++h) {
+                        if (!hmask[h]) continue;
+                        full.h.push_back(h);
+                        full.hm.push_back(hmask[h]);
+                        const Hole<float>& o = G.holes_f[h];
+                        if (th[h] && box_meets_cone(q, tcf, rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f, o.r,
+                                           0.5f * tabs(o.y2 - o.y1), o.r)) {
+                            nar.h.push_back(h);
+                            nar.hm.push_back(hmask[h]);
+                        }
+                    }
+                    nsb += nar.b.size();
+                    nsh += nar.h.size();
+                    cells += 1;
+                    for (int l = 0; l < tx * ty * tz; ++l) {
+                        if (!valid[l]) continue;
+                        full.lane = nar.lane = l;
+                        bool n1 = false, n2 = false;
+                        double v1 = pe_cell_ray<double>(G, d, P[l], PS[l], cc, full, n1);
+                        double v2 = pe_cell_ray<double>(G, d, P[l], PS[l], cc, nar, n2);
+                        checked += 1;
+                        if (!(v1 == v2)) bad += 1;
+                    }
+                }
+    }
+    stats[0] = bad;
+    stats[1] = checked;
+    stats[2] = cells > 0 ? nsb / cells : 0;
+    stats[3] = cells > 0 ? nsh / cells : 0;
+    stats[4] = checked > 0 ? nfull / (checked / (sx * sy * sz)) : 0;
+    stats[5] = nwav > 0 ? ntile / nwav : 0;
+    return 0;
+}
 
 // out[i] = A[d[i]][v[i]] of rotation rot; exact = 1 runs everything in FP64.
 // Returns the number of FP64 fallback rays, or -1 on a geometry error.

@@ -4,7 +4,7 @@ PESysMatGen.cu:1368-1437): one call per rotation, [N][V] float32 block."""
 import numpy as np
 import pytest
 
-from conftest import TOL_FP32, TOL_FP64, assert_parity
+from conftest import TOL_FP32, TOL_FP64, assert_parity, small_geometry
 
 pytestmark = pytest.mark.gpu
 
@@ -44,36 +44,10 @@ def test_cfg2_sampled(smg_mod, oracle_mod, params_mod, prec, tol):
     assert_parity(got[d, v], ref, tol, f"cfg2 PE {prec}")
 
 
-def _small_geometry(params_mod, beta=0.0, layers=1, nx=13, ny=11, nz=3, dphi=0.0, n_rot=1):
-    """A small geometry exercising rotated bars, several collimator layers and
-    ragged voxel tiles (13 x 11 x 3 does not divide into 64-voxel tiles)."""
-    p = params_mod.Params.empty()
-    p.coll[0] = layers
-    rng = np.random.default_rng(11)
-    h = 0
-    for m in range(layers):
-        nh = 9
-        params_mod.set_layer(p, m, nh, 40.0, 3.0, 40.0, 6.0 * m, 3.65, 3.3, 0.35)
-        for i in range(nh):
-            x, z = rng.uniform(-6, 6, 2)
-            params_mod.set_hole(p, h, x, 6.0 * m - 1.5, 6.0 * m + 1.5, z, rng.uniform(0.4, 0.9))
-            h += 1
-    p.det[0] = 18
-    for k in range(18):
-        ix, iz = k % 6, k // 6
-        params_mod.set_bar(p, k, -5.0 + 2.0 * ix, 12.0 + 6.0 * (layers - 1) + (k % 2) * 0.5,
-                           -2.0 + 2.0 * iz,
-                           1.8, 6.0, 1.9, 0.25, 0.18, 0.07, 0.1, beta * (1 + 0.1 * k))
-    params_mod.set_image(p, nx, ny, nz, 0.8, 0.9, 1.1, n_rot=n_rot, dphi=dphi,
-                         shift=(0.3, -0.2, 0.1), fov2coll=25.0)
-    params_mod.set_physics(p)
-    return p
-
-
 @pytest.mark.parametrize("beta,layers", [(0.0, 1), (0.07, 1), (0.0, 2), (0.05, 3)])
 @pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
 def test_edge_geometries(smg_mod, oracle_mod, params_mod, beta, layers, prec, tol):
-    p = _small_geometry(params_mod, beta=beta, layers=layers)
+    p = small_geometry(params_mod, beta=beta, layers=layers)
     sub = (3, 4, 2)
     ref = oracle_mod.Oracle(p).pe_matrix(sub=sub)
     with _ctx(smg_mod, p, sub) as ctx:
@@ -84,7 +58,7 @@ def test_edge_geometries(smg_mod, oracle_mod, params_mod, beta, layers, prec, to
 @pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
 def test_rotations(smg_mod, oracle_mod, params_mod, prec, tol):
     """Rotation r turns the voxel grid by r * dphi about z (PESysMatGen.cu:1004-1053)."""
-    p = _small_geometry(params_mod, nx=9, ny=9, nz=2, dphi=0.3, n_rot=3)
+    p = small_geometry(params_mod, nx=9, ny=9, nz=2, dphi=0.3, n_rot=3)
     sub = (2, 3, 2)
     o = oracle_mod.Oracle(p)
     with _ctx(smg_mod, p, sub) as ctx:

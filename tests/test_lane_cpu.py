@@ -4,7 +4,7 @@ must stay within 1e-5, the FP64 path within float output rounding."""
 import numpy as np
 import pytest
 
-from conftest import TOL_FP32, TOL_FP64, assert_parity
+from conftest import TOL_FP32, TOL_FP64, assert_parity, small_geometry
 
 
 def lane_pe(L, p, d, v, sub, exact, rot=0):
@@ -48,3 +48,29 @@ def test_lane_scatter(lane_lib, oracle_mod, params_mod, kind, variant, exact):
     assert (ref > 0).any()
     assert_parity(out, ref, TOL_FP64 if exact else TOL_FP32,
                   f"lane scatter kind={kind} variant={variant} exact={exact} fallbacks={fb}")
+
+
+@pytest.mark.parametrize("case", ["cfg1", "cfg2", "rotated", "layers3", "rot2"])
+def test_lane_narrowing_is_exact(lane_lib, params_mod, case):
+    """The wave-level candidate selection (bar_tile_may_hit per tile,
+    box_meets_cone per sub-cell) drops only candidates whose chord is zero for
+    every lane of the tile: the FP64 sub-cell terms with the narrowed lists
+    equal the terms with each lane's full culling set bit for bit."""
+    rot = 0
+    if case == "cfg1":
+        p, sub, tile, waves = params_mod.config(1), (4, 4, 4), (8, 8, 1), 24
+    elif case == "cfg2":
+        p, sub, tile, waves = params_mod.config(2), (3, 3, 3), (4, 4, 4), 6
+    elif case == "rotated":
+        p, sub, tile, waves = small_geometry(params_mod, beta=0.07), (3, 4, 2), (8, 4, 2), 40
+    elif case == "layers3":
+        p, sub, tile, waves = small_geometry(params_mod, beta=0.05, layers=3), (3, 4, 2), (8, 4, 2), 40
+    else:
+        p, sub, tile, waves = small_geometry(params_mod, nx=9, ny=9, nz=2, dphi=0.3, n_rot=3), (2, 3, 2), (8, 4, 2), 40
+        rot = 2
+    st = np.zeros(6)
+    assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, rot, *sub, *tile, waves, 5, st) == 0
+    print(f"narrowing {case}: {int(st[1])} terms, {st[2]:.2f} bars and {st[3]:.2f} holes per "
+          f"sub-cell (full sets {st[4]:.1f} bars, tile candidates {st[5]:.1f})")
+    assert st[1] > 0
+    assert st[0] == 0
