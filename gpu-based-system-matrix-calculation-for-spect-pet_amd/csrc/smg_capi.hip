@@ -62,6 +62,7 @@ struct smg_ctx {
     DevBuf<Hole<double>> holes_d;
     DevBuf<Layer<float>> layers_f;
     DevBuf<Layer<double>> layers_d;
+    DevBuf<GroupBox> bar_groups, hole_groups;
     DevBuf<float> out;        // staging for host outputs
     DevBuf<float> in;         // staging for host inputs (PE weights)
     DevBuf<unsigned long long> counters;
@@ -116,6 +117,8 @@ PeGeom device_geom(smg_ctx* c) {
     G.holes_d = c->holes_d.ptr;
     G.layers_f = c->layers_f.ptr;
     G.layers_d = c->layers_d.ptr;
+    G.bar_groups = c->bar_groups.ptr;
+    G.hole_groups = c->hole_groups.ptr;
     return G;
 }
 
@@ -388,6 +391,8 @@ smg_status smg_destroy(smg_ctx* c) {
     c->holes_d.release();
     c->layers_f.release();
     c->layers_d.release();
+    c->bar_groups.release();
+    c->hole_groups.release();
     c->out.release();
     c->in.release();
     c->counters.release();
@@ -442,6 +447,8 @@ smg_status smg_set_geometry(smg_ctx* c, const float* coll, size_t n_coll, const 
     if ((s = upload(c, c->holes_d, hg.holes_d)) != SMG_OK) return s;
     if ((s = upload(c, c->layers_f, hg.layers_f)) != SMG_OK) return s;
     if ((s = upload(c, c->layers_d, hg.layers_d)) != SMG_OK) return s;
+    if ((s = upload(c, c->bar_groups, hg.bar_groups)) != SMG_OK) return s;
+    if ((s = upload(c, c->hole_groups, hg.hole_groups)) != SMG_OK) return s;
     std::vector<Sphere<float>> sf;
     std::vector<Sphere<double>> sd, hd;
     std::vector<Sphere<float>> hf;

@@ -25,7 +25,8 @@ struct Bar {
     T cb, sb;       // cos(beta), sin(beta); beta = rotation about y
     T rdiag;        // sqrt(w^2 + h^2 + t^2): culling uses rdiag / (2 D)
     T beta;
-    T pad0, pad1;
+    T pe_frac;      // mu_pe / mu_total
+    T pad1;
 };
 
 template <class T>
@@ -55,6 +56,14 @@ struct Sphere {
 template <class T>
 struct ScatterPt {
     T x, y, z, dy;
+};
+
+// Axis-aligned world box (centre, half sizes) around kGroup consecutive bars
+// or holes: the candidate scan of the PE kernel tests a group's box before its
+// members.
+constexpr int kGroup = 64;
+struct GroupBox {
+    float c[3], h[3];
 };
 
 struct LayerRange {

@@ -7,6 +7,7 @@
 // tests of the lane arithmetic.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -27,6 +28,7 @@ struct HostGeom {
     std::vector<Hole<double>> holes_d;
     std::vector<Layer<float>> layers_f;
     std::vector<Layer<double>> layers_d;
+    std::vector<GroupBox> bar_groups, hole_groups;
     int n_rot = 1;
     int n_holes_l0 = 0;
     std::vector<float> coll, det, img, phys;  // raw arrays (first 80000/80000/100/100)
@@ -44,6 +46,44 @@ inline int perr(std::string& why, int code, const char* fmt, ...) {
     va_end(ap);
     why = buf;
     return code;
+}
+
+// World boxes around each run of kGroup bars (turned by beta about y) and
+// holes (cylinders along y), padded by 0.01 for the FP32 rounding of centres.
+inline void make_group_boxes(HostGeom& hg) {
+    auto build = [](int n, auto box_of, std::vector<GroupBox>& out) {
+        out.assign((n + kGroup - 1) / kGroup, GroupBox{});
+        for (int gi = 0; gi < (int)out.size(); ++gi) {
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int i = gi * kGroup; i < n && i < (gi + 1) * kGroup; ++i) {
+                double c[3], h[3];
+                box_of(i, c, h);
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = std::min(lo[k], c[k] - h[k]);
+                    hi[k] = std::max(hi[k], c[k] + h[k]);
+                }
+            }
+            for (int k = 0; k < 3; ++k) {
+                out[gi].c[k] = (float)(0.5 * (lo[k] + hi[k]));
+                out[gi].h[k] = (float)(0.5 * (hi[k] - lo[k]) + 0.01 + 1e-5 * std::fabs(0.5 * (lo[k] + hi[k])));
+            }
+        }
+    };
+    build(hg.g.n_det, [&](int i, double c[3], double h[3]) {
+        const Bar<double>& b = hg.bars_d[i];
+        const double ac = std::fabs(b.cb), as = std::fabs(b.sb);
+        c[0] = b.cx; c[1] = b.cy; c[2] = b.cz;
+        h[0] = ac * 0.5 * b.w + as * 0.5 * b.h;
+        h[1] = 0.5 * b.t;
+        h[2] = as * 0.5 * b.w + ac * 0.5 * b.h;
+    }, hg.bar_groups);
+    build(hg.g.n_holes, [&](int i, double c[3], double h[3]) {
+        const Hole<double>& o = hg.holes_d[i];
+        c[0] = o.xc; c[1] = o.ymid; c[2] = o.zc;
+        h[0] = o.r;
+        h[1] = 0.5 * std::fabs(o.y2 - o.y1);
+        h[2] = o.r;
+    }, hg.hole_groups);
 }
 
 // Parse the four parameter arrays the way the reference reads them.
@@ -158,6 +198,8 @@ inline int parse_geometry(const float* coll, const float* det, const float* img,
         B.cb = std::cos(B.beta);
         B.sb = std::sin(B.beta);
         B.rdiag = std::sqrt(B.w * B.w + B.h * B.h + B.t * B.t);
+        B.pe_frac = B.mut != 0.0 ? B.mupe / B.mut : 0.0;
+        B.pad1 = 0.0;
         if (!(B.w > 0 && B.t > 0 && B.h > 0))
             return perr(why, kBadParam, "bar %d has a non-positive size", k);
         Bar<float>& Bf = hg.bars_f[k];
@@ -175,8 +217,10 @@ inline int parse_geometry(const float* coll, const float* det, const float* img,
         Bf.sb = (float)B.sb;
         Bf.rdiag = (float)B.rdiag;
         Bf.beta = (float)B.beta;
-        Bf.pad0 = Bf.pad1 = 0.0f;
+        Bf.pe_frac = (float)B.pe_frac;
+        Bf.pad1 = 0.0f;
     }
+    make_group_boxes(hg);
     return 0;
 }
 

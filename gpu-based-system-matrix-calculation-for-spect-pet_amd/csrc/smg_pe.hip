@@ -98,10 +98,11 @@ __device__ __forceinline__ void wave_setup(const PeLaunch& A, int d, int tile, i
 }
 
 // Step 1: candidate bars and holes of the tile: those whose box a segment from
-// bar d to the tile can meet (bar_tile_may_hit; any other has chord 0 for
-// every ray of the tile, so its culling decision does not matter), each with
-// the ballot of the lanes whose reference culling test keeps it.  Returns
-// false on overflow.
+// bar d to the tile can meet (bar_tile_may_hit, first on the box of each
+// group of kGroup consecutive bars or holes, then on the members of the
+// groups it keeps; any other has chord 0 for every ray of the tile, so its
+// culling decision does not matter), each with the ballot of the lanes whose
+// reference culling test keeps it.  Returns false on overflow.
 template <bool kExact>
 __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveSetup& W,
                                  WaveLists& L, int& nb, int& nh) {
@@ -110,55 +111,92 @@ __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveS
     const Bar<float>& bdf = G.bars_f[d];
     nb = 0;
     nh = 0;
-    for (int base = 0; base < g.n_det; base += 64) {
-        int k = base + lane;
-        bool pass = false;
-        if (k < g.n_det && k != d) {
-            const Bar<float>& b = G.bars_f[k];
-            pass = bar_tile_may_hit(bdf, W.tcf, W.rhof, b.cx, b.cy, b.cz, b.cb, b.sb, 0.5f * b.w,
-                                    0.5f * b.t, 0.5f * b.h);
+    // groups of kGroup bars first (one group box per lane), then the members
+    // of each group that can be hit
+    const int n_bg = (g.n_det + kGroup - 1) / kGroup;
+    for (int g0 = 0; g0 < n_bg; g0 += 64) {
+        bool gpass = false;
+        if (g0 + lane < n_bg) {
+            const GroupBox& B = G.bar_groups[g0 + lane];
+            gpass = bar_tile_may_hit(bdf, W.tcf, W.rhof, B.c[0], B.c[1], B.c[2], 1.0f, 0.0f, B.h[0],
+                                     B.h[1], B.h[2]);
         }
-        uint64_t bal = __ballot(pass);
-        while (bal) {
-            int bit = __ffsll((unsigned long long)bal) - 1;
-            bal &= bal - 1;
-            int kk = base + bit;
-            bool keep = W.valid && cull_bar<kExact>(G, kk, W.p, W.pf, W.sc, W.udf, W.critf);
-            uint64_t m = __ballot(keep);
-            if (m) {
-                if (nb < kCapBars && lane == 0) {
-                    L.bar_idx[nb] = (uint16_t)kk;
-                    L.bar_mask[nb] = m;
+        uint64_t gbal = __ballot(gpass);
+        while (gbal) {
+            const int base = (g0 + __ffsll((unsigned long long)gbal) - 1) * kGroup;
+            gbal &= gbal - 1;
+            int k = base + lane;
+            bool pass = false;
+            if (k < g.n_det && k != d) {
+                const Bar<float>& b = G.bars_f[k];
+                pass = bar_tile_may_hit(bdf, W.tcf, W.rhof, b.cx, b.cy, b.cz, b.cb, b.sb, 0.5f * b.w,
+                                        0.5f * b.t, 0.5f * b.h);
+            }
+            uint64_t bal = __ballot(pass);
+            while (bal) {
+                int bit = __ffsll((unsigned long long)bal) - 1;
+                bal &= bal - 1;
+                int kk = base + bit;
+                bool keep = W.valid && cull_bar<kExact>(G, kk, W.p, W.pf, W.sc, W.udf, W.critf);
+                uint64_t m = __ballot(keep);
+                if (m) {
+                    if (nb < kCapBars && lane == 0) {
+                        L.bar_idx[nb] = (uint16_t)kk;
+                        L.bar_mask[nb] = m;
+                    }
+                    ++nb;
                 }
-                ++nb;
             }
         }
     }
-    for (int base = 0; base < g.n_holes; base += 64) {
-        int h = base + lane;
-        bool pass = false;
-        if (h < g.n_holes) {
-            const Hole<float>& o = G.holes_f[h];
-            pass = bar_tile_may_hit(bdf, W.tcf, W.rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f, o.r,
-                                    0.5f * tabs(o.y2 - o.y1), o.r);
+    const int n_hg = (g.n_holes + kGroup - 1) / kGroup;
+    for (int g0 = 0; g0 < n_hg; g0 += 64) {
+        bool gpass = false;
+        if (g0 + lane < n_hg) {
+            const GroupBox& B = G.hole_groups[g0 + lane];
+            gpass = bar_tile_may_hit(bdf, W.tcf, W.rhof, B.c[0], B.c[1], B.c[2], 1.0f, 0.0f, B.h[0],
+                                     B.h[1], B.h[2]);
         }
-        uint64_t bal = __ballot(pass);
-        while (bal) {
-            int bit = __ffsll((unsigned long long)bal) - 1;
-            bal &= bal - 1;
-            int hh = base + bit;
-            bool keep = W.valid && cull_hole<kExact>(G, hh, W.p, W.pf, W.sc, W.udf, W.critf);
-            uint64_t m = __ballot(keep);
-            if (m) {
-                if (nh < kCapHoles && lane == 0) {
-                    L.hole_idx[nh] = (uint16_t)hh;
-                    L.hole_mask[nh] = m;
+        uint64_t gbal = __ballot(gpass);
+        while (gbal) {
+            const int base = (g0 + __ffsll((unsigned long long)gbal) - 1) * kGroup;
+            gbal &= gbal - 1;
+            int h = base + lane;
+            bool pass = false;
+            if (h < g.n_holes) {
+                const Hole<float>& o = G.holes_f[h];
+                pass = bar_tile_may_hit(bdf, W.tcf, W.rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f, o.r,
+                                        0.5f * tabs(o.y2 - o.y1), o.r);
+            }
+            uint64_t bal = __ballot(pass);
+            while (bal) {
+                int bit = __ffsll((unsigned long long)bal) - 1;
+                bal &= bal - 1;
+                int hh = base + bit;
+                bool keep = W.valid && cull_hole<kExact>(G, hh, W.p, W.pf, W.sc, W.udf, W.critf);
+                uint64_t m = __ballot(keep);
+                if (m) {
+                    if (nh < kCapHoles && lane == 0) {
+                        L.hole_idx[nh] = (uint16_t)hh;
+                        L.hole_mask[nh] = m;
+                    }
+                    ++nh;
                 }
-                ++nh;
             }
         }
     }
     return nb <= kCapBars && nh <= kCapHoles;
+}
+
+// Sub-cell `cell` of bar d for this wave's tile, with the plate gates that no
+// ray of the tile can trigger (one layer per lane, then a ballot).
+__device__ __forceinline__ void tile_cell(const PeLaunch& A, const Bar<double>& bd, int cell,
+                                          const WaveSetup& W, int lane, CellCtx& cc) {
+    make_cell_ctx_linear(bd, cell, A.sub, cc);
+    const GeomDesc& g = A.geom.g;
+    bool clear = lane < g.n_layers && plate_gate_clear(A.geom.layers_d[lane < g.n_layers ? lane : 0],
+                                                       cc.q, W.tc, W.rho);
+    cc.gate_clear = (uint32_t)__ballot(clear);
 }
 
 // Step 2: candidates whose box can meet a segment from the sub-cell point to
@@ -238,7 +276,7 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
         int n_redo = 0;
         for (; cell < n_cells && n_redo < kRedoCap; ++cell) {
             CellCtx cc;
-            make_cell_ctx_linear(bd, cell, A.sub, cc);
+            tile_cell(A, bd, cell, W, lane, cc);
             int nsb, nsh;
             narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
             __syncthreads();
@@ -263,7 +301,7 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
         if constexpr (!kExact) {
             for (int i = 0; i < n_redo; ++i) {
                 CellCtx cc;
-                make_cell_ctx_linear(bd, redo_cell[i], A.sub, cc);
+                tile_cell(A, bd, redo_cell[i], W, lane, cc);
                 int nsb, nsh;
                 narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
                 __syncthreads();
@@ -409,7 +447,7 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                 int n_redo = 0;
                 for (; cell < n_cells && n_redo < kRedoCap; ++cell) {
                     CellCtx cc;
-                    make_cell_ctx_linear(bd, cell, A.sub, cc);
+                    tile_cell(A, bd, cell, W, lane, cc);
                     int nsb, nsh;
                     narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
                     __syncthreads();
@@ -431,7 +469,7 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                 if constexpr (!kExact) {
                     for (int i = 0; i < n_redo; ++i) {
                         CellCtx cc;
-                        make_cell_ctx_linear(bd, redo_cell[i], A.sub, cc);
+                        tile_cell(A, bd, redo_cell[i], W, lane, cc);
                         int nsb, nsh;
                         narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
                         __syncthreads();

@@ -31,6 +31,8 @@ struct PeGeom {
     const Hole<double>* holes_d;
     const Layer<float>* layers_f;
     const Layer<double>* layers_d;
+    const GroupBox* bar_groups;
+    const GroupBox* hole_groups;
 };
 
 // Voxel centre, FOV shift, rotation about z by rot * dphi
@@ -107,10 +109,18 @@ template <class T>
 SMG_FN bool cull_test(T px, T py, T pz, T cx, T cy, T cz, T crit_num, T half, const T ud[3],
                       T crit_self, bool& near) {
     T dx = px - cx, dy = py - cy, dz = pz - cz;
-    T dk = sqrt(dx * dx + dy * dy + dz * dz);
-    T crit = half ? crit_num / dk / T(2) : crit_num / dk;
-    T ux = (cx - px) / dk, uy = (cy - py) / dk, uz = (cz - pz) / dk;
-    T s = sqrt((uz - ud[2]) * (uz - ud[2]) + (uy - ud[1]) * (uy - ud[1]) + (ux - ud[0]) * (ux - ud[0]));
+    T dk = tsqrt(dx * dx + dy * dy + dz * dz);
+    if constexpr (std::is_same<T, double>::value) {  // the reference's operation order
+        T crit = half ? crit_num / dk / T(2) : crit_num / dk;
+        T ux = (cx - px) / dk, uy = (cy - py) / dk, uz = (cz - pz) / dk;
+        T s = sqrt((uz - ud[2]) * (uz - ud[2]) + (uy - ud[1]) * (uy - ud[1]) + (ux - ud[0]) * (ux - ud[0]));
+        T lim = crit + crit_self;
+        return g_le(s, lim, T(64) * kUnit<T> * (s + lim + T(1)), near);
+    }
+    const T rk = trcp(dk);
+    T crit = half ? crit_num * rk * T(0.5) : crit_num * rk;
+    T ux = -dx * rk, uy = -dy * rk, uz = -dz * rk;
+    T s = tsqrt((uz - ud[2]) * (uz - ud[2]) + (uy - ud[1]) * (uy - ud[1]) + (ux - ud[0]) * (ux - ud[0]));
     T lim = crit + crit_self;
     return g_le(s, lim, T(64) * kUnit<T> * (s + lim + T(1)), near);
 }
@@ -201,11 +211,11 @@ SMG_FN bool box_meets_frustum(const float q[3], const float rq[3], const float t
         float m = 2e-3f + 1e-5f * (tabs(a[i]) + tabs(b[i]) + h[i] + rho + rq[i]);
         float c0 = u - dr, r0 = h[i] + m + rq[i] - a[i];   // a + s u <= h + r(s)
         float c1 = -u - dr, r1 = h[i] + m + rq[i] + a[i];  // -(a + s u) <= h + r(s)
-        if (c0 > 0.0f) hi = tmin2(hi, r0 / c0);
-        else if (c0 < 0.0f) lo = tmax2(lo, r0 / c0);
+        if (c0 > 0.0f) hi = tmin2(hi, tdiv(r0, c0));
+        else if (c0 < 0.0f) lo = tmax2(lo, tdiv(r0, c0));
         else ok &= r0 >= 0.0f;
-        if (c1 > 0.0f) hi = tmin2(hi, r1 / c1);
-        else if (c1 < 0.0f) lo = tmax2(lo, r1 / c1);
+        if (c1 > 0.0f) hi = tmin2(hi, tdiv(r1, c1));
+        else if (c1 < 0.0f) lo = tmax2(lo, tdiv(r1, c1));
         else ok &= r1 >= 0.0f;
     }
     return ok && lo <= hi + 1e-5f;
@@ -263,7 +273,9 @@ struct CellCtx {
     double q[3];      // sub-cell centre, world
     double qs[3];     // sub-cell centre, frame of bar d
     double lo[3], hi[3];  // sub-cell box, frame of bar d
-    double area[3];   // projected cell-face areas over 4 pi: x, y, z faces
+    double area[3];   // projected cell-face areas: x, y, z faces
+    float hc[3];      // sub-cell half sizes
+    uint32_t gate_clear = 0;  // bit m: no ray of the tile can be stopped by the gate of layer m
 };
 
 SMG_FN void make_cell_ctx(const Bar<double>& bd, int a, int b, int c, int nsx, int nsy, int nsz,
@@ -273,6 +285,42 @@ SMG_FN void make_cell_ctx(const Bar<double>& bd, int a, int b, int c, int nsx, i
     cc.area[0] = bd.t * bd.h / nsz / nsy;
     cc.area[1] = bd.w * bd.h / nsx / nsz;
     cc.area[2] = bd.w * bd.t / nsx / nsy;
+    for (int k = 0; k < 3; ++k) cc.hc[k] = float(0.5 * (cc.hi[k] - cc.lo[k]));
+}
+
+// Plate gate of a whole tile (PESysMatGen.cu:1240-1254 blocks a ray whose
+// chord through a plate is below 0.1).  True when every segment from a point
+// of the ball (tc, rho) to q crosses the plate's two y faces inside its x and
+// z extents, strictly between its end points: its chord is then at least the
+// plate thickness, here at least 0.11, and the gate cannot trigger.  Ray
+// points are q + s (tc + e - q), |e| <= rho; the y faces are met at
+// s = (Y - q_y) / (u_y + e_y), bounded over e_y = +-rho, and x, z over that s
+// range are bounded at its ends.  Margins of 0.01 cover the reference's
+// direction-component branches (a component below 0.001 moves the ray less
+// than 0.001 along that axis).
+SMG_FN bool plate_gate_clear(const Layer<double>& L, const double q[3], const double tc[3],
+                             double rho) {
+    const double m = 0.01;
+    if (!(L.hi[1] - L.lo[1] >= 0.11)) return false;
+    const double u[3] = {tc[0] - q[0], tc[1] - q[1], tc[2] - q[2]};
+    const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    if (!(tabs(u[1]) > rho + 1e-6) || !(lu > rho + 1.0)) return false;
+    double s_lo = 1e300, s_hi = -1e300;
+    for (int f = 0; f < 2; ++f)
+        for (int e = -1; e <= 1; e += 2) {
+            const double s = ((f ? L.hi[1] : L.lo[1]) - q[1]) / (u[1] + e * rho);
+            s_lo = tmin2(s_lo, s);
+            s_hi = tmax2(s_hi, s);
+        }
+    // strictly inside the segment: more than 0.01 from either end point
+    const double ds = m / (lu - rho);
+    if (!(s_lo > ds) || !(s_hi < 1.0 - ds)) return false;
+    for (int k = 0; k < 3; k += 2) {
+        const double a = tmin2(s_lo * (u[k] - rho), s_hi * (u[k] - rho));
+        const double z = tmax2(s_lo * (u[k] + rho), s_hi * (u[k] + rho));
+        if (!(q[k] + a > L.lo[k] + m) || !(q[k] + z < L.hi[k] - m)) return false;
+    }
+    return true;
 }
 
 // Sub-cell number `cell` in the reference's loop order: z outermost, then x,
@@ -351,15 +399,21 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
 
     // solid angle: largest of the three projected cell faces (PESysMatGen.cu:1223-1231)
     T om;
-    {
-        T cb = T(bd.cb), sb = T(bd.sb);
-        T Dx = T(D[0]), Dy = T(D[1]), Dz = T(D[2]);
-        T tdsq = T(dsq), tdd = T(dd);
-        const T four_pi = T(4.0 * 3.14159265358979323846);
-        T omx = T(cc.area[0]) / (four_pi * tdsq) * tabs(Dx * cb - Dz * sb) / tdd;
-        T omz = T(cc.area[2]) / (four_pi * tdsq) * tabs(Dx * sb + Dz * cb) / tdd;
-        T omy = T(cc.area[1]) / (four_pi * tdsq) * tabs(Dy) / tdd;
+    if constexpr (std::is_same<T, double>::value) {
+        const double cb = bd.cb, sb = bd.sb, four_pi = 4.0 * 3.14159265358979323846;
+        double omx = cc.area[0] / (four_pi * dsq) * tabs(D[0] * cb - D[2] * sb) / dd;
+        double omz = cc.area[2] / (four_pi * dsq) * tabs(D[0] * sb + D[2] * cb) / dd;
+        double omy = cc.area[1] / (four_pi * dsq) * tabs(D[1]) / dd;
         om = tmax2(tmax2(omx, omy), omz);
+    } else {
+        // one reciprocal of 4 pi |D|^3 for the three faces
+        const float cb = float(bd.cb), sb = float(bd.sb);
+        const float Dx = float(D[0]), Dy = float(D[1]), Dz = float(D[2]);
+        const float r = trcp(float(4.0 * 3.14159265358979323846 * dsq * dd));
+        float omx = float(cc.area[0]) * tabs(Dx * cb - Dz * sb);
+        float omz = float(cc.area[2]) * tabs(Dx * sb + Dz * cb);
+        float omy = float(cc.area[1]) * tabs(Dy);
+        om = tmax2(tmax2(omx, omy), omz) * r;
     }
 
     double att = 0.0;
@@ -371,6 +425,7 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
 
     // plate gate: a ray that does not cross a plate is blocked (PESysMatGen.cu:1240-1254)
     for (int m = 0; m < g.n_layers; ++m) {
+        if ((cc.gate_clear >> m) & 1u) continue;
         T lo[3], hi[3];
         if constexpr (std::is_same<T, float>::value) {
             const Layer<float>& L = G.layers_f[m];
@@ -425,24 +480,37 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
     double Ds[3] = {cc.qs[0] - ps[0], cc.qs[1] - ps[1], cc.qs[2] - ps[2]};
     T l1, l2, la;
     if constexpr (std::is_same<T, float>::value) {
+        // The sub-cell centre q lies inside bar d and is the centre of its
+        // sub-cell, and the voxel lies outside bar d.  Then the reference's
+        // chords (PESysMatGen.cu:1329-1357) reduce to distances from q:
+        //   l1 = len - t_entry(bar)  = min_k dist_k(q to the bar face on the voxel's side) / |u_k|
+        //   l2 = len - t_entry(cell) = min_k hc_k / |u_k|
+        //   la = t_exit - t_entry of the extended ray through the cell = 2 l2
+        // with u the unit direction; no difference of two ray parameters is
+        // formed.  Rays outside these premises (a direction component below
+        // eps, |D_y| near 0.001, the voxel within 0.01 of bar d along the ray,
+        // sub-cells below 0.01) are flagged for the FP64 path, which keeps the
+        // reference's general form.
         const Bar<float>& bf = G.bars_f[d];
         SegDir<float> ss;
         float tds = 4.0f * kUnit<float> * (tabs(float(Ds[0])) + tabs(float(Ds[1])) + tabs(float(Ds[2])));
         segdir_init<float>(ss, float(Ds[0]), float(Ds[1]), float(Ds[2]), tds, near);
-        float of[3] = {float(cc.qs[0]), float(cc.qs[1]), float(cc.qs[2])};
-        float blo[3] = {-bf.w / 2.0f, -bf.t / 2.0f, -bf.h / 2.0f};
-        float bhi[3] = {bf.w / 2.0f, bf.t / 2.0f, bf.h / 2.0f};
-        float clo[3] = {float(cc.lo[0]), float(cc.lo[1]), float(cc.lo[2])};
-        float chi[3] = {float(cc.hi[0]), float(cc.hi[1]), float(cc.hi[2])};
-        float lim_lo = float(double(kSlabEps<float>) - dd);
-        float fdd = float(dd);
-        l1 = seg_box_inside_local(ss, of, fdd, lim_lo, 0.0f, blo, bhi, near);
-        l2 = seg_box_inside_local(ss, of, fdd, lim_lo, 0.0f, clo, chi, near);
-        // The extended segment runs along the same direction with length
-        // dd + 1000: its slab ratios len / D_k equal those of ss, any
-        // component below eps is flagged through ss, and its end lies 1000 mm
-        // past the cell, far beyond any slab parameter of the cell box.
-        la = seg_box_local(ss, of, lim_lo, 1000.0f, clo, chi, near);
+        near |= ss.ns != 0;
+        const float hb[3] = {0.5f * bf.w, 0.5f * bf.t, 0.5f * bf.h};
+        float d1 = INFINITY, d2 = INFINITY;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float ia = tabs(ss.inv[k]);
+            const float qk = float(cc.qs[k]);
+            const float face = ss.D[k] > 0.0f ? qk + hb[k] : hb[k] - qk;
+            d1 = tmin2(d1, face * ia);
+            d2 = tmin2(d2, cc.hc[k] * ia);
+        }
+        const bool flat = g_lt(tabs(ss.D[1]), float(kInsideDy), ss.tol_d, near);
+        near |= (ss.len - d1 < 0.01f) | (d2 < 0.01f);
+        l1 = flat ? 0.0f : d1;
+        l2 = flat ? 0.0f : d2;
+        la = 2.0f * d2;
     } else {
         double ext[3], De[3];
         for (int k = 0; k < 3; ++k) {
@@ -462,7 +530,7 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
     T eff = exp_neg<T>(att);
     T mut = T(bd.mut), mupe = T(bd.mupe);
     T ea = exp_neg<T>(double(la) * bd.mut);
-    T absf = (T(1) - ea) * mupe / mut;
+    T absf = (T(1) - ea) * T(bd.pe_frac);
     if (aux) {  // quantities the adjoint needs
         aux[0] = double(eff) * double(om);  // Omega e^-att
         aux[1] = double(l1) - double(l2);   // own-bar path

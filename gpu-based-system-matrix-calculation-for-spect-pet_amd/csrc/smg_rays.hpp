@@ -49,6 +49,32 @@ SMG_FN T tmin2(T a, T b) {
     return a <= b ? a : b;
 }
 
+// Quotient and square root.  FP32 on the device: the hardware reciprocal and
+// square root (about 1 ulp each, no denormal rescaling); every FP32 decision
+// fed by them carries a tolerance of at least 4 ulp.  FP64, and the host
+// build of the lane code: the operators.
+template <class T>
+SMG_FN T tdiv(T a, T b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_same<T, float>::value) return a * __builtin_amdgcn_rcpf(b);
+#endif
+    return a / b;
+}
+template <class T>
+SMG_FN T trcp(T b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_rcpf(b);
+#endif
+    return T(1) / b;
+}
+template <class T>
+SMG_FN T tsqrt(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_sqrtf(x);
+#endif
+    return sqrt(x);
+}
+
 // Element k of a 3-vector by selects: indexing with a run-time k would put
 // the whole array in scratch memory.
 template <class T>
@@ -118,9 +144,9 @@ SMG_FN void segdir_init(SegDir<T>& s, T dx, T dy, T dz, T tol_d, bool& near) {
     // With one small component the reference works with the 2-D length of
     // the other two (PESysMatGen.cu:75, :131, :186); with two or three small
     // components the length is not used.
-    s.len = sqrt(l2);
+    s.len = tsqrt(l2);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) s.inv[k] = s.small[k] ? T(0) : s.len / s.D[k];
+    for (int k = 0; k < 3; ++k) s.inv[k] = s.small[k] ? T(0) : tdiv(s.len, s.D[k]);
 }
 
 // Slab intersection over the active axes.  Also returns error bounds of tmin
@@ -347,19 +373,24 @@ SMG_FN float seg_box_inside_local(const SegDir<float>& s, const float o[3], floa
     return chord;
 }
 
-// Per-ray part of the cylinder chord: the unit direction and the ratios the
-// reference forms for every hole (identical values, formed once per ray).
+// Per-ray part of the cylinder chord: the unit direction, the ratios the
+// reference forms for every hole, and the reciprocals of k_y and of
+// k_x^2 + k_z^2 that replace its per-hole divisions (a change of at most an
+// ulp in FP64).
 struct CylRay {
-    double kx, ky, kz, sxk, szk, qa;
+    double kx, ky, kz, sxk, szk, qa, inv_ky, inv_qa;
 };
 
 SMG_FN void cyl_ray(const double D[3], double len, CylRay& c) {
-    c.kx = D[0] / len;
-    c.ky = D[1] / len;
-    c.kz = D[2] / len;
-    c.sxk = c.kx / c.ky;
-    c.szk = c.kz / c.ky;
+    const double il = 1.0 / len;
+    c.kx = D[0] * il;
+    c.ky = D[1] * il;
+    c.kz = D[2] * il;
+    c.inv_ky = 1.0 / c.ky;
+    c.sxk = c.kx * c.inv_ky;
+    c.szk = c.kz * c.inv_ky;
     c.qa = c.kx * c.kx + c.kz * c.kz;
+    c.inv_qa = 1.0 / c.qa;
 }
 
 // Chord through the finite cylinder of radius r, axis parallel to y at
@@ -370,13 +401,14 @@ SMG_FN void cyl_ray(const double D[3], double len, CylRay& c) {
 SMG_FN double cyl_chord(const double a[3], const CylRay& c, double len, double xc, double y1,
                         double y2, double zc, double r) {
     if (fabs(y1 - y2) < kCylDy) return 0.0;
-    const double kx = c.kx, ky = c.ky, kz = c.kz, sxk = c.sxk, szk = c.szk;
-    double xl = a[0] + sxk * (y1 - a[1]) - xc;
-    double zl = a[2] + szk * (y1 - a[1]) - zc;
-    double xr = a[0] + sxk * (y2 - a[1]) - xc;
-    double zr = a[2] + szk * (y2 - a[1]) - zc;
-    double t_lo = (y1 - a[1]) / ky;
-    double t_hi = (y2 - a[1]) / ky;
+    const double kx = c.kx, kz = c.kz, sxk = c.sxk, szk = c.szk;
+    const double a1 = y1 - a[1], a2 = y2 - a[1];
+    double xl = a[0] + sxk * a1 - xc;
+    double zl = a[2] + szk * a1 - zc;
+    double xr = a[0] + sxk * a2 - xc;
+    double zr = a[2] + szk * a2 - zc;
+    double t_lo = a1 * c.inv_ky;
+    double t_hi = a2 * c.inv_ky;
     double r2 = r * r;
     bool in_l = xl * xl + zl * zl <= r2;
     bool in_r = xr * xr + zr * zr <= r2;
@@ -392,8 +424,8 @@ SMG_FN double cyl_chord(const double a[3], const CylRay& c, double len, double x
     if (disc <= kCylDisc) return 0.0;
     double sq = sqrt(disc);
     double qb = kx * x0 + kz * z0;
-    double t1 = (-qb - sq) / qa;
-    double t2 = (-qb + sq) / qa;
+    double t1 = (-qb - sq) * c.inv_qa;
+    double t2 = (-qb + sq) * c.inv_qa;
     if (!in_l && !in_r) {
         if (t1 >= t_lo && t2 <= t_hi) {
             if (t2 <= kCylT || t1 <= kCylT) return 0.0;

@@ -124,3 +124,14 @@ def small_geometry(params_mod, beta=0.0, layers=1, nx=13, ny=11, nz=3, dphi=0.0,
                          shift=(0.3, -0.2, 0.1), fov2coll=25.0)
     params_mod.set_physics(p)
     return p
+
+
+def narrow_plate_geometry(params_mod):
+    """small_geometry with two plates of 9 x 7 mm (x by z): the plate edges cut
+    through the bundle of rays, so some rays miss a plate or leave through its
+    side and are stopped by the plate gate."""
+    p = small_geometry(params_mod, beta=0.05, layers=2)
+    for m in range(2):
+        p.coll[(m + 1) * 10 + 1] = 9.0
+        p.coll[(m + 1) * 10 + 3] = 7.0
+    return p

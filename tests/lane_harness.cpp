@@ -43,6 +43,8 @@ bool make_ctx(const float* coll, const float* det, const float* img, const float
     c.G.holes_d = c.hg.holes_d.data();
     c.G.layers_f = c.hg.layers_f.data();
     c.G.layers_d = c.hg.layers_d.data();
+    c.G.bar_groups = c.hg.bar_groups.data();
+    c.G.hole_groups = c.hg.hole_groups.data();
     return true;
 }
 
@@ -105,13 +107,17 @@ struct MaskCands {
 
 extern "C" {
 
-// Checks the wave-level candidate selection (bar_tile_may_hit per tile, then
-// box_meets_cone per sub-cell) on n_waves seeded (bar, tile) pairs: every lane's FP64 sub-cell term
+// Checks the wave-level candidate selection (bar_tile_may_hit per tile on
+// group boxes and members, then
+// box_meets_cone per sub-cell) and the tile-level plate-gate skip
+// (plate_gate_clear) on n_waves seeded (bar, tile) pairs: every lane's FP64 sub-cell term
 // with the narrowed lists must equal, bit for bit, the term with the lane's
 // full culling set.  stats[0] = mismatching (lane, sub-cell) terms,
 // stats[1] = terms checked, stats[2] = mean narrowed bars per sub-cell,
 // stats[3] = mean narrowed holes, stats[4] = mean bars in the full sets,
-// stats[5] = mean tile candidates (bars) kept by a culling test of some lane.
+// stats[5] = mean tile candidates (bars) kept by a culling test of some lane,
+// stats[6] = share of (sub-cell, layer) plate gates skipped (plate_gate_clear);
+// the narrowed terms are computed with those gates skipped.
 int lane_narrow_check(const float* coll, const float* det, const float* img, const float* phys,
                       int rot, int sx, int sy, int sz, int tx, int ty, int tz, int n_waves,
                       unsigned seed, double* stats) {
@@ -129,7 +135,7 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
         st ^= st << 17;
         return st;
     };
-    double bad = 0, checked = 0, nsb = 0, nsh = 0, nfull = 0, cells = 0, ntile = 0, nwav = 0;
+    double bad = 0, checked = 0, nsb = 0, nsh = 0, nfull = 0, cells = 0, ntile = 0, nwav = 0, gates = 0;
     for (int w = 0; w < n_waves; ++w) {
         int d = (int)(next() % g.n_det);
         int t = (int)(next() % ((uint64_t)nt[0] * nt[1] * nt[2]));
@@ -145,14 +151,20 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
         // candidates (bar_tile_may_hit first, then the culling test)
         const Bar<float>& bdf = G.bars_f[d];
         std::vector<char> tb(g.n_det, 0), th(g.n_holes, 0);
+        auto group_hit = [&](const GroupBox& B) {
+            return bar_tile_may_hit(bdf, tcf, rhof, B.c[0], B.c[1], B.c[2], 1.0f, 0.0f, B.h[0],
+                                    B.h[1], B.h[2]);
+        };
         for (int k = 0; k < g.n_det; ++k) {
             const Bar<float>& b = G.bars_f[k];
-            tb[k] = k != d && bar_tile_may_hit(bdf, tcf, rhof, b.cx, b.cy, b.cz, b.cb, b.sb,
-                                               0.5f * b.w, 0.5f * b.t, 0.5f * b.h);
+            tb[k] = k != d && group_hit(G.bar_groups[k / kGroup]) &&
+                    bar_tile_may_hit(bdf, tcf, rhof, b.cx, b.cy, b.cz, b.cb, b.sb, 0.5f * b.w,
+                                     0.5f * b.t, 0.5f * b.h);
         }
         for (int h = 0; h < g.n_holes; ++h) {
             const Hole<float>& o = G.holes_f[h];
-            th[h] = bar_tile_may_hit(bdf, tcf, rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f, o.r,
+            th[h] = group_hit(G.hole_groups[h / kGroup]) &&
+                    bar_tile_may_hit(bdf, tcf, rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f, o.r,
                                      0.5f * tabs(o.y2 - o.y1), o.r);
         }
         std::vector<uint64_t> bmask(g.n_det, 0), hmask(g.n_holes, 0);
@@ -209,12 +221,17 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
                     nsb += nar.b.size();
                     nsh += nar.h.size();
                     cells += 1;
+                    // the same sub-cell with the tile's plate gates skipped
+                    CellCtx cg = cc;
+                    for (int m = 0; m < g.n_layers; ++m)
+                        if (plate_gate_clear(G.layers_d[m], cc.q, tc, rho)) cg.gate_clear |= 1u << m;
+                    gates += __builtin_popcount(cg.gate_clear);
                     for (int l = 0; l < tx * ty * tz; ++l) {
                         if (!valid[l]) continue;
                         full.lane = nar.lane = l;
                         bool n1 = false, n2 = false;
                         double v1 = pe_cell_ray<double>(G, d, P[l], PS[l], cc, full, n1);
-                        double v2 = pe_cell_ray<double>(G, d, P[l], PS[l], cc, nar, n2);
+                        double v2 = pe_cell_ray<double>(G, d, P[l], PS[l], cg, nar, n2);
                         checked += 1;
                         if (!(v1 == v2)) bad += 1;
                     }
@@ -226,6 +243,7 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
     stats[3] = cells > 0 ? nsh / cells : 0;
     stats[4] = checked > 0 ? nfull / (checked / (sx * sy * sz)) : 0;
     stats[5] = nwav > 0 ? ntile / nwav : 0;
+    stats[6] = cells > 0 ? gates / (cells * (g.n_layers > 0 ? g.n_layers : 1)) : 0;
     return 0;
 }
 

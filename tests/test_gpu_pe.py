@@ -4,7 +4,7 @@ PESysMatGen.cu:1368-1437): one call per rotation, [N][V] float32 block."""
 import numpy as np
 import pytest
 
-from conftest import TOL_FP32, TOL_FP64, assert_parity, small_geometry
+from conftest import TOL_FP32, TOL_FP64, assert_parity, narrow_plate_geometry, small_geometry
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +53,20 @@ def test_edge_geometries(smg_mod, oracle_mod, params_mod, beta, layers, prec, to
     with _ctx(smg_mod, p, sub) as ctx:
         got, _ = ctx.pe_forward(0, prec)
     assert_parity(got, ref, tol, f"edge beta={beta} layers={layers} {prec}")
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
+def test_narrow_plates(smg_mod, oracle_mod, params_mod, prec, tol):
+    """Plates narrower than the ray bundle: some tiles keep their plate gates
+    (rays leaving through a plate side are stopped, PESysMatGen.cu:1240-1254)
+    while others skip them."""
+    p = narrow_plate_geometry(params_mod)
+    sub = (3, 4, 2)
+    ref = oracle_mod.Oracle(p).pe_matrix(sub=sub)
+    assert (ref == 0).any()
+    with _ctx(smg_mod, p, sub) as ctx:
+        got, _ = ctx.pe_forward(0, prec)
+    assert_parity(got, ref, tol, f"narrow plates {prec}")
 
 
 @pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])

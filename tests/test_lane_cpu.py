@@ -4,7 +4,7 @@ must stay within 1e-5, the FP64 path within float output rounding."""
 import numpy as np
 import pytest
 
-from conftest import TOL_FP32, TOL_FP64, assert_parity, small_geometry
+from conftest import TOL_FP32, TOL_FP64, assert_parity, narrow_plate_geometry, small_geometry
 
 
 def lane_pe(L, p, d, v, sub, exact, rot=0):
@@ -50,12 +50,14 @@ def test_lane_scatter(lane_lib, oracle_mod, params_mod, kind, variant, exact):
                   f"lane scatter kind={kind} variant={variant} exact={exact} fallbacks={fb}")
 
 
-@pytest.mark.parametrize("case", ["cfg1", "cfg2", "rotated", "layers3", "rot2"])
+@pytest.mark.parametrize("case", ["cfg1", "cfg2", "rotated", "layers3", "rot2", "narrowplate"])
 def test_lane_narrowing_is_exact(lane_lib, params_mod, case):
     """The wave-level candidate selection (bar_tile_may_hit per tile,
     box_meets_cone per sub-cell) drops only candidates whose chord is zero for
-    every lane of the tile: the FP64 sub-cell terms with the narrowed lists
-    equal the terms with each lane's full culling set bit for bit."""
+    every lane of the tile, and plate_gate_clear skips only gates that cannot
+    trigger: the FP64 sub-cell terms with the narrowed lists and skipped gates
+    equal the terms with each lane's full culling set and every gate, bit for
+    bit."""
     rot = 0
     if case == "cfg1":
         p, sub, tile, waves = params_mod.config(1), (4, 4, 4), (8, 8, 1), 24
@@ -65,12 +67,17 @@ def test_lane_narrowing_is_exact(lane_lib, params_mod, case):
         p, sub, tile, waves = small_geometry(params_mod, beta=0.07), (3, 4, 2), (8, 4, 2), 40
     elif case == "layers3":
         p, sub, tile, waves = small_geometry(params_mod, beta=0.05, layers=3), (3, 4, 2), (8, 4, 2), 40
-    else:
+    elif case == "rot2":
         p, sub, tile, waves = small_geometry(params_mod, nx=9, ny=9, nz=2, dphi=0.3, n_rot=3), (2, 3, 2), (8, 4, 2), 40
         rot = 2
-    st = np.zeros(6)
+    else:
+        p, sub, tile, waves = narrow_plate_geometry(params_mod), (3, 4, 2), (8, 4, 2), 60
+    st = np.zeros(7)
     assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, rot, *sub, *tile, waves, 5, st) == 0
     print(f"narrowing {case}: {int(st[1])} terms, {st[2]:.2f} bars and {st[3]:.2f} holes per "
-          f"sub-cell (full sets {st[4]:.1f} bars, tile candidates {st[5]:.1f})")
+          f"sub-cell (full sets {st[4]:.1f} bars, tile candidates {st[5]:.1f}), "
+          f"{100 * st[6]:.1f}% of plate gates skipped")
     assert st[1] > 0
     assert st[0] == 0
+    if case == "narrowplate":
+        assert 0.05 < st[6] < 0.95  # both the skipped and the traced gates are exercised
