@@ -486,21 +486,25 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
         //   l1 = len - t_entry(bar)  = min_k dist_k(q to the bar face on the voxel's side) / |u_k|
         //   l2 = len - t_entry(cell) = min_k hc_k / |u_k|
         //   la = t_exit - t_entry of the extended ray through the cell = 2 l2
-        // with u the unit direction; no difference of two ray parameters is
-        // formed.  Rays outside these premises (a direction component below
-        // eps, |D_y| near 0.001, the voxel within 0.01 of bar d along the ray,
-        // sub-cells below 0.01) are flagged for the FP64 path, which keeps the
-        // reference's general form.
+        // with u the unit direction over the axes whose component is not below
+        // eps (the reference's one- and two-small-component branches,
+        // PESysMatGen.cu:333-434, drop those axes; the premises put the start
+        // strictly inside their faces, as the branches require); no
+        // difference of two ray parameters is formed.  Rays outside the
+        // premises (all components below eps, |D_y| near 0.001, the voxel
+        // within 0.01 of bar d along the ray, sub-cells below 0.01) are
+        // flagged for the FP64 path, which keeps the reference's general form.
         const Bar<float>& bf = G.bars_f[d];
         SegDir<float> ss;
         float tds = 4.0f * kUnit<float> * (tabs(float(Ds[0])) + tabs(float(Ds[1])) + tabs(float(Ds[2])));
         segdir_init<float>(ss, float(Ds[0]), float(Ds[1]), float(Ds[2]), tds, near);
-        near |= ss.ns != 0;
+        near |= ss.ns == 3;
         const float hb[3] = {0.5f * bf.w, 0.5f * bf.t, 0.5f * bf.h};
         float d1 = INFINITY, d2 = INFINITY;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float ia = tabs(ss.inv[k]);
+            if (ss.small[k]) continue;
+            const float ia = tabs(ss.inv[k]);  // 1 / |u_k|
             const float qk = float(cc.qs[k]);
             const float face = ss.D[k] > 0.0f ? qk + hb[k] : hb[k] - qk;
             d1 = tmin2(d1, face * ia);

@@ -30,6 +30,21 @@ def test_lane_pe(lane_lib, oracle_mod, params_mod, cfg, n, exact):
                   f"lane PE cfg{cfg} exact={exact} fallbacks={fb}")
 
 
+def test_lane_pe_axis_aligned_rays(lane_lib, oracle_mod, params_mod):
+    """With 2x2x2 sub-cells the sub-cell centres of the 32x32x1-voxel, 64-bar
+    geometry share x and z with voxel columns, so many rays have direction
+    components exactly 0: the reference's one- and two-small-component
+    branches (PESysMatGen.cu:69-73, :333-434) on the FP32 path."""
+    p = params_mod.config(1)
+    o = oracle_mod.Oracle(p)
+    rng = np.random.default_rng(41)
+    d = rng.integers(0, o.n_det, 2000)
+    v = rng.integers(0, o.n_vox, 2000)
+    got, fb = lane_pe(lane_lib, p, d, v, (2, 2, 2), 0)
+    assert_parity(got, o.pe_pairs(d, v, sub=(2, 2, 2)), TOL_FP32,
+                  f"lane PE axis-aligned rays fallbacks={fb}")
+
+
 @pytest.mark.parametrize("kind,variant", [(0, 0), (1, 0), (1, 1)])
 @pytest.mark.parametrize("exact", [0, 1])
 def test_lane_scatter(lane_lib, oracle_mod, params_mod, kind, variant, exact):
