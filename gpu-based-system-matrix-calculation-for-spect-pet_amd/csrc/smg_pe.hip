@@ -199,39 +199,48 @@ __device__ __forceinline__ void tile_cell(const PeLaunch& A, const Bar<double>& 
     cc.gate_clear = (uint32_t)__ballot(clear);
 }
 
-// Step 2: candidates whose box can meet a segment from the sub-cell point to
-// the tile (box_meets_cone); writes their list indices to bar_sel/hole_sel.
-__device__ void narrow_candidates(const PeLaunch& A, const CellCtx& cc, const WaveSetup& W,
+// Step 2: candidates whose box a segment from the sub-cell centre qd to the
+// tile can meet (box_meets_frustum with a point source), bars and holes in
+// one pass; writes their list indices to bar_sel / hole_sel.
+__device__ void narrow_candidates(const PeLaunch& A, const double qd[3], const WaveSetup& W,
                                   int lane, WaveLists& L, int nb, int nh, int& nsb, int& nsh) {
     const PeGeom& G = A.geom;
-    const float q[3] = {float(cc.q[0]), float(cc.q[1]), float(cc.q[2])};
+    const float q[3] = {float(qd[0]), float(qd[1]), float(qd[2])};
+    const float rq[3] = {0.0f, 0.0f, 0.0f};
     nsb = 0;
     nsh = 0;
-    for (int base = 0; base < nb; base += 64) {
-        int j = base + lane;
+    const int n = nb + nh;
+    for (int base = 0; base < n; base += 64) {
+        const int j = base + lane;
+        const bool is_bar = j < nb;
         bool pass = false;
-        if (j < nb) {
-            const Bar<float>& bf = G.bars_f[L.bar_idx[j]];
-            pass = box_meets_cone(q, W.tcf, W.rhof, bf.cx, bf.cy, bf.cz, bf.cb, bf.sb, 0.5f * bf.w,
-                                  0.5f * bf.t, 0.5f * bf.h);
+        if (j < n) {
+            float c[3], cb, sb, h[3];
+            if (is_bar) {
+                const Bar<float>& bf = G.bars_f[L.bar_idx[j]];
+                c[0] = bf.cx; c[1] = bf.cy; c[2] = bf.cz;
+                cb = bf.cb; sb = bf.sb;
+                h[0] = 0.5f * bf.w; h[1] = 0.5f * bf.t; h[2] = 0.5f * bf.h;
+            } else {
+                const Hole<float>& o = G.holes_f[L.hole_idx[j - nb]];
+                c[0] = o.xc; c[1] = o.ymid; c[2] = o.zc;
+                cb = 1.0f; sb = 0.0f;
+                h[0] = o.r; h[1] = 0.5f * tabs(o.y2 - o.y1); h[2] = o.r;
+            }
+            pass = box_meets_frustum(q, rq, W.tcf, W.rhof, c[0], c[1], c[2], cb, sb, h[0], h[1], h[2]);
         }
-        uint64_t bal = __ballot(pass);
-        if (pass) L.bar_sel[nsb + lanes_below(bal, lane)] = (uint16_t)j;
-        nsb += __popcll(bal);
-    }
-    for (int base = 0; base < nh; base += 64) {
-        int j = base + lane;
-        bool pass = false;
-        if (j < nh) {
-            const Hole<float>& o = G.holes_f[L.hole_idx[j]];
-            pass = box_meets_cone(q, W.tcf, W.rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f, o.r,
-                                  0.5f * tabs(o.y2 - o.y1), o.r);
+        const uint64_t bb = __ballot(pass && is_bar), bh = __ballot(pass && !is_bar);
+        if (pass) {
+            if (is_bar)
+                L.bar_sel[nsb + lanes_below(bb, lane)] = (uint16_t)j;
+            else
+                L.hole_sel[nsh + lanes_below(bh, lane)] = (uint16_t)(j - nb);
         }
-        uint64_t bal = __ballot(pass);
-        if (pass) L.hole_sel[nsh + lanes_below(bal, lane)] = (uint16_t)j;
-        nsh += __popcll(bal);
+        nsb += __popcll(bb);
+        nsh += __popcll(bh);
     }
 }
+
 
 template <bool kExact>
 __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
@@ -278,7 +287,8 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
             CellCtx cc;
             tile_cell(A, bd, cell, W, lane, cc);
             int nsb, nsh;
-            narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
+            __syncthreads();
+            narrow_candidates(A, cc.q, W, lane, L, nb, nh, nsb, nsh);
             __syncthreads();
             bool near = false;
             if (W.valid) {
@@ -296,22 +306,21 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
                     ++n_redo;
                 }
             }
-            __syncthreads();
         }
         if constexpr (!kExact) {
             for (int i = 0; i < n_redo; ++i) {
                 CellCtx cc;
                 tile_cell(A, bd, redo_cell[i], W, lane, cc);
-                int nsb, nsh;
-                narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
+                int rsb, rsh;
+                __syncthreads();
+                narrow_candidates(A, cc.q, W, lane, L, nb, nh, rsb, rsh);
                 __syncthreads();
                 if ((redo_mask[i] >> lane) & 1ull) {
-                    LdsCands C{&L, nsb, nsh, lane};
+                    LdsCands C{&L, rsb, rsh, lane};
                     bool n2 = false;
                     acc.add((T)pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2));
                     ++fallbacks;
                 }
-                __syncthreads();
             }
         }
     }
@@ -449,7 +458,8 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                     CellCtx cc;
                     tile_cell(A, bd, cell, W, lane, cc);
                     int nsb, nsh;
-                    narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
+                    __syncthreads();
+                    narrow_candidates(A, cc.q, W, lane, L, nb, nh, nsb, nsh);
                     __syncthreads();
                     LdsCands C{&L, nsb, nsh, lane};
                     const bool near = adjoint_cell<T>(G, d, lane, W, L, C, cc, W.valid, wv, acc,
@@ -470,14 +480,14 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                     for (int i = 0; i < n_redo; ++i) {
                         CellCtx cc;
                         tile_cell(A, bd, redo_cell[i], W, lane, cc);
-                        int nsb, nsh;
-                        narrow_candidates(A, cc, W, lane, L, nb, nh, nsb, nsh);
+                        int rsb, rsh;
                         __syncthreads();
-                        LdsCands C{&L, nsb, nsh, lane};
+                        narrow_candidates(A, cc.q, W, lane, L, nb, nh, rsb, rsh);
+                        __syncthreads();
+                        LdsCands C{&L, rsb, rsh, lane};
                         const bool mine = (redo_mask[i] >> lane) & 1ull;
                         adjoint_cell<double>(G, d, lane, W, L, C, cc, mine, wv, acc, lay_acc, cand_acc);
                         fallbacks += mine;
-                        __syncthreads();
                     }
                 }
             }

@@ -175,7 +175,7 @@ SMG_FN void tile_ball(const GeomDesc& g, int ix0, int iy0, int iz0, const int ti
                      ((ez - 1) * g.wz) * ((ez - 1) * g.wz)) + 1e-6;
 }
 
-// FP32 copy of the ball for box_meets_cone, radius rounded up.
+// FP32 copy of the ball for box_meets_frustum, radius rounded up.
 SMG_FN void tile_ball_f(const double tc[3], double rho, float tcf[3], float& rhof) {
     for (int k = 0; k < 3; ++k) tcf[k] = float(tc[k]);
     rhof = float(rho) * 1.0001f;
@@ -221,12 +221,6 @@ SMG_FN bool box_meets_frustum(const float q[3], const float rq[3], const float t
     return ok && lo <= hi + 1e-5f;
 }
 
-// Segments from the point q (a sub-cell centre) to the tile ball.
-SMG_FN bool box_meets_cone(const float q[3], const float tc[3], float rho, float cx, float cy,
-                           float cz, float cb, float sb, float hx, float hy, float hz) {
-    const float rq[3] = {0.0f, 0.0f, 0.0f};
-    return box_meets_frustum(q, rq, tc, rho, cx, cy, cz, cb, sb, hx, hy, hz);
-}
 
 // Segments from any point of bar d to the tile ball, against a box turned by
 // beta_k (cos, sin in cbk, sbk): bar d's half sizes become the axis-aligned
@@ -287,6 +281,7 @@ SMG_FN void make_cell_ctx(const Bar<double>& bd, int a, int b, int c, int nsx, i
     cc.area[2] = bd.w * bd.t / nsx / nsy;
     for (int k = 0; k < 3; ++k) cc.hc[k] = float(0.5 * (cc.hi[k] - cc.lo[k]));
 }
+
 
 // Plate gate of a whole tile (PESysMatGen.cu:1240-1254 blocks a ray whose
 // chord through a plate is below 0.1).  True when every segment from a point

@@ -23,6 +23,8 @@
 #define SMG_FN inline
 using std::sqrt;
 using std::fabs;
+using std::fmax;
+using std::fmin;
 #endif
 
 #include <type_traits>
@@ -38,15 +40,18 @@ constexpr T kUnit = std::is_same<T, float>::value ? T(5.9604645e-08) : T(1.11022
 
 template <class T>
 SMG_FN T tabs(T x) {
-    return x < T(0) ? -x : x;
+    if constexpr (std::is_floating_point<T>::value) return fabs(x);  // an operand modifier
+    else return x < T(0) ? -x : x;
 }
 template <class T>
 SMG_FN T tmax2(T a, T b) {
-    return a >= b ? a : b;
+    if constexpr (std::is_floating_point<T>::value) return fmax(a, b);
+    else return a >= b ? a : b;
 }
 template <class T>
 SMG_FN T tmin2(T a, T b) {
-    return a <= b ? a : b;
+    if constexpr (std::is_floating_point<T>::value) return fmin(a, b);
+    else return a <= b ? a : b;
 }
 
 // Quotient and square root.  FP32 on the device: the hardware reciprocal and

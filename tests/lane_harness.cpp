@@ -108,16 +108,16 @@ struct MaskCands {
 extern "C" {
 
 // Checks the wave-level candidate selection (bar_tile_may_hit per tile on
-// group boxes and members, then
-// box_meets_cone per sub-cell) and the tile-level plate-gate skip
-// (plate_gate_clear) on n_waves seeded (bar, tile) pairs: every lane's FP64 sub-cell term
-// with the narrowed lists must equal, bit for bit, the term with the lane's
-// full culling set.  stats[0] = mismatching (lane, sub-cell) terms,
-// stats[1] = terms checked, stats[2] = mean narrowed bars per sub-cell,
-// stats[3] = mean narrowed holes, stats[4] = mean bars in the full sets,
-// stats[5] = mean tile candidates (bars) kept by a culling test of some lane,
-// stats[6] = share of (sub-cell, layer) plate gates skipped (plate_gate_clear);
-// the narrowed terms are computed with those gates skipped.
+// group boxes and members, then box_meets_frustum per sub-cell)
+// and the tile-level plate-gate skip (plate_gate_clear) on n_waves seeded
+// (bar, tile) pairs: every lane's FP64 sub-cell term with the narrowed lists
+// must equal, bit for bit, the term with the lane's full culling set.
+// stats[0] = mismatching (lane, sub-cell) terms, stats[1] = terms checked,
+// stats[2] = mean narrowed bars per sub-cell, stats[3] = mean narrowed holes,
+// stats[4] = mean bars in the full sets, stats[5] = mean tile candidates
+// (bars) kept by a culling test of some lane, stats[6] = share of (sub-cell,
+// layer) plate gates skipped; the narrowed terms are computed with those
+// gates skipped.
 int lane_narrow_check(const float* coll, const float* det, const float* img, const float* phys,
                       int rot, int sx, int sy, int sz, int tx, int ty, int tz, int n_waves,
                       unsigned seed, double* stats) {
@@ -195,14 +195,15 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
                     CellCtx cc;
                     make_cell_ctx(bd, a, b, cz, sx, sy, sz, cc);
                     const float q[3] = {float(cc.q[0]), float(cc.q[1]), float(cc.q[2])};
+                    const float rq[3] = {0.0f, 0.0f, 0.0f};
                     MaskCands full, nar;
                     for (int k = 0; k < g.n_det; ++k) {
                         if (!bmask[k]) continue;
                         full.b.push_back(k);
                         full.bm.push_back(bmask[k]);
                         const Bar<float>& bf = G.bars_f[k];
-                        if (tb[k] && box_meets_cone(q, tcf, rhof, bf.cx, bf.cy, bf.cz, bf.cb, bf.sb, 0.5f * bf.w,
-                                           0.5f * bf.t, 0.5f * bf.h)) {
+                        if (tb[k] && box_meets_frustum(q, rq, tcf, rhof, bf.cx, bf.cy, bf.cz, bf.cb, bf.sb,
+                                                       0.5f * bf.w, 0.5f * bf.t, 0.5f * bf.h)) {
                             nar.b.push_back(k);
                             nar.bm.push_back(bmask[k]);
                         }
@@ -212,8 +213,8 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
                         full.h.push_back(h);
                         full.hm.push_back(hmask[h]);
                         const Hole<float>& o = G.holes_f[h];
-                        if (th[h] && box_meets_cone(q, tcf, rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f, o.r,
-                                           0.5f * tabs(o.y2 - o.y1), o.r)) {
+                        if (th[h] && box_meets_frustum(q, rq, tcf, rhof, o.xc, o.ymid, o.zc, 1.0f, 0.0f,
+                                                       o.r, 0.5f * tabs(o.y2 - o.y1), o.r)) {
                             nar.h.push_back(h);
                             nar.hm.push_back(hmask[h]);
                         }

@@ -68,7 +68,7 @@ def test_lane_scatter(lane_lib, oracle_mod, params_mod, kind, variant, exact):
 @pytest.mark.parametrize("case", ["cfg1", "cfg2", "rotated", "layers3", "rot2", "narrowplate"])
 def test_lane_narrowing_is_exact(lane_lib, params_mod, case):
     """The wave-level candidate selection (bar_tile_may_hit per tile,
-    box_meets_cone per sub-cell) drops only candidates whose chord is zero for
+    box_meets_frustum per sub-cell) drops only candidates whose chord is zero for
     every lane of the tile, and plate_gate_clear skips only gates that cannot
     trigger: the FP64 sub-cell terms with the narrowed lists and skipped gates
     equal the terms with each lane's full culling set and every gate, bit for
