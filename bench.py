@@ -32,6 +32,8 @@ METRIC = "matrix elements/s (PE, scatter) at ≤1e-5 rel err vs FP64; fwd+grad w
 # MI355X published peaks
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# measured HBM traffic of one FP32 launch of the full-size PE step (rocprofv3 PMC)
+TRAFFIC_BYTES = 2.18e9
 
 
 def log(*a):
@@ -228,7 +230,10 @@ def main():
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-        "traffic": None,
+        # HBM bytes per launch: FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc passes of this
+        # command, scaled by the same counters on a plain 1 GiB copy (bench.py --copy-calibration)
+        "traffic": TRAFFIC_BYTES if args.precision == "fp32" else None,
+        "traffic_source": "profiles/pe_forward_kernel_64x64x32_2048bars.md",
         "kernel": "pe_forward_kernel",
         "flops_per_ray": round(f_ray, 1),
         "rays_per_launch": rays,
