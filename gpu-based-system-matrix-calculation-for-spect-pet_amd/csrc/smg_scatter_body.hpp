@@ -48,7 +48,10 @@ SMG_FN T kn_of_cos(T c, T alpha) {
 
 // Inclusive rectangle rule of KN sin on lo + i*step, i = 0..n with
 // n = (int)((hi - lo)/step) (scatter.cu:980-993).  FP32: flags a step count
-// within rounding distance of an integer.
+// within rounding distance of an integer, and takes cos and sin of the nodes
+// from an angle-addition recurrence in FP64 started at lo (four FMAs a step
+// instead of two transcendental calls; the recurrence drifts by about one FP64
+// ulp per step).  FP64: cos and sin of every node, as the reference.
 template <class T>
 SMG_FN T kn_sum(T lo, T hi, T alpha, bool& near) {
     const T step = T(kKnStep);
@@ -57,14 +60,26 @@ SMG_FN T kn_sum(T lo, T hi, T alpha, bool& near) {
     if constexpr (kGuarded<T>) {
         T fr = x - T(n);
         near |= (fr < T(2e-4) && n > 0) | (fr > T(1) - T(2e-4));
+        // cos and sin of the step (double)0.01f = 0.009999999776482582
+        const double cst = 0.99995000041890041468, snt = 0.0099998331106604225523;
+        double c = cos((double)lo), sv = sin((double)lo);
+        KahanSum<T> s;
+        for (int i = 0; i <= n; ++i) {
+            s.add(kn_of_cos(T(c), alpha) * T(sv) * step);
+            const double c2 = c * cst - sv * snt;
+            sv = sv * cst + c * snt;
+            c = c2;
+        }
+        return s.value();
+    } else {
+        KahanSum<T> s;
+        for (int i = 0; i <= n; ++i) {
+            T th = lo + T(i) * step;
+            T c = cos(th), sn = sin(th);
+            s.add(kn_of_cos(c, alpha) * sn * step);
+        }
+        return s.value();
     }
-    KahanSum<T> s;
-    for (int i = 0; i <= n; ++i) {
-        T th = lo + T(i) * step;
-        T c = cos(th), sn = sin(th);
-        s.add(kn_of_cos(c, alpha) * sn * step);
-    }
-    return s.value();
 }
 
 // Angle between u and w (vectors src -> a and a -> det), used for the polar
@@ -186,12 +201,39 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
     T us[3];
     for (int k = 0; k < 3; ++k) us[k] = T(sc.qss[k] - ps[k]);
     T tmin = T(0), tmax = T(0);
-    for (int c = 0; c < 8; ++c) {
-        T wc[3];
-        for (int k = 0; k < 3; ++k) wc[k] = T(sc.corner[c][k] - sc.qss[k]);
-        T th = cone_angle<T>(us, wc);
-        if (c == 0 || th > tmax) tmax = th;
-        if (c == 0 || th < tmin) tmin = th;
+    if constexpr (std::is_same<T, double>::value) {
+        for (int c = 0; c < 8; ++c) {
+            T wc[3];
+            for (int k = 0; k < 3; ++k) wc[k] = T(sc.corner[c][k] - sc.qss[k]);
+            T th = cone_angle<T>(us, wc);
+            if (c == 0 || th > tmax) tmax = th;
+            if (c == 0 || th < tmin) tmin = th;
+        }
+    } else {
+        // The smallest and largest angle by exact ordering, then two atan2: for
+        // angles a, b in [0, pi] given as (|u x w|, u.w) = r (sin, cos),
+        // a < b exactly when sin(b - a) > 0, i.e. s_b c_a - c_b s_a > 0.
+        float s_lo = 0.0f, c_lo = 1.0f, s_hi = 0.0f, c_hi = 1.0f;
+        for (int c = 0; c < 8; ++c) {
+            float w[3];
+            for (int k = 0; k < 3; ++k) w[k] = float(sc.corner[c][k] - sc.qss[k]);
+            const float cx = us[1] * w[2] - us[2] * w[1], cy = us[2] * w[0] - us[0] * w[2],
+                        cz = us[0] * w[1] - us[1] * w[0];
+            // (r sin, r cos) with r = |u| |w| > 0: the sign tests and atan2 do
+            // not depend on r
+            const float sn = tsqrt(cx * cx + cy * cy + cz * cz);
+            const float cn = us[0] * w[0] + us[1] * w[1] + us[2] * w[2];
+            if (c == 0 || sn * c_hi - cn * s_hi > 0.0f) {
+                s_hi = sn;
+                c_hi = cn;
+            }
+            if (c == 0 || s_lo * cn - c_lo * sn > 0.0f) {
+                s_lo = sn;
+                c_lo = cn;
+            }
+        }
+        tmin = atan2f(s_lo, c_lo);
+        tmax = atan2f(s_hi, c_hi);
     }
     // the line voxel -> scatter point, extended 1000 mm, through the sub-cell
     T lthru;
