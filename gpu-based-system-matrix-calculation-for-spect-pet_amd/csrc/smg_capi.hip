@@ -77,6 +77,8 @@ struct smg_ctx {
     DevBuf<ScatterPt<double>> pts_bar;   // bar centres (crystal scatter)
     DevBuf<ScatterPt<double>> pts_hole;  // layer-0 hole centres (collimator scatter)
     DevBuf<PairTerm> pair;
+    MuTables mu_tab{};                 // host copy; n = 0: power laws
+    DevBuf<MuTables> mu_tab_d;
 };
 
 namespace {
@@ -314,6 +316,15 @@ smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
     A.kn_norm = kn_norm(A.e0);
     A.coll_ratio = (double)hg.coll[17] / (double)hg.coll[15];
     if (collimator && !(hg.coll[15] != 0.0f)) A.coll_ratio = 0.0;
+    A.tables = nullptr;
+    if (c->mu_tab.n > 0) {
+        const double le0 = std::log(A.e0);
+        for (int t = 0; t < 4; ++t) c->mu_tab.lmu0[t] = mu_table_log(c->mu_tab, t, le0);
+        SMG_HIP(c, c->mu_tab_d.reserve(1));
+        SMG_HIP(c, hipMemcpyAsync(c->mu_tab_d.ptr, &c->mu_tab, sizeof(MuTables), hipMemcpyHostToDevice,
+                                  c->stream));
+        A.tables = c->mu_tab_d.ptr;
+    }
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
     SMG_HIP(c, launch_pair_terms(A, c->stream));
@@ -407,6 +418,7 @@ smg_status smg_destroy(smg_ctx* c) {
     c->pts_bar.release();
     c->pts_hole.release();
     c->pair.release();
+    c->mu_tab_d.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -484,6 +496,36 @@ smg_status smg_set_pe_subsamples(smg_ctx* c, int sx, int sy, int sz) {
     c->pe_sub[0] = sx;
     c->pe_sub[1] = sy;
     c->pe_sub[2] = sz;
+    return SMG_OK;
+}
+
+smg_status smg_set_mu_tables(smg_ctx* c, int n_nodes, const double* energy_kev,
+                             const double* crystal_pe, const double* crystal_compton,
+                             const double* coll_pe, const double* coll_compton) {
+    if (!c) return SMG_ERR_BAD_PARAM;
+    if (n_nodes == 0) {
+        SMG_HIP(c, hipStreamSynchronize(c->stream));
+        c->mu_tab.n = 0;
+        return SMG_OK;
+    }
+    if (n_nodes < 2 || n_nodes > kMaxMuNodes)
+        return fail(c, SMG_ERR_BAD_PARAM, "mu(E) tables need 2..%d nodes (got %d)", kMaxMuNodes, n_nodes);
+    if (!energy_kev || !crystal_pe || !crystal_compton || !coll_pe || !coll_compton)
+        return fail(c, SMG_ERR_BAD_PARAM, "mu(E) table pointer is NULL");
+    const double* v[4] = {crystal_pe, crystal_compton, coll_pe, coll_compton};
+    MuTables t{};
+    t.n = n_nodes;
+    for (int i = 0; i < n_nodes; ++i) {
+        if (!(energy_kev[i] > 0.0) || (i > 0 && !(energy_kev[i] > energy_kev[i - 1])))
+            return fail(c, SMG_ERR_BAD_PARAM, "mu(E) node energies must be positive and increasing");
+        t.le[i] = std::log(energy_kev[i]);
+        for (int k = 0; k < 4; ++k) {
+            if (!(v[k][i] > 0.0)) return fail(c, SMG_ERR_BAD_PARAM, "mu(E) values must be positive");
+            t.lmu[k][i] = std::log(v[k][i]);
+        }
+    }
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    c->mu_tab = t;
     return SMG_OK;
 }
 

@@ -68,6 +68,7 @@ struct ScatterLaunch {
     double kn_norm;                  // KN sum over [0, pi]
     double coll_ratio;               // mu_compton / mu_total of collimator layer 0
     int same_window, compton_on;
+    const MuTables* tables;          // mu(E) tables, or nullptr for the power laws
 };
 
 hipError_t launch_pair_terms(const ScatterLaunch& a, hipStream_t stream);

@@ -128,6 +128,7 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A) {
         ph.hi = (1 + bd.res / 2.0) * A.e0;
     }
     ph.norm = A.kn_norm;
+    ph.tab = A.tables;
     ph.res = bd.res;
     ph.mupe = bd.mupe;
     ph.muc = bd.muc;

@@ -24,12 +24,50 @@ struct PairTerm {
     double s_pe, s_c, la, len;  // len = |scatter point - sub-cell centre|
 };
 
+// mu(E) tables (an extension with no reference counterpart): per process
+// (0 crystal photoelectric, 1 crystal Compton, 2 collimator photoelectric,
+// 3 collimator Compton) log mu on increasing log-energy nodes, linear in
+// (log E, log mu) between nodes and continuing the end segments beyond them;
+// lmu0 holds each table at E0.  n = 0: the reference's power laws.
+constexpr int kMaxMuNodes = 256;
+struct MuTables {
+    int n;
+    double le[kMaxMuNodes];
+    double lmu[4][kMaxMuNodes];
+    double lmu0[4];
+};
+
+SMG_FN double mu_table_log(const MuTables& t, int which, double le) {
+    int lo = 0, hi = t.n - 1;
+    while (hi - lo > 1) {  // the last node not above le, within [0, n - 2]
+        const int mid = (lo + hi) >> 1;
+        if (t.le[mid] <= le) lo = mid;
+        else hi = mid;
+    }
+    const double f = (le - t.le[lo]) / (t.le[lo + 1] - t.le[lo]);
+    return t.lmu[which][lo] + f * (t.lmu[which][lo + 1] - t.lmu[which][lo]);
+}
+
+// mu(E')/mu(E0) of the photoelectric (r3) and Compton (r1) processes of the
+// crystal: the tables when given, else (E0/E')^3 and E0/E' (scatter.cu:1371-1375).
+SMG_FN void mu_ratios(const MuTables* t, double e0, double ep, double& r3, double& r1) {
+    if (!t || t->n == 0) {
+        r1 = e0 / ep;
+        r3 = e0 * e0 * e0 / ep / ep / ep;
+        return;
+    }
+    const double le = log(ep);
+    r3 = exp(mu_table_log(*t, 0, le) - t->lmu0[0]);
+    r1 = exp(mu_table_log(*t, 1, le) - t->lmu0[1]);
+}
+
 struct ScatterPhys {
     double e0, lo, hi;     // photopeak energy and energy window (keV)
     double norm;           // sum of KN sin over [0, pi] (scatter.cu:1169)
     double res;            // energy resolution of bar d at E0
     double rcell;          // bounding radius of a detection sub-cell
     double mupe, muc;      // bar d
+    const MuTables* tab = nullptr;  // mu(E) tables, or power laws
 };
 
 // Thresholds with their reference values.
@@ -264,8 +302,8 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
     T K = kn_sum<T>(tmin, tmax, alpha_t, near) / T(ph.norm);
 
     // second leg and absorption at E' (scatter.cu:1371-1427)
-    const double e1 = ph.e0 / ep;
-    const double e3 = ph.e0 * ph.e0 * ph.e0 / ep / ep / ep;
+    double e3, e1;
+    mu_ratios(ph.tab, ph.e0, ep, e3, e1);
     double att2 = e3 * sc.pt.s_pe + e1 * sc.pt.s_c;
     double mupe = ph.mupe * e3;
     double mut = mupe + ph.muc * e1;

@@ -35,7 +35,7 @@ EXPORTS = (
     "smg_set_pe_subsamples", "smg_set_scatter_subsamples", "smg_pe_forward", "smg_pe_backward",
     "smg_bitmap_crystal_words", "smg_bitmap_collimator_words", "smg_bitmap_crystal",
     "smg_bitmap_collimator", "smg_load_bitmap_crystal", "smg_load_bitmap_collimator",
-    "smg_scatter_crystal", "smg_scatter_collimator",
+    "smg_scatter_crystal", "smg_scatter_collimator", "smg_set_mu_tables",
 )
 
 
@@ -91,6 +91,7 @@ def lib():
                                C.POINTER(C.c_int)]),
         "smg_set_pe_subsamples": (i32, [vp, i32, i32, i32]),
         "smg_set_scatter_subsamples": (i32, [vp, i32, i32, i32]),
+        "smg_set_mu_tables": (i32, [vp, i32, dp, dp, dp, dp, dp]),
         "smg_pe_forward": (i32, [vp, i32, i32, vp, i32, st]),
         "smg_pe_backward": (i32, [vp, i32, vp, i32, dp, dp, st]),
         "smg_bitmap_crystal_words": (i64, [i32]),
@@ -117,6 +118,10 @@ def device_count() -> int:
     n = C.c_int(0)
     lib().smg_device_count(C.byref(n))
     return n.value
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
 def _fptr(a: np.ndarray):
@@ -188,6 +193,21 @@ class Context:
 
     def set_scatter_subsamples(self, sx: int, sy: int, sz: int):
         self._check(self._L.smg_set_scatter_subsamples(self._h, sx, sy, sz))
+
+    def set_mu_tables(self, energy=None, crystal_pe=None, crystal_compton=None, coll_pe=None,
+                      coll_compton=None):
+        """mu(E) tables (node energies in keV, increasing; one value array per
+        process) for the scatter kernels; None restores the power laws."""
+        if energy is None:
+            z = np.zeros(1)
+            self._check(self._L.smg_set_mu_tables(self._h, 0, *[_dptr(z)] * 5))
+            return
+        arrs = [np.ascontiguousarray(a, np.float64) for a in
+                (energy, crystal_pe, crystal_compton, coll_pe, coll_compton)]
+        n = len(arrs[0])
+        if any(len(a) != n for a in arrs):
+            raise ValueError("mu(E) tables must have one value per node")
+        self._check(self._L.smg_set_mu_tables(self._h, n, *[_dptr(a) for a in arrs]))
 
     # ---- PE ----------------------------------------------------------------------
     def pe_forward(self, rotation: int = 0, precision: str = "fp32", out=None,

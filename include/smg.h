@@ -94,6 +94,18 @@ smg_status smg_set_pe_subsamples(smg_ctx* ctx, int sx, int sy, int sz);
 /* Detection sub-cells of the scatter matrices; the reference uses 1 x 1 x 1
  * (ScatterGen_RayTracing_CircularHole/scatter.cu:1140), the default here. */
 smg_status smg_set_scatter_subsamples(smg_ctx* ctx, int sx, int sy, int sz);
+/* mu(E) tables for the Compton kernels' second leg and absorption at E'.  No
+ * counterpart in the reference, which scales each coefficient from E0 by
+ * (E0/E')^3 (photoelectric) and E0/E' (Compton) (ScatterGen_RayTracing_
+ * CircularHole/scatter.cu:1371-1375).  With tables, each bar's coefficients
+ * at E0 are scaled by mu(E')/mu(E0) of the crystal tables and the collimator
+ * coefficients by the collimator tables; interpolation is linear in
+ * (log E, log mu) between nodes and continues the end segments beyond them.
+ * n_nodes: 2..256 increasing energies in keV, positive values; 0 restores the
+ * power laws (the default). */
+smg_status smg_set_mu_tables(smg_ctx* ctx, int n_nodes, const double* energy_kev,
+                             const double* crystal_pe, const double* crystal_compton,
+                             const double* coll_pe, const double* coll_compton);
 
 /* ---- photoelectric matrix (replaces PESysMatGen) ---------------------------
  * Writes the N x V block of rotation `rotation` to dst. */

@@ -85,6 +85,10 @@ def lib():
     L.smg_o_bit_collimator.restype = C.c_int
     L.smg_o_bit_collimator.argtypes = [_up, C.c_int, C.c_int, C.c_int, C.c_int]
     L.smg_o_crystal_scatter_element.restype = C.c_double
+    L.smg_o_set_mu_tables.restype = C.c_int
+    L.smg_o_set_mu_tables.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.smg_o_mu_ratio.restype = C.c_double
+    L.smg_o_mu_ratio.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
     L.smg_o_crystal_scatter_element.argtypes = [C.c_void_p, _fp, _up] + [C.c_int] * 6
     L.smg_o_collimator_scatter_element.restype = C.c_double
     L.smg_o_collimator_scatter_element.argtypes = [C.c_void_p, _up] + [C.c_int] * 7
@@ -216,6 +220,20 @@ class Oracle:
 
     def coll_pred(self, h, j, k) -> int:
         return lib().smg_o_coll_pred(self._g, h, j, k)
+
+    def set_mu_tables(self, energy=None, crystal_pe=None, crystal_c=None, coll_pe=None,
+                      coll_c=None):
+        """mu(E) tables (keV, increasing; one array per process); None clears them."""
+        if energy is None:
+            lib().smg_o_set_mu_tables(self._g, 0, *([np.zeros(1)] * 5))
+            return
+        arrs = [np.ascontiguousarray(a, np.float64) for a in
+                (energy, crystal_pe, crystal_c, coll_pe, coll_c)]
+        if lib().smg_o_set_mu_tables(self._g, len(arrs[0]), *arrs) != 0:
+            raise ValueError("bad mu(E) tables")
+
+    def mu_ratio(self, which, e, e0) -> float:
+        return lib().smg_o_mu_ratio(self._g, which, e, e0)
 
     def crystal_scatter_pairs(self, pe, bitmap, d, v, rot=0, sub=(1, 1, 1), nthreads=0):
         pe = np.ascontiguousarray(pe, dtype=np.float32).reshape(-1)

@@ -40,6 +40,8 @@ static _Thread_local double g_ops = 0.0;
         if (g_count_ops) g_ops += (n);  \
     } while (0)
 
+#define SMG_O_MAX_NODES 256
+
 struct smg_o_geom {
     const float* coll;
     const float* det;
@@ -48,6 +50,11 @@ struct smg_o_geom {
     int n_layers, n_det, nx, ny, nz, n_vox, n_holes_total, n_holes_l0;
     int holes_in_layer[16];
     double F;
+    /* mu(E) tables: log E of the nodes, log mu per process (0 crystal pe,
+     * 1 crystal compton, 2 collimator pe, 3 collimator compton) */
+    int n_tab;
+    double tab_le[SMG_O_MAX_NODES];
+    double tab_lmu[4][SMG_O_MAX_NODES];
 };
 
 smg_o_geom* smg_o_geom_new(const float* coll, const float* det, const float* img,
@@ -795,6 +802,56 @@ void smg_o_bitmap_collimator(const smg_o_geom* g, uint32_t* words, int nthreads)
 }
 
 /* ======================================================================== */
+/* mu(E) tables (an extension with no reference counterpart)               */
+/* ======================================================================== */
+
+/* Tables on increasing node energies (keV), one per process; interpolation
+ * is linear in (log E, log mu) between nodes and continues the end segments
+ * beyond them.  n = 0 returns to the reference's power laws. */
+int smg_o_set_mu_tables(smg_o_geom* g, int n, const double* energy, const double* crystal_pe,
+                        const double* crystal_c, const double* coll_pe, const double* coll_c) {
+    if (n == 0) {
+        g->n_tab = 0;
+        return 0;
+    }
+    if (n < 2 || n > SMG_O_MAX_NODES) return -1;
+    const double* v[4] = {crystal_pe, crystal_c, coll_pe, coll_c};
+    for (int i = 0; i < n; ++i) {
+        if (!(energy[i] > 0.0) || (i > 0 && !(energy[i] > energy[i - 1]))) return -1;
+        g->tab_le[i] = log(energy[i]);
+        for (int t = 0; t < 4; ++t) {
+            if (!(v[t][i] > 0.0)) return -1;
+            g->tab_lmu[t][i] = log(v[t][i]);
+        }
+    }
+    g->n_tab = n;
+    return 0;
+}
+
+static double tab_lmu_at(const smg_o_geom* g, int which, double le) {
+    int lo = 0, hi = g->n_tab - 1;
+    while (hi - lo > 1) { /* the last node not above le, within [0, n - 2] */
+        int mid = (lo + hi) / 2;
+        if (g->tab_le[mid] <= le) lo = mid;
+        else hi = mid;
+    }
+    double f = (le - g->tab_le[lo]) / (g->tab_le[lo + 1] - g->tab_le[lo]);
+    return g->tab_lmu[which][lo] + f * (g->tab_lmu[which][lo + 1] - g->tab_lmu[which][lo]);
+}
+
+/* mu(e) / mu(e0) of process `which` (0 crystal pe, 1 crystal compton,
+ * 2 collimator pe, 3 collimator compton): from the tables when set, else the
+ * reference's scaling (e0/e)^3 for photoelectric and e0/e for Compton
+ * (SC:1371-1375). */
+double smg_o_mu_ratio(const smg_o_geom* g, int which, double e, double e0) {
+    if (g->n_tab == 0) {
+        double r = e0 / e;
+        return (which & 1) ? r : r * r * r;
+    }
+    return exp(tab_lmu_at(g, which, log(e)) - tab_lmu_at(g, which, log(e0)));
+}
+
+/* ======================================================================== */
 /* Primary Compton scatter (SC:1077-1437, SC:1544-1936, SCC:1207-1215)      */
 /* ======================================================================== */
 
@@ -872,7 +929,14 @@ static double scatter_body(const smg_o_geom* g, const bar_t* bd, int d, const do
                 if (lthru > (double)0.001f) tmin = 0.0;
                 double ratio = smg_o_kn_sum(ph->e0, tmin, tmax, KN_STEP) / ph->norm;
                 /* second leg: bars flagged in the bitmap, mu scaled to E' (SC:1343-1401) */
-                double e3 = ph->e0 * ph->e0 * ph->e0 / ep / ep / ep, e1 = ph->e0 / ep;
+                double e3, e1; /* mu(E')/mu(E0): photoelectric, Compton (crystal) */
+                if (g->n_tab == 0) {
+                    e3 = ph->e0 * ph->e0 * ph->e0 / ep / ep / ep;
+                    e1 = ph->e0 / ep;
+                } else {
+                    e3 = smg_o_mu_ratio(g, 0, ep, ph->e0);
+                    e1 = smg_o_mu_ratio(g, 1, ep, ph->e0);
+                }
                 double att = 0.0;
                 for (int k = 0; k < N; ++k) {
                     if (k == d) continue;

@@ -84,6 +84,15 @@ int smg_o_coll_pred(const smg_o_geom* g, int i, int j, int k);
 int smg_o_bit_crystal(const uint32_t* words, int n_det, int i, int j, int k);
 int smg_o_bit_collimator(const uint32_t* words, int n_det, int i, int j, int k);
 
+/* ---- mu(E) tables (extension, no reference counterpart) ------------------ */
+/* Node energies (keV, increasing) and per-process values: crystal pe, crystal
+ * Compton, collimator pe, collimator Compton.  Linear in (log E, log mu).
+ * n = 0 restores the reference's power laws.  Returns 0, or -1 on bad input. */
+int smg_o_set_mu_tables(smg_o_geom* g, int n, const double* energy, const double* crystal_pe,
+                        const double* crystal_c, const double* coll_pe, const double* coll_c);
+/* mu(e)/mu(e0) of process `which` (0..3 as above). */
+double smg_o_mu_ratio(const smg_o_geom* g, int which, double e, double e0);
+
 /* ---- primary Compton scatter -------------------------------------------- */
 /* pe is the [N][V] PE slice of the voxel grid; sx,sy,sz = detection-cell
  * subsampling (the reference uses 1,1,1). */

@@ -131,3 +131,63 @@ def test_compton_off_gives_zero(smg_mod, params_mod):
     with smg_mod.open_context(p) as ctx:
         a, _ = ctx.scatter_collimator(0, "combined", "fp32")
     assert np.all(a == 0)
+
+
+def mu_tables(power_law: bool, e0: float = 140.5):
+    """64 log-spaced nodes on 20-200 keV (crystal pe, crystal Compton,
+    collimator pe, collimator Compton).  power_law: mu(E0) (E0/E)^3 and
+    mu(E0) E0/E, which log-log interpolation reproduces exactly; otherwise
+    smooth tables with other slopes and a modulation."""
+    E = 20.0 * 10.0 ** (np.arange(64) / 63.0)
+    r = e0 / E
+    if power_law:
+        return E, 0.18 * r ** 3, 0.07 * r, 3.3 * r ** 3, 0.35 * r
+    lE = np.log(E)
+    return (E, 0.18 * r ** 2.7 * (1 + 0.08 * np.sin(3 * lE)), 0.07 * r ** 0.85 * (1 + 0.05 * np.cos(2 * lE)),
+            3.3 * r ** 2.9 * (1 + 0.06 * np.sin(2.5 * lE)), 0.35 * r ** 1.1)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("fp64", 1e-12)])
+def test_power_law_tables_equal_power_laws(smg_mod, params_mod, prec, tol):
+    """Per-process power-law tables give the reference's scaling: the same
+    matrices as without tables, up to the rounding of exp(log)."""
+    p = params_mod.config(3)
+    with smg_mod.open_context(p) as ctx:
+        ctx.set_pe_subsamples(2, 2, 2)
+        pe, _ = ctx.pe_forward(0, "fp32")
+        a_coll, _ = ctx.scatter_collimator(0, "combined", prec)
+        a_cry, _ = ctx.scatter_crystal(pe, 0, prec)
+        ctx.set_mu_tables(*mu_tables(True))
+        b_coll, _ = ctx.scatter_collimator(0, "combined", prec)
+        b_cry, _ = ctx.scatter_crystal(pe, 0, prec)
+        ctx.set_mu_tables(None)
+        c_coll, _ = ctx.scatter_collimator(0, "combined", prec)
+    assert np.array_equal(a_coll, c_coll)
+    for a, b in ((a_coll, b_coll), (a_cry, b_cry)):
+        m = a > 0
+        assert m.mean() > 0.05
+        assert np.array_equal(a == 0, b == 0)
+        assert np.max(np.abs(b[m] / a[m] - 1)) <= tol
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
+def test_mu_tables_vs_oracle(smg_mod, oracle_mod, params_mod, prec, tol):
+    """Smooth non-power-law tables: collimator and crystal scatter against
+    the oracle with the same tables."""
+    p = params_mod.config(3)
+    o = oracle_mod.Oracle(p)
+    tabs = mu_tables(False)
+    o.set_mu_tables(*tabs)
+    d, v = _sample(o, 1500, 12)
+    with smg_mod.open_context(p) as ctx:
+        ctx.set_mu_tables(*tabs)
+        ctx.set_pe_subsamples(2, 2, 2)
+        pe, _ = ctx.pe_forward(0, "fp32")
+        got_coll, _ = ctx.scatter_collimator(0, "combined", prec)
+        got_cry, _ = ctx.scatter_crystal(pe, 0, prec)
+        bl = ctx.bitmap_collimator()
+        bc = ctx.bitmap_crystal()
+    assert_parity(got_coll[d, v], o.collimator_scatter_pairs(bl, d, v, variant=0), tol,
+                  f"collimator scatter, mu(E) tables, {prec}")
+    assert_parity(got_cry[d, v], o.crystal_scatter_pairs(pe, bc, d, v), tol,
+                  f"crystal scatter, mu(E) tables, {prec}")

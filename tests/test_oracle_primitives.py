@@ -245,3 +245,31 @@ def test_gauss_window(oracle_mod):
             sigma = res / fwhm * e
             ref = special.ndtr((154.55 - e) / sigma) - special.ndtr((126.45 - e) / sigma)
             assert oracle_mod.gauss_window(e, res, 126.45, 154.55) == pytest.approx(ref, rel=1e-12, abs=1e-15)
+
+
+def test_mu_tables_loglog_interpolation(oracle_mod, params_mod):
+    """The oracle's mu(E) tables: power laws are reproduced exactly, between
+    two nodes log mu is linear in log E (the value at the geometric mean of
+    the node energies is the geometric mean of the node values), and the end
+    segments continue beyond the nodes."""
+    o = oracle_mod.Oracle(params_mod.config(1))
+    E = 20.0 * 10.0 ** (np.arange(64) / 63.0)
+    e0 = 140.5
+    o.set_mu_tables(E, 0.18 * (e0 / E) ** 3, 0.07 * (e0 / E), 3.3 * (e0 / E) ** 3, 0.35 * (e0 / E))
+    for e in (15.0, 90.6, 126.45, 140.5, 154.55, 230.0):
+        assert o.mu_ratio(0, e, e0) == pytest.approx((e0 / e) ** 3, rel=1e-13)
+        assert o.mu_ratio(1, e, e0) == pytest.approx(e0 / e, rel=1e-13)
+        assert o.mu_ratio(2, e, e0) == pytest.approx((e0 / e) ** 3, rel=1e-13)
+    # two nodes, arbitrary values
+    En = np.array([100.0, 150.0])
+    v = np.array([7.0, 2.0])
+    o.set_mu_tables(En, v, v[::-1].copy(), v, v)
+    gm = np.sqrt(100.0 * 150.0)
+    assert o.mu_ratio(0, gm, 100.0) == pytest.approx(np.sqrt(7.0 * 2.0) / 7.0, rel=1e-13)
+    assert o.mu_ratio(1, gm, 150.0) == pytest.approx(np.sqrt(7.0 * 2.0) / 7.0, rel=1e-13)
+    slope = np.log(2.0 / 7.0) / np.log(1.5)
+    assert o.mu_ratio(0, 50.0, 100.0) == pytest.approx(0.5 ** slope, rel=1e-13)
+    with pytest.raises(ValueError):
+        o.set_mu_tables(En[::-1].copy(), v, v, v, v)
+    o.set_mu_tables(None)
+    assert o.mu_ratio(0, 100.0, 140.5) == pytest.approx((1.405) ** 3, rel=1e-15)
