@@ -79,6 +79,8 @@ struct smg_ctx {
     DevBuf<PairTerm> pair;
     MuTables mu_tab{};                 // host copy; n = 0: power laws
     DevBuf<MuTables> mu_tab_d;
+    DevBuf<ScatterPt<double>> pts_grid;  // collimator grid scatterers
+    DevBuf<float> w1;                    // their first-leg weights, one chunk
 };
 
 namespace {
@@ -325,6 +327,8 @@ smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
                                   c->stream));
         A.tables = c->mu_tab_d.ptr;
     }
+    A.w1 = nullptr;
+    A.accumulate = 0;
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
     SMG_HIP(c, launch_pair_terms(A, c->stream));
@@ -419,6 +423,8 @@ smg_status smg_destroy(smg_ctx* c) {
     c->pts_hole.release();
     c->pair.release();
     c->mu_tab_d.release();
+    c->pts_grid.release();
+    c->w1.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -626,6 +632,102 @@ smg_status smg_scatter_collimator(smg_ctx* c, int rotation, smg_coll_variant var
     if (variant != SMG_COLL_COMBINED && variant != SMG_COLL_STANDALONE)
         return fail(c, SMG_ERR_BAD_PARAM, "unknown collimator variant %d", (int)variant);
     return scatter_entry(c, true, rotation, (int)variant, prec, nullptr, 0, dst, dst_is_device, stats);
+}
+
+smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm,
+                                       smg_precision prec, float* dst, int dst_is_device,
+                                       int64_t* n_points, smg_stats* stats) {
+    double t_start = now_ms();
+    smg_status s = check_ready(c, rotation);
+    if (s != SMG_OK) return s;
+    if (!dst) return fail(c, SMG_ERR_BAD_PARAM, "dst is NULL");
+    if (!(pitch_mm > 0.0)) return fail(c, SMG_ERR_BAD_PARAM, "grid pitch must be positive");
+    const HostGeom& hg = c->hg;
+    const GeomDesc& g = hg.g;
+    if (g.n_layers < 1) return fail(c, SMG_ERR_BAD_PARAM, "the collimator has no layer");
+    const size_t n_el = (size_t)g.n_det * (size_t)g.n_vox;
+    std::vector<ScatterPt<double>> pts;
+    double dv = 0.0;
+    make_coll_grid(hg, pitch_mm, pts, dv);
+    if (pts.size() > (size_t)INT32_MAX)
+        return fail(c, SMG_ERR_CAPACITY, "%zu grid points", pts.size());
+    if (n_points) *n_points = (int64_t)pts.size();
+    if ((s = upload(c, c->pts_grid, pts)) != SMG_OK) return s;
+    const int ncell = c->sc_sub[0] * c->sc_sub[1] * c->sc_sub[2];
+    // chunks of at most 65536 scatter points, first-leg weights <= 2^28 floats,
+    // pair terms <= 2^25 records
+    int64_t chunk = std::min<int64_t>(65536, std::max<int64_t>(1, ((int64_t)1 << 28) / std::max<int64_t>(1, g.n_vox)));
+    chunk = std::min<int64_t>(chunk, std::max<int64_t>(1, ((int64_t)1 << 25) / ((int64_t)g.n_det * ncell)));
+    chunk = std::min<int64_t>(chunk, std::max<int64_t>(1, (int64_t)pts.size()));
+    SMG_HIP(c, c->w1.reserve((size_t)chunk * (size_t)g.n_vox));
+    SMG_HIP(c, c->pair.reserve((size_t)g.n_det * (size_t)chunk * ncell));
+    ScatterLaunch A{};
+    A.geom = device_geom(c);
+    A.kind = 2;
+    A.variant = 0;
+    A.bitmap = nullptr;
+    A.pe = nullptr;
+    A.pair = c->pair.ptr;
+    A.w1 = c->w1.ptr;
+    float* out = dst;
+    if (!dst_is_device) {
+        SMG_HIP(c, c->out.reserve(n_el));
+        out = c->out.ptr;
+    }
+    A.dst = out;
+    A.counters = c->counters.ptr;
+    const double phi = (double)rotation * g.dphi;
+    A.cphi = std::cos(phi);
+    A.sphi = std::sin(phi);
+    for (int k = 0; k < 3; ++k) A.sub[k] = c->sc_sub[k];
+    const float* ph = hg.phys.data();
+    A.compton_on = (int)std::floor((double)ph[0] + 0.5) == 1;
+    A.same_window = (int)std::floor((double)ph[4] + 0.5) > 0;
+    A.win_lo = ph[5];
+    A.win_hi = ph[6];
+    A.e0 = ph[7];
+    A.kn_norm = kn_norm(A.e0);
+    A.coll_ratio = 0.0;
+    A.tables = nullptr;
+    if (c->mu_tab.n > 0) {
+        const double le0 = std::log(A.e0);
+        for (int t = 0; t < 4; ++t) c->mu_tab.lmu0[t] = mu_table_log(c->mu_tab, t, le0);
+        SMG_HIP(c, c->mu_tab_d.reserve(1));
+        SMG_HIP(c, hipMemcpyAsync(c->mu_tab_d.ptr, &c->mu_tab, sizeof(MuTables), hipMemcpyHostToDevice,
+                                  c->stream));
+        A.tables = c->mu_tab_d.ptr;
+    }
+    const double muc0 = hg.layers_d[0].muc;
+    SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    if (pts.empty()) {
+        SMG_HIP(c, hipMemsetAsync(out, 0, n_el * sizeof(float), c->stream));
+    }
+    for (int64_t s0 = 0; s0 < (int64_t)pts.size(); s0 += chunk) {
+        A.pts_d = c->pts_grid.ptr + s0;
+        A.n_pts = (int)std::min<int64_t>(chunk, (int64_t)pts.size() - s0);
+        A.accumulate = s0 > 0;
+        SMG_HIP(c, launch_first_leg(A, dv, muc0, c->w1.ptr, c->stream));
+        SMG_HIP(c, launch_pair_terms(A, c->stream));
+        SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));
+    }
+    SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    if (!dst_is_device)
+        SMG_HIP(c, hipMemcpyAsync(dst, out, n_el * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    SMG_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t_start;
+        stats->elements = n_el;
+        stats->rays = n_el * (uint64_t)pts.size() * (uint64_t)ncell;
+        stats->fp64_fallbacks = cnt[0];
+        stats->overflow_waves = 0;
+    }
+    return SMG_OK;
 }
 
 smg_status smg_pe_backward(smg_ctx* c, int rotation, const float* w, int w_is_device, double* grad,

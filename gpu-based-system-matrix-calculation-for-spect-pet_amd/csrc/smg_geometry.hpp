@@ -35,12 +35,14 @@ struct Hole {
     T r, mu;           // radius, mu_total of the hole filling
     T ymid;            // (y1 + y2) / 2
     T crit;            // sqrt(4 r^2 + (y1 - y2)^2) / 2: culling uses crit / D
+    T mupe, muc;       // photoelectric and Compton mu of the filling
 };
 
 template <class T>
 struct Layer {
     T lo[3], hi[3];   // plate box (world frame, F included)
     T thick, mu;      // thickness (y) and mu_total of the plate
+    T mupe, muc;      // photoelectric and Compton mu of the plate
 };
 
 // Bounding sphere of a bar, or centre of a layer-0 hole, in the raw

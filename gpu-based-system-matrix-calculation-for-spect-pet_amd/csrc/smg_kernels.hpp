@@ -55,7 +55,8 @@ struct ScatterLaunch {
     PeGeom geom;
     const ScatterPt<double>* pts_d;  // scatter points
     int n_pts;
-    int kind;                        // 0: crystal scatter, 1: collimator scatter
+    int kind;                        // 0: crystal scatter, 1: collimator scatter,
+                                     // 2: collimator scatter from grid points
     int variant;                     // collimator: 0 combined (r = 0.1), 1 standalone (r = 0.2)
     const float* pe;                 // crystal: PE block [N][V]
     const uint32_t* bitmap;          // crystal or collimator bitmap
@@ -69,7 +70,14 @@ struct ScatterLaunch {
     double coll_ratio;               // mu_compton / mu_total of collimator layer 0
     int same_window, compton_on;
     const MuTables* tables;          // mu(E) tables, or nullptr for the power laws
+    const float* w1;                 // grid: first-leg weights [n_pts][V] of this chunk
+    int accumulate;                  // add to dst instead of writing it
 };
+
+// Grid scatterers: first-leg weights w1[s][v] = dv mu_c0 / (4 pi L^2) exp(-att)
+// of the chunk's scatter points.
+hipError_t launch_first_leg(const ScatterLaunch& a, double dv, double muc0, float* w1,
+                            hipStream_t stream);
 
 hipError_t launch_pair_terms(const ScatterLaunch& a, hipStream_t stream);
 hipError_t launch_scatter(const ScatterLaunch& a, bool exact, hipStream_t stream);

@@ -86,6 +86,44 @@ inline void make_group_boxes(HostGeom& hg) {
     }, hg.hole_groups);
 }
 
+// Scatter points on a grid over the layer-0 plate (the collimator
+// grid-scatterer mode, an extension with no reference counterpart):
+// n_k = max(1, round(extent / pitch)) cells per axis, points at the cell
+// centres, z outermost and x innermost, without the points inside a layer-0
+// hole; dv = the cell volume.
+inline void make_coll_grid(const HostGeom& hg, double pitch, std::vector<ScatterPt<double>>& pts,
+                           double& dv) {
+    pts.clear();
+    dv = 0.0;
+    if (hg.g.n_layers < 1 || !(pitch > 0.0)) return;
+    const Layer<double>& L = hg.layers_d[0];
+    int n[3];
+    double pk[3];
+    // extents as the parameters give them: width, thickness, height
+    const double exts[3] = {L.hi[0] - L.lo[0], L.thick, L.hi[2] - L.lo[2]};
+    for (int k = 0; k < 3; ++k) {
+        const double ext = exts[k];
+        n[k] = (int)std::floor(ext / pitch + 0.5);
+        if (n[k] < 1) n[k] = 1;
+        pk[k] = ext / n[k];
+    }
+    dv = pk[0] * pk[1] * pk[2];
+    const int nh = hg.g.ranges[0].count;
+    for (int kz = 0; kz < n[2]; ++kz)
+        for (int jy = 0; jy < n[1]; ++jy)
+            for (int ix = 0; ix < n[0]; ++ix) {
+                const double x = L.lo[0] + (ix + 0.5) * pk[0], y = L.lo[1] + (jy + 0.5) * pk[1],
+                             z = L.lo[2] + (kz + 0.5) * pk[2];
+                bool inside = false;
+                for (int h = 0; h < nh && !inside; ++h) {
+                    const Hole<double>& o = hg.holes_d[h];
+                    const double dx = x - o.xc, dz = z - o.zc;
+                    inside = dx * dx + dz * dz < o.r * o.r && y >= o.y1 && y <= o.y2;
+                }
+                if (!inside) pts.push_back(ScatterPt<double>{x, y, z, 0.0});
+            }
+}
+
 // Parse the four parameter arrays the way the reference reads them.
 inline int parse_geometry(const float* coll, const float* det, const float* img, const float* phys,
                           HostGeom& hg, std::string& why) {
@@ -139,6 +177,8 @@ inline int parse_geometry(const float* coll, const float* det, const float* img,
         L.hi[2] = (double)Lp[3] / 2.0;
         L.thick = Lp[2];
         L.mu = Lp[5];
+        L.mupe = Lp[6];
+        L.muc = Lp[7];
         Layer<float>& Lf = hg.layers_f[m];
         for (int k = 0; k < 3; ++k) {
             Lf.lo[k] = (float)L.lo[k];
@@ -146,6 +186,8 @@ inline int parse_geometry(const float* coll, const float* det, const float* img,
         }
         Lf.thick = (float)L.thick;
         Lf.mu = (float)L.mu;
+        Lf.mupe = (float)L.mupe;
+        Lf.muc = (float)L.muc;
     }
     g.n_holes = first;
     if (g.n_holes > kMaxHoles)
@@ -165,6 +207,8 @@ inline int parse_geometry(const float* coll, const float* det, const float* img,
         o.zc = hp[3];
         o.r = hp[4];
         o.mu = hp[5];
+        o.mupe = hp[6];
+        o.muc = hp[7];
         o.ymid = (o.y1 + o.y2) / 2.0;
         o.crit = std::sqrt(o.r * o.r * 4 + (o.y1 - o.y2) * (o.y1 - o.y2)) / 2.0;
         Hole<float>& of = hg.holes_f[h];
@@ -174,6 +218,8 @@ inline int parse_geometry(const float* coll, const float* det, const float* img,
         of.zc = (float)o.zc;
         of.r = (float)o.r;
         of.mu = (float)o.mu;
+        of.mupe = (float)o.mupe;
+        of.muc = (float)o.muc;
         of.ymid = (float)o.ymid;
         of.crit = (float)o.crit;
     }

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
     const int cell = (int)(tid % ncell);
     const int s = (int)((tid / ncell) % A.n_pts);
     const int d = (int)(tid / ((int64_t)ncell * A.n_pts));
-    PairTerm out{0.0, 0.0, 0.0, 0.0};
+    PairTerm out{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (A.kind == 0 && s == d) {
         A.pair[tid] = out;
         return;
@@ -50,17 +50,18 @@ __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
     const ScatterPt<double>& sp = A.pts_d[s];
     double qs[3] = {sp.x, sp.y, sp.z};
     bool near = false;
-    // bars flagged between the scatter point and bar d (scatter.cu:1343-1401)
-    int64_t base;
+    // bars flagged between the scatter point and bar d (scatter.cu:1343-1401);
+    // grid scatterers: every other bar
+    int64_t base = 0;
     if (A.kind == 0) {
         int64_t i = s < d ? s : d, j = s < d ? d : s;
         base = (i * N - i * (i + 1) / 2 + j) * N;
-    } else {
+    } else if (A.kind == 1) {
         base = ((int64_t)s * N + d) * N;
     }
     for (int k = 0; k < N; ++k) {
         if (k == d || (A.kind == 0 && k == s)) continue;
-        if (!bit_set(A.bitmap, base + k)) continue;
+        if (A.kind != 2 && !bit_set(A.bitmap, base + k)) continue;
         const Bar<double>& bk = G.bars_d[k];
         double pa[3], pb[3];
         to_frame(bk, qs, pa);
@@ -72,6 +73,10 @@ __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
         double l = seg_box<double>(sd, pa, lo, hi, near);
         out.s_pe += l * bk.mupe;
         out.s_c += l * bk.muc;
+    }
+    if (A.kind == 2) {  // collimator material from the grid point to the sub-cell
+        double att_t;
+        coll_material(G, qs, cc.q, att_t, out.s_wpe, out.s_wc);
     }
     // own bar from its surface to the sub-cell (scatter.cu:1405-1417)
     double qss[3];
@@ -168,6 +173,8 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A) {
             T wgt;
             if (A.kind == 0) {
                 wgt = T(A.pe[(int64_t)s * g.n_vox + v]) * wscale;
+            } else if (A.kind == 2) {
+                wgt = T(A.w1[(int64_t)s * g.n_vox + v]);
             } else {
                 double dx = p[0] - sp.x, dy = p[1] - sp.y, dz = p[2] - sp.z;
                 double lv = sqrt(dx * dx + dy * dy + dz * dz);
@@ -195,7 +202,7 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A) {
             acc.add(val);
         }
     }
-    if (valid) out[v] = float(acc.value());
+    if (valid) out[v] = A.accumulate ? out[v] + float(acc.value()) : float(acc.value());
     if constexpr (!kExact) {
         unsigned tot = fallbacks;
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
@@ -203,7 +210,41 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A) {
     }
 }
 
+// Grid scatterers (an extension with no reference counterpart): one thread
+// per (scatter point s, voxel v) of the chunk, the chance of a Compton
+// interaction in the cell of volume dv around s for a photon from v,
+// dv mu_c0 / (4 pi |s - v|^2) exp(-collimator material along v -> s), in FP64.
+__global__ __launch_bounds__(256) void first_leg_kernel(ScatterLaunch A, double dv, double muc0,
+                                                        float* w1) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const GeomDesc& g = A.geom.g;
+    const int64_t total = (int64_t)A.n_pts * g.n_vox;
+    if (tid >= total) return;
+    const int s = (int)(tid / g.n_vox);
+    const int64_t v = tid % g.n_vox;
+    const int nxy = g.nx * g.ny;
+    const int iz = (int)(v / nxy), rem = (int)(v % nxy);
+    double p[3];
+    voxel_position(g, rem % g.nx, rem / g.nx, iz, A.cphi, A.sphi, p);
+    const ScatterPt<double>& sp = A.pts_d[s];
+    const double q[3] = {sp.x, sp.y, sp.z};
+    double att_t, s_pe, s_c;
+    coll_material(A.geom, p, q, att_t, s_pe, s_c);
+    const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    const double L2 = dx * dx + dy * dy + dz * dz;
+    w1[tid] = float(dv * muc0 / (4.0 * 3.14159265358979323846 * L2) * exp(-att_t));
+}
+
 }  // namespace
+
+hipError_t launch_first_leg(const ScatterLaunch& A, double dv, double muc0, float* w1,
+                            hipStream_t stream) {
+    const int64_t total = (int64_t)A.n_pts * A.geom.g.n_vox;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(first_leg_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
+                       A, dv, muc0, w1);
+    return hipGetLastError();
+}
 
 hipError_t launch_pair_terms(const ScatterLaunch& A, hipStream_t stream) {
     const int64_t total = (int64_t)A.geom.g.n_det * A.n_pts * A.sub[0] * A.sub[1] * A.sub[2];

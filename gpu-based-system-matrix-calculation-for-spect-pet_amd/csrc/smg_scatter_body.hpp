@@ -22,7 +22,96 @@ namespace smg {
 
 struct PairTerm {
     double s_pe, s_c, la, len;  // len = |scatter point - sub-cell centre|
+    double s_wpe, s_wc;         // collimator material on the second leg (grid scatterers)
 };
+
+// ---- collimator material for grid scatterers ----------------------------------------
+// (An extension with no reference counterpart.)  Plain intersection lengths of
+// a segment with a box and with a finite cylinder along y, without the
+// reference chord functions' thresholds, in FP64.
+
+SMG_FN double seg_box_geo(const double a[3], const double b[3], const double lo[3],
+                          const double hi[3]) {
+    double t0 = 0.0, t1 = 1.0;
+    for (int k = 0; k < 3; ++k) {
+        const double D = b[k] - a[k];
+        if (D == 0.0) {
+            if (a[k] < lo[k] || a[k] > hi[k]) return 0.0;
+            continue;
+        }
+        double u = (lo[k] - a[k]) / D, w = (hi[k] - a[k]) / D;
+        if (u > w) {
+            const double x = u;
+            u = w;
+            w = x;
+        }
+        t0 = tmax2(t0, u);
+        t1 = tmin2(t1, w);
+    }
+    if (!(t1 > t0)) return 0.0;
+    const double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+    return (t1 - t0) * sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+SMG_FN double seg_cyl_geo(const double a[3], const double b[3], double xc, double y1, double y2,
+                          double zc, double r) {
+    const double D[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double t0 = 0.0, t1 = 1.0;
+    if (D[1] == 0.0) {
+        if (a[1] < y1 || a[1] > y2) return 0.0;
+    } else {
+        double u = (y1 - a[1]) / D[1], w = (y2 - a[1]) / D[1];
+        if (u > w) {
+            const double x = u;
+            u = w;
+            w = x;
+        }
+        t0 = tmax2(t0, u);
+        t1 = tmin2(t1, w);
+    }
+    const double x0 = a[0] - xc, z0 = a[2] - zc;
+    const double A = D[0] * D[0] + D[2] * D[2], B = x0 * D[0] + z0 * D[2],
+                 Cq = x0 * x0 + z0 * z0 - r * r;
+    if (A == 0.0) {
+        if (Cq > 0.0) return 0.0;
+    } else {
+        const double disc = B * B - A * Cq;
+        if (disc <= 0.0) return 0.0;
+        const double sq = sqrt(disc);
+        t0 = tmax2(t0, (-B - sq) / A);
+        t1 = tmin2(t1, (-B + sq) / A);
+    }
+    if (!(t1 > t0)) return 0.0;
+    return (t1 - t0) * sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+}
+
+// Collimator material along a -> b at E0: each layer's plate chord less the
+// chords of its holes (at least 0) with the layer's mu, plus the holes' own
+// mu.  att_t: sum of mu_total l; s_pe, s_c: sums with the photoelectric and
+// Compton coefficients.
+SMG_FN void coll_material(const PeGeom& G, const double a[3], const double b[3], double& att_t,
+                          double& s_pe, double& s_c) {
+    att_t = s_pe = s_c = 0.0;
+    for (int m = 0; m < G.g.n_layers; ++m) {
+        const Layer<double>& L = G.layers_d[m];
+        const double c = seg_box_geo(a, b, L.lo, L.hi);
+        if (!(c > 0.0)) continue;  // a hole of the layer lies inside its plate box
+        double holes = 0.0;
+        const int first = G.g.ranges[m].first, last = first + G.g.ranges[m].count;
+        for (int h = first; h < last; ++h) {
+            const Hole<double>& o = G.holes_d[h];
+            const double l = seg_cyl_geo(a, b, o.xc, o.y1, o.y2, o.zc, o.r);
+            holes += l;
+            att_t += o.mu * l;
+            s_pe += o.mupe * l;
+            s_c += o.muc * l;
+        }
+        const double wl = tmax2(c - holes, 0.0);
+        att_t += L.mu * wl;
+        s_pe += L.mupe * wl;
+        s_c += L.muc * wl;
+    }
+}
 
 // mu(E) tables (an extension with no reference counterpart): per process
 // (0 crystal photoelectric, 1 crystal Compton, 2 collimator photoelectric,
@@ -305,6 +394,15 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
     double e3, e1;
     mu_ratios(ph.tab, ph.e0, ep, e3, e1);
     double att2 = e3 * sc.pt.s_pe + e1 * sc.pt.s_c;
+    if (sc.pt.s_wpe != 0.0 || sc.pt.s_wc != 0.0) {  // collimator material (grid scatterers)
+        double w3 = e3, w1 = e1;
+        if (ph.tab && ph.tab->n > 0) {
+            const double le = log(ep);
+            w3 = exp(mu_table_log(*ph.tab, 2, le) - ph.tab->lmu0[2]);
+            w1 = exp(mu_table_log(*ph.tab, 3, le) - ph.tab->lmu0[3]);
+        }
+        att2 += w3 * sc.pt.s_wpe + w1 * sc.pt.s_wc;
+    }
     double mupe = ph.mupe * e3;
     double mut = mupe + ph.muc * e1;
     T ea = exp_neg<T>(sc.pt.la * mut);

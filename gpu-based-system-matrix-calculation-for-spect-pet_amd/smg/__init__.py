@@ -36,6 +36,7 @@ EXPORTS = (
     "smg_bitmap_crystal_words", "smg_bitmap_collimator_words", "smg_bitmap_crystal",
     "smg_bitmap_collimator", "smg_load_bitmap_crystal", "smg_load_bitmap_collimator",
     "smg_scatter_crystal", "smg_scatter_collimator", "smg_set_mu_tables",
+    "smg_scatter_collimator_grid",
 )
 
 
@@ -102,6 +103,8 @@ def lib():
         "smg_load_bitmap_collimator": (i32, [vp, up, i64]),
         "smg_scatter_crystal": (i32, [vp, i32, i32, vp, i32, vp, i32, st]),
         "smg_scatter_collimator": (i32, [vp, i32, i32, i32, vp, i32, st]),
+        "smg_scatter_collimator_grid": (i32, [vp, i32, C.c_double, i32, vp, i32,
+                                              C.POINTER(C.c_int64), st]),
     }
     missing = [name for name in sig if not hasattr(L, name)]
     if missing:
@@ -281,6 +284,19 @@ class Context:
                                                    out.ctypes.data_as(C.c_void_p), 0,
                                                    C.byref(st)))
         return out, st.as_dict()
+
+    def scatter_collimator_grid(self, pitch_mm: float, rotation: int = 0,
+                                precision: str = "fp32"):
+        """Collimator scatter from grid points over the layer-0 tungsten.
+        Returns (matrix [N][V], stats, number of grid points)."""
+        out = np.empty((self.dims.n_det, self.dims.n_vox), np.float32)
+        st = Stats()
+        npts = C.c_int64(0)
+        self._check(self._L.smg_scatter_collimator_grid(self._h, rotation, float(pitch_mm),
+                                                        PREC[precision],
+                                                        out.ctypes.data_as(C.c_void_p), 0,
+                                                        C.byref(npts), C.byref(st)))
+        return out, st.as_dict(), npts.value
 
 
 def open_context(p, device: int = 0) -> Context:

@@ -85,6 +85,17 @@ def lib():
     L.smg_o_bit_collimator.restype = C.c_int
     L.smg_o_bit_collimator.argtypes = [_up, C.c_int, C.c_int, C.c_int, C.c_int]
     L.smg_o_crystal_scatter_element.restype = C.c_double
+    L.smg_o_seg_box.restype = C.c_double
+    L.smg_o_seg_box.argtypes = [_dp, _dp, _dp, _dp]
+    L.smg_o_seg_cyl.restype = C.c_double
+    L.smg_o_seg_cyl.argtypes = [_dp, _dp] + [C.c_double] * 5
+    L.smg_o_coll_grid_points.restype = C.c_int64
+    L.smg_o_coll_grid_points.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int64,
+                                         C.POINTER(C.c_double)]
+    L.smg_o_coll_grid_weight.restype = C.c_double
+    L.smg_o_coll_grid_weight.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
+    L.smg_o_coll_grid_scatter_pairs.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
+                                                C.c_int, _ip, _ip, C.c_int, _dp, C.c_int]
     L.smg_o_set_mu_tables.restype = C.c_int
     L.smg_o_set_mu_tables.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
     L.smg_o_mu_ratio.restype = C.c_double
@@ -118,6 +129,16 @@ def box_ray_inside(a, b, lo, hi) -> float:
 
 def cylinder_ray(a, b, xc, y1, y2, zc, r) -> float:
     return lib().smg_o_cylinder_ray(_v3(a), _v3(b), xc, y1, y2, zc, r)
+
+
+def seg_box(a, b, lo, hi) -> float:
+    """Length of segment a -> b inside the box (no reference thresholds)."""
+    return lib().smg_o_seg_box(_v3(a), _v3(b), _v3(lo), _v3(hi))
+
+
+def seg_cyl(a, b, xc, y1, y2, zc, r) -> float:
+    """Length of segment a -> b inside the finite cylinder along y."""
+    return lib().smg_o_seg_cyl(_v3(a), _v3(b), xc, y1, y2, zc, r)
 
 
 def kn(theta, e0) -> float:
@@ -234,6 +255,24 @@ class Oracle:
 
     def mu_ratio(self, which, e, e0) -> float:
         return lib().smg_o_mu_ratio(self._g, which, e, e0)
+
+    def coll_grid_points(self, pitch):
+        """(points [n][3], cell volume) of the layer-0 tungsten grid."""
+        dv = C.c_double(0.0)
+        n = lib().smg_o_coll_grid_points(self._g, pitch, None, 0, C.byref(dv))
+        pts = np.zeros((max(n, 1), 3))
+        lib().smg_o_coll_grid_points(self._g, pitch, pts.ctypes.data_as(C.c_void_p), n, C.byref(dv))
+        return pts[:n], dv.value
+
+    def coll_grid_weight(self, p, s, dv) -> float:
+        return lib().smg_o_coll_grid_weight(self._g, _v3(p), _v3(s), dv)
+
+    def coll_grid_scatter_pairs(self, pitch, d, v, rot=0, sub=(1, 1, 1), nthreads=0):
+        d = np.ascontiguousarray(d, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        out = np.zeros(len(d))
+        lib().smg_o_coll_grid_scatter_pairs(self._g, pitch, rot, *sub, d, v, len(d), out, nthreads)
+        return out
 
     def crystal_scatter_pairs(self, pe, bitmap, d, v, rot=0, sub=(1, 1, 1), nthreads=0):
         pe = np.ascontiguousarray(pe, dtype=np.float32).reshape(-1)

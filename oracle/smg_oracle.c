@@ -855,6 +855,144 @@ double smg_o_mu_ratio(const smg_o_geom* g, int which, double e, double e0) {
 /* Primary Compton scatter (SC:1077-1437, SC:1544-1936, SCC:1207-1215)      */
 /* ======================================================================== */
 
+/* ---- geometric chords for the grid-scatterer extension ----------------- */
+/* These have no reference counterpart: plain intersection lengths of a
+ * segment with a box and with a finite cylinder along y, without the
+ * reference chord functions' thresholds. */
+
+/* Length of the segment a -> b inside the box [lo, hi]. */
+double smg_o_seg_box(const double a[3], const double b[3], const double lo[3], const double hi[3]) {
+    double t0 = 0.0, t1 = 1.0;
+    for (int k = 0; k < 3; ++k) {
+        double D = b[k] - a[k];
+        if (D == 0.0) {
+            if (a[k] < lo[k] || a[k] > hi[k]) return 0.0;
+            continue;
+        }
+        double u = (lo[k] - a[k]) / D, w = (hi[k] - a[k]) / D;
+        if (u > w) { double x = u; u = w; w = x; }
+        if (u > t0) t0 = u;
+        if (w < t1) t1 = w;
+    }
+    return t1 > t0 ? (t1 - t0) * dist3(a, b) : 0.0;
+}
+
+/* Length of the segment a -> b inside the cylinder of radius r about the
+ * line x = xc, z = zc, for y1 <= y <= y2. */
+double smg_o_seg_cyl(const double a[3], const double b[3], double xc, double y1, double y2,
+                     double zc, double r) {
+    double D[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double t0 = 0.0, t1 = 1.0;
+    if (D[1] == 0.0) {
+        if (a[1] < y1 || a[1] > y2) return 0.0;
+    } else {
+        double u = (y1 - a[1]) / D[1], w = (y2 - a[1]) / D[1];
+        if (u > w) { double x = u; u = w; w = x; }
+        if (u > t0) t0 = u;
+        if (w < t1) t1 = w;
+    }
+    double x0 = a[0] - xc, z0 = a[2] - zc;
+    double A = D[0] * D[0] + D[2] * D[2], B = x0 * D[0] + z0 * D[2], Cq = x0 * x0 + z0 * z0 - r * r;
+    if (A == 0.0) {
+        if (Cq > 0.0) return 0.0;
+    } else {
+        double disc = B * B - A * Cq;
+        if (disc <= 0.0) return 0.0;
+        double sq = sqrt(disc);
+        double u = (-B - sq) / A, w = (-B + sq) / A;
+        if (u > t0) t0 = u;
+        if (w < t1) t1 = w;
+    }
+    return t1 > t0 ? (t1 - t0) * dist3(a, b) : 0.0;
+}
+
+/* Collimator material along the segment a -> b at E0: for each layer whose
+ * plate box the segment meets, its plate chord less the chords of its holes
+ * (at least 0) with the layer's mu, plus the holes' own mu.  att_t = sum mu_total l; s_pe, s_c the sums with
+ * the photoelectric and Compton coefficients (for the scaling to E'). */
+static void coll_material(const smg_o_geom* g, const double a[3], const double b[3],
+                          double* att_t, double* s_pe, double* s_c) {
+    *att_t = *s_pe = *s_c = 0.0;
+    int h0 = 0;
+    for (int m = 0; m < g->n_layers; ++m) {
+        const float* L = g->coll + (m + 1) * 10;
+        double lo[3] = {-(double)L[1] / 2.0, -(double)L[2] / 2.0 + g->F + L[4], -(double)L[3] / 2.0};
+        double hi[3] = {(double)L[1] / 2.0, (double)L[2] / 2.0 + g->F + L[4], (double)L[3] / 2.0};
+        double c = smg_o_seg_box(a, b, lo, hi);
+        if (!(c > 0.0)) { /* the layer's holes lie inside its plate box */
+            h0 += g->holes_in_layer[m];
+            continue;
+        }
+        double holes = 0.0;
+        for (int i = 0; i < g->holes_in_layer[m]; ++i) {
+            const float* hp = g->coll + 100 + 9 * (h0 + i);
+            double l = smg_o_seg_cyl(a, b, hp[0], (double)hp[1] + g->F, (double)hp[2] + g->F, hp[3], hp[4]);
+            holes += l;
+            *att_t += (double)hp[5] * l;
+            *s_pe += (double)hp[6] * l;
+            *s_c += (double)hp[7] * l;
+        }
+        h0 += g->holes_in_layer[m];
+        double wl = c - holes > 0.0 ? c - holes : 0.0;
+        *att_t += (double)L[5] * wl;
+        *s_pe += (double)L[6] * wl;
+        *s_c += (double)L[7] * wl;
+    }
+}
+
+/* Scatter points on a grid over the layer-0 plate: n_k = max(1, round(extent
+ * / pitch)) cells per axis, points at the cell centres, z outermost and x
+ * innermost, without the points inside a layer-0 hole.  Writes up to max
+ * points (x, y, z) when xyz is given and the cell volume to *dv; returns the
+ * number of points. */
+int64_t smg_o_coll_grid_points(const smg_o_geom* g, double pitch, double* xyz, int64_t max,
+                               double* dv) {
+    if (g->n_layers < 1 || !(pitch > 0.0)) return 0;
+    const float* L = g->coll + 10;
+    double lo[3] = {-(double)L[1] / 2.0, -(double)L[2] / 2.0 + g->F + L[4], -(double)L[3] / 2.0};
+    double ext[3] = {L[1], L[2], L[3]};
+    int n[3];
+    double pk[3];
+    for (int k = 0; k < 3; ++k) {
+        n[k] = (int)floor(ext[k] / pitch + 0.5);
+        if (n[k] < 1) n[k] = 1;
+        pk[k] = ext[k] / n[k];
+    }
+    if (dv) *dv = pk[0] * pk[1] * pk[2];
+    int64_t cnt = 0;
+    for (int kz = 0; kz < n[2]; ++kz)
+        for (int jy = 0; jy < n[1]; ++jy)
+            for (int ix = 0; ix < n[0]; ++ix) {
+                double x = lo[0] + (ix + 0.5) * pk[0], y = lo[1] + (jy + 0.5) * pk[1],
+                       z = lo[2] + (kz + 0.5) * pk[2];
+                int inside = 0;
+                for (int h = 0; h < g->holes_in_layer[0] && !inside; ++h) {
+                    const float* hp = g->coll + 100 + 9 * h;
+                    double dx = x - hp[0], dz = z - hp[3];
+                    inside = dx * dx + dz * dz < (double)hp[4] * hp[4] &&
+                             y >= (double)hp[1] + g->F && y <= (double)hp[2] + g->F;
+                }
+                if (inside) continue;
+                if (xyz && cnt < max) {
+                    xyz[3 * cnt] = x;
+                    xyz[3 * cnt + 1] = y;
+                    xyz[3 * cnt + 2] = z;
+                }
+                ++cnt;
+            }
+    return cnt;
+}
+
+/* First-leg weight of grid scatter point s for a source at p: the chance of
+ * a Compton interaction in the cell of volume dv around s, dv mu_c0 /
+ * (4 pi |s - p|^2) exp(-collimator material along p -> s). */
+double smg_o_coll_grid_weight(const smg_o_geom* g, const double p[3], const double s[3], double dv) {
+    double att_t, s_pe, s_c;
+    coll_material(g, p, s, &att_t, &s_pe, &s_c);
+    double L = dist3(p, s);
+    return dv * (double)g->coll[17] / (4.0 * M_PI * L * L) * exp(-att_t);
+}
+
 typedef struct {
     double e0, lo, hi, norm;
     int compton;
@@ -944,8 +1082,10 @@ static double scatter_body(const smg_o_geom* g, const bar_t* bd, int d, const do
                     if (kind == 0) {
                         if (k == sidx) continue;
                         bit = smg_o_bit_crystal(bitmap, N, sidx, d, k);
-                    } else {
+                    } else if (kind == 1) {
                         bit = smg_o_bit_collimator(bitmap, N, sidx, d, k);
+                    } else {
+                        bit = 1; /* grid scatterers: every other bar */
                     }
                     if (!bit) continue;
                     bar_t bk = get_bar(g, k);
@@ -956,6 +1096,11 @@ static double scatter_body(const smg_o_geom* g, const bar_t* bd, int d, const do
                     double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
                     double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
                     att += smg_o_box_ray(sa, qa, lo, hi) * mut;
+                }
+                if (kind == 2) { /* collimator material on the second leg, at E' */
+                    double att_t, s_pe, s_c;
+                    coll_material(g, qsw, q, &att_t, &s_pe, &s_c);
+                    att += s_pe * smg_o_mu_ratio(g, 2, ep, ph->e0) + s_c * smg_o_mu_ratio(g, 3, ep, ph->e0);
                 }
                 double l1 = smg_o_box_ray_inside(qss, qs, blo, bhi);
                 double l2 = smg_o_box_ray_inside(qss, qs, clo, chi);
@@ -1018,6 +1163,41 @@ double smg_o_collimator_scatter_element(const smg_o_geom* g, const uint32_t* bit
         acc += scatter_body(g, &bd, d, p, qsw, wgt, bitmap, 1, h, sx, sy, sz, &ph);
     }
     return acc;
+}
+
+/* Element of the collimator scatter matrix with scatter points on a grid
+ * over the layer-0 tungsten (an extension with no reference counterpart):
+ * first-leg weight smg_o_coll_grid_weight, the second leg through every
+ * other bar and through the collimator material at E', the rest as the
+ * reference's scatter element (SC:1226-1428). */
+double smg_o_coll_grid_scatter_element(const smg_o_geom* g, const double* pts, int64_t n_pts,
+                                       double dv, int d, int v, int rot, int sx, int sy, int sz) {
+    bar_t bd = get_bar(g, d);
+    phys_t ph = phys_for(g, &bd);
+    if (ph.compton != 1 || !(bd.mut > (double)0.01f)) return 0.0;
+    double p[3];
+    voxel_pos(g, v, rot, p);
+    double acc = 0.0;
+    for (int64_t i = 0; i < n_pts; ++i) {
+        const double* s = pts + 3 * i;
+        double w = smg_o_coll_grid_weight(g, p, s, dv);
+        acc += scatter_body(g, &bd, d, p, s, w, NULL, 2, -1, sx, sy, sz, &ph);
+    }
+    return acc;
+}
+
+void smg_o_coll_grid_scatter_pairs(const smg_o_geom* g, double pitch, int rot, int sx, int sy,
+                                   int sz, const int* d, const int* v, int n, double* out,
+                                   int nthreads) {
+    double dv = 0.0;
+    int64_t np = smg_o_coll_grid_points(g, pitch, NULL, 0, &dv);
+    double* pts = (double*)malloc(sizeof(double) * 3 * (size_t)(np > 0 ? np : 1));
+    smg_o_coll_grid_points(g, pitch, pts, np, &dv);
+    if (nthreads <= 0) nthreads = smg_o_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+    for (int i = 0; i < n; ++i)
+        out[i] = smg_o_coll_grid_scatter_element(g, pts, np, dv, d[i], v[i], rot, sx, sy, sz);
+    free(pts);
 }
 
 void smg_o_crystal_scatter_pairs(const smg_o_geom* g, const float* pe, const uint32_t* bitmap,

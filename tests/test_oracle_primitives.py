@@ -273,3 +273,55 @@ def test_mu_tables_loglog_interpolation(oracle_mod, params_mod):
         o.set_mu_tables(En[::-1].copy(), v, v, v, v)
     o.set_mu_tables(None)
     assert o.mu_ratio(0, 100.0, 140.5) == pytest.approx((1.405) ** 3, rel=1e-15)
+
+
+def _sampled_length(a, b, inside, n=200001):
+    t = (np.arange(n) + 0.5) / n
+    pts = np.asarray(a)[None, :] + t[:, None] * (np.asarray(b) - np.asarray(a))[None, :]
+    return inside(pts).mean() * np.linalg.norm(np.asarray(b) - np.asarray(a))
+
+
+def test_seg_box_and_seg_cyl_against_sampling(oracle_mod):
+    """The grid-scatterer extension's plain intersection lengths against
+    dense sampling of random segments (start and end inside or outside)."""
+    rng = np.random.default_rng(17)
+    lo, hi = np.array([-2.0, 1.0, -1.5]), np.array([3.0, 4.0, 2.5])
+    for _ in range(40):
+        a = rng.uniform(-5, 6, 3)
+        b = rng.uniform(-5, 6, 3) if rng.random() < 0.7 else rng.uniform(lo, hi)
+        ref = _sampled_length(a, b, lambda x: np.all((x >= lo) & (x <= hi), axis=1))
+        assert oracle_mod.seg_box(a, b, lo, hi) == pytest.approx(ref, abs=2e-4 * np.linalg.norm(b - a))
+    xc, zc, r, y1, y2 = 0.4, -0.3, 0.8, -1.0, 2.0
+    inside_cyl = lambda x: ((x[:, 0] - xc) ** 2 + (x[:, 2] - zc) ** 2 <= r * r) & (x[:, 1] >= y1) & (x[:, 1] <= y2)
+    for _ in range(40):
+        a = rng.uniform(-2, 2, 3)
+        b = a + rng.normal(0, 1, 3) * np.array([0.6, 3.0, 0.6])
+        ref = _sampled_length(a, b, inside_cyl)
+        assert oracle_mod.seg_cyl(a, b, xc, y1, y2, zc, r) == pytest.approx(ref, abs=2e-4 * np.linalg.norm(b - a))
+    # along the axis: the overlap with [y1, y2]
+    assert oracle_mod.seg_cyl([xc, -5.0, zc], [xc, 1.0, zc], xc, y1, y2, zc, r) == pytest.approx(2.0)
+
+
+def test_coll_grid_points_and_weight(oracle_mod, params_mod):
+    """Grid over the layer-0 plate: the cells tile the plate less the holes,
+    and the first-leg weight is dv mu_c / (4 pi L^2) exp(-mu_t l_W) with l_W
+    the tungsten path (here a point behind a hole-free stretch of plate)."""
+    p = params_mod.config(1)
+    o = oracle_mod.Oracle(p)
+    pts, dv = o.coll_grid_points(1.0)
+    w, t, h = p.coll[11], p.coll[12], p.coll[13]
+    holes = sum(np.pi * p.coll[100 + 9 * i + 4] ** 2 * (p.coll[100 + 9 * i + 2] - p.coll[100 + 9 * i + 1])
+                for i in range(int(p.coll[10])))
+    assert len(pts) * dv == pytest.approx(w * t * h - holes, rel=2e-3)
+    F = p.img[11]
+    y_front = -t / 2 + F + p.coll[14]
+    # a source straight in front of a point s, the path avoiding every hole
+    hx = np.array([p.coll[100 + 9 * i] for i in range(int(p.coll[10]))])
+    hz = np.array([p.coll[100 + 9 * i + 3] for i in range(int(p.coll[10]))])
+    cand = [q for q in pts if np.min((hx - q[0]) ** 2 + (hz - q[2]) ** 2) > 4.0]
+    s = cand[len(cand) // 2]
+    src = np.array([s[0], y_front - 20.0, s[2]])
+    depth = s[1] - y_front
+    L = np.linalg.norm(s - src)
+    ref = dv * p.coll[17] / (4 * np.pi * L * L) * np.exp(-p.coll[15] * depth)
+    assert o.coll_grid_weight(src, s, dv) == pytest.approx(ref, rel=1e-12)
