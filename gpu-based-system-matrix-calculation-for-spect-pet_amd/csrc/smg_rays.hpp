@@ -300,6 +300,7 @@ SMG_FN T seg_box_inside(const SegDir<T>& s, const T a[3], const T lo[3], const T
 // |len / D_k| > kGrazeInv, are flagged for the FP64 path.
 constexpr float kGrazeInv = 32.0f;
 
+template <bool kSkipSmall = false>
 SMG_FN void slab_local(const SegDir<float>& s, const float o[3], const float lo[3],
                        const float hi[3], float& tmin, float& tmax, float& emin, float& emax,
                        float& imin, float& imax) {
@@ -308,6 +309,9 @@ SMG_FN void slab_local(const SegDir<float>& s, const float o[3], const float lo[
     emin = emax = imin = imax = 0.0f;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
+        if constexpr (kSkipSmall) {
+            if (s.small[k]) continue;
+        }
         float inv = s.inv[k];
         float t_lo = inv < 0.0f ? (hi[k] - o[k]) * inv : (lo[k] - o[k]) * inv;
         float t_hi = inv < 0.0f ? (lo[k] - o[k]) * inv : (hi[k] - o[k]) * inv;
@@ -327,15 +331,28 @@ SMG_FN void slab_local(const SegDir<float>& s, const float o[3], const float lo[
 }
 
 // length_box_ray semantics (PESysMatGen.cu:236-312) with a local origin.
+// kOneSmall: a segment with one direction component below eps takes the
+// reference's branch for it (PESysMatGen.cu:69-73): the start point a (same
+// frame as lo, hi) must lie strictly between that axis' faces, and the slab
+// runs over the other two axes in units of their 2-D length.  Otherwise, and
+// with two or three small components, the segment is flagged.
+template <bool kOneSmall = false>
 SMG_FN float seg_box_local(const SegDir<float>& s, const float o[3], float lim_lo, float lim_hi,
-                           const float lo[3], const float hi[3], bool& near) {
+                           const float lo[3], const float hi[3], bool& near,
+                           const float* a = nullptr) {
     const float eps = kSlabEps<float>;
     if (s.ns != 0) {
-        near = true;
-        return 0.0f;
+        if (!kOneSmall || s.ns > 1) {
+            near = true;
+            return 0.0f;
+        }
+        const int k = s.small[0] ? 0 : (s.small[1] ? 1 : 2);
+        const float ak = at3(a, k), lok = at3(lo, k), hik = at3(hi, k);
+        const float tk = 4.0f * kUnit<float> * (tabs(ak) + tabs(lok) + tabs(hik)) + 1e-7f;
+        if (g_ge(ak, hik, tk, near) || g_le(ak, lok, tk, near)) return 0.0f;
     }
     float tmin, tmax, emin, emax, imin, imax;
-    slab_local(s, o, lo, hi, tmin, tmax, emin, emax, imin, imax);
+    slab_local<kOneSmall>(s, o, lo, hi, tmin, tmax, emin, emax, imin, imax);
     float chord = tmax - tmin;
     float tol_min = slab_tol(emin), tol_max = slab_tol(emax);
     if (g_lt(chord, eps, tol_min + tol_max, near)) return 0.0f;

@@ -380,7 +380,9 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
             float of[3] = {float(sc.qss[0]), float(sc.qss[1]), float(sc.qss[2])};
             float lo[3] = {float(sc.lo[0]), float(sc.lo[1]), float(sc.lo[2])};
             float hi[3] = {float(sc.hi[0]), float(sc.hi[1]), float(sc.hi[2])};
-            lthru = seg_box_local(se, of, float(double(kSlabEps<float>) - dis), float(le - dis), lo, hi, near);
+            const float af[3] = {float(ps[0]), float(ps[1]), float(ps[2])};
+            lthru = seg_box_local<true>(se, of, float(double(kSlabEps<float>) - dis), float(le - dis),
+                                        lo, hi, near, af);
         } else {
             SegDir<double> se;
             segdir_init<double>(se, De[0], De[1], De[2], 0.0, near);
