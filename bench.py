@@ -11,8 +11,9 @@ the GPU.  Prints one JSON line.
 Unless --no-extras is given, the line also carries (under "extras") the
 all-FP64 time of the same step, the primary-Compton scatter throughputs with
 the reference's scatter points (crystal scatter on 32x32x1 voxels x 2048 bars,
-collimator scatter on 32x32x16 voxels x 64 bars), and the forward and
-adjoint (dL/dmu) times on the 32x32x1-voxel, 64-bar geometry.
+collimator scatter on 32x32x16 voxels x 64 bars) and with scatter points on a
+0.5 mm grid over the collimator tungsten, and the forward and adjoint
+(dL/dmu) times on the 32x32x1-voxel, 64-bar geometry.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision fp32|fp64]
 """
@@ -141,6 +142,20 @@ def extras(smg, params, pe_ctx, out, precision):
             log(f"collimator scatter {prec}: {ms:.2f} ms")
         ex["collimator_scatter_workload"] = ("32x32x16 voxels x 64 bars, scatter points at the 16 "
                                              "hole centres (reference semantics), 1x1x1 cells")
+
+    # collimator scatter from a 0.5 mm grid over the tungsten (first- and second-leg
+    # collimator attenuation), 32x32x16 voxels, 64 bars
+    with smg.open_context(p3) as c:
+        for prec in ("fp32", "fp64"):
+            _, st, n = c.scatter_collimator_grid(0.5, 0, prec)
+            ex[f"collimator_grid_scatter_{prec}"] = {
+                "elements_per_s": round(st["elements"] / st["kernel_ms"] * 1e3, 1),
+                "pairs_per_s": round(st["rays"] / st["kernel_ms"] * 1e3, 1),
+                "kernel_ms": round(st["kernel_ms"], 1), "fp64_fallbacks": st["fp64_fallbacks"]}
+            log(f"collimator grid scatter {prec}: {st['kernel_ms']:.0f} ms, {n} points")
+        ex["collimator_grid_scatter_workload"] = (f"32x32x16 voxels x 64 bars, {n} scatter points on "
+                                                  "a 0.5 mm grid over the 60x5x60 mm tungsten, "
+                                                  "1x1x1 cells, power-law mu(E)")
 
     # adjoint: forward + dL/dmu on the 32x32x1-voxel, 64-bar geometry, 4x4x4 sub-cells
     p5 = params.config(5)
