@@ -79,8 +79,9 @@ struct smg_ctx {
     DevBuf<PairTerm> pair;
     MuTables mu_tab{};                 // host copy; n = 0: power laws
     DevBuf<MuTables> mu_tab_d;
-    DevBuf<ScatterPt<double>> pts_grid;  // collimator grid scatterers
-    DevBuf<float> w1;                    // their first-leg weights, one chunk
+    DevBuf<ScatterPt<double>> pts_grid;  // grid scatterers (one chunk)
+    DevBuf<float> w1;                    // their first-leg weights
+    DevBuf<int> rows;                    // detector rows of a row-subset call
 };
 
 namespace {
@@ -329,6 +330,10 @@ smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
     }
     A.w1 = nullptr;
     A.accumulate = 0;
+    A.rows = nullptr;
+    A.n_rows = 0;
+    A.pts_per_bar = 1;
+    A.pt_bar0 = 0;
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
     SMG_HIP(c, launch_pair_terms(A, c->stream));
@@ -425,6 +430,7 @@ smg_status smg_destroy(smg_ctx* c) {
     c->mu_tab_d.release();
     c->pts_grid.release();
     c->w1.release();
+    c->rows.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -726,6 +732,130 @@ smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm
         stats->rays = n_el * (uint64_t)pts.size() * (uint64_t)ncell;
         stats->fp64_fallbacks = cnt[0];
         stats->overflow_waves = 0;
+    }
+    return SMG_OK;
+}
+
+smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, int gz,
+                                    const int* rows, int n_rows, smg_precision prec, float* dst,
+                                    int dst_is_device, smg_stats* stats) {
+    double t_start = now_ms();
+    smg_status s = check_ready(c, rotation);
+    if (s != SMG_OK) return s;
+    if (!dst) return fail(c, SMG_ERR_BAD_PARAM, "dst is NULL");
+    const HostGeom& hg = c->hg;
+    const GeomDesc& g = hg.g;
+    if (gx < 1 || gy < 1 || gz < 1 || (int64_t)gx * gy * gz > 65536)
+        return fail(c, SMG_ERR_BAD_PARAM, "grid cells %d x %d x %d", gx, gy, gz);
+    const int npb = gx * gy * gz;
+    std::vector<int> row_list;
+    if (rows) {
+        if (n_rows < 1) return fail(c, SMG_ERR_BAD_PARAM, "n_rows must be positive");
+        for (int i = 0; i < n_rows; ++i) {
+            if (rows[i] < 0 || rows[i] >= g.n_det)
+                return fail(c, SMG_ERR_BAD_PARAM, "row %d: bar %d out of range", i, rows[i]);
+            row_list.push_back(rows[i]);
+        }
+    } else {
+        n_rows = g.n_det;
+    }
+    const size_t n_el = (size_t)n_rows * (size_t)g.n_vox;
+    if ((s = ensure_bitmap(c, false)) != SMG_OK) return s;
+    if (rows && (s = upload(c, c->rows, row_list)) != SMG_OK) return s;
+    const int ncell = c->sc_sub[0] * c->sc_sub[1] * c->sc_sub[2];
+    // chunks of bars: first-leg weights <= 2^28 floats, pair terms <= 2^25 records
+    int64_t nbc = std::max<int64_t>(1, ((int64_t)1 << 28) / ((int64_t)npb * g.n_vox));
+    nbc = std::min<int64_t>(nbc, std::max<int64_t>(1, ((int64_t)1 << 25) / ((int64_t)n_rows * npb * ncell)));
+    nbc = std::min<int64_t>(nbc, g.n_det);
+    SMG_HIP(c, c->w1.reserve((size_t)nbc * npb * (size_t)g.n_vox));
+    SMG_HIP(c, c->pair.reserve((size_t)n_rows * (size_t)nbc * npb * ncell));
+    SMG_HIP(c, c->pts_grid.reserve((size_t)nbc * npb));
+    float* out = dst;
+    if (!dst_is_device) {
+        SMG_HIP(c, c->out.reserve(n_el));
+        out = c->out.ptr;
+    }
+    const double phi = (double)rotation * g.dphi;
+    PeLaunch P{};
+    P.geom = device_geom(c);
+    P.dst = nullptr;
+    P.counters = c->counters.ptr;
+    P.cphi = std::cos(phi);
+    P.sphi = std::sin(phi);
+    P.sub[0] = gx;
+    P.sub[1] = gy;
+    P.sub[2] = gz;
+    pick_tile(g, P.tile, P.tiles);
+    P.pts = c->w1.ptr;
+    ScatterLaunch A{};
+    A.geom = P.geom;
+    A.kind = 3;
+    A.bitmap = c->bm_crystal.ptr;
+    A.pair = c->pair.ptr;
+    A.w1 = c->w1.ptr;
+    A.pts_d = c->pts_grid.ptr;
+    A.dst = out;
+    A.counters = c->counters.ptr;
+    A.cphi = P.cphi;
+    A.sphi = P.sphi;
+    for (int k = 0; k < 3; ++k) A.sub[k] = c->sc_sub[k];
+    const float* ph = hg.phys.data();
+    A.compton_on = (int)std::floor((double)ph[0] + 0.5) == 1;
+    A.same_window = (int)std::floor((double)ph[4] + 0.5) > 0;
+    A.win_lo = ph[5];
+    A.win_hi = ph[6];
+    A.e0 = ph[7];
+    A.kn_norm = kn_norm(A.e0);
+    A.tables = nullptr;
+    if (c->mu_tab.n > 0) {
+        const double le0 = std::log(A.e0);
+        for (int t = 0; t < 4; ++t) c->mu_tab.lmu0[t] = mu_table_log(c->mu_tab, t, le0);
+        SMG_HIP(c, c->mu_tab_d.reserve(1));
+        SMG_HIP(c, hipMemcpyAsync(c->mu_tab_d.ptr, &c->mu_tab, sizeof(MuTables), hipMemcpyHostToDevice,
+                                  c->stream));
+        A.tables = c->mu_tab_d.ptr;
+    }
+    A.rows = rows ? c->rows.ptr : nullptr;
+    A.n_rows = n_rows;
+    A.pts_per_bar = npb;
+    SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    std::vector<ScatterPt<double>> pts((size_t)nbc * npb);
+    const int sub[3] = {gx, gy, gz};
+    for (int b0 = 0; b0 < g.n_det; b0 += (int)nbc) {
+        const int nb = (int)std::min<int64_t>(nbc, g.n_det - b0);
+        // grid points: the sub-cell centres, in the PE kernel's sub-cell order
+        SMG_HIP(c, hipStreamSynchronize(c->stream));  // pts is reused
+        for (int b = 0; b < nb; ++b)
+            for (int cell = 0; cell < npb; ++cell) {
+                CellCtx cc;
+                make_cell_ctx_linear(hg.bars_d[b0 + b], cell, sub, cc);
+                pts[(size_t)b * npb + cell] = ScatterPt<double>{cc.q[0], cc.q[1], cc.q[2], 0.0};
+            }
+        SMG_HIP(c, hipMemcpyAsync(c->pts_grid.ptr, pts.data(), sizeof(ScatterPt<double>) * nb * npb,
+                                  hipMemcpyHostToDevice, c->stream));
+        SMG_HIP(c, launch_pe_points(P, prec == SMG_PREC_FP64, b0, nb, c->stream));
+        A.n_pts = nb * npb;
+        A.pt_bar0 = b0;
+        A.accumulate = b0 > 0;
+        SMG_HIP(c, launch_pair_terms(A, c->stream));
+        SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));
+    }
+    SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    if (!dst_is_device)
+        SMG_HIP(c, hipMemcpyAsync(dst, out, n_el * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    SMG_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t_start;
+        stats->elements = n_el;
+        stats->rays = n_el * (uint64_t)(g.n_det - 1) * (uint64_t)npb * (uint64_t)ncell;
+        stats->fp64_fallbacks = cnt[0];
+        stats->overflow_waves = cnt[1];
     }
     return SMG_OK;
 }

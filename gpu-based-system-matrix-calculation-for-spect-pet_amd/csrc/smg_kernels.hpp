@@ -23,9 +23,14 @@ struct PeLaunch {
     int sub[3];                   // sub-cells per bar (x, y, z)
     int tile[3];                  // voxel tile of one wave, product 64
     int tiles[3];                 // number of tiles per axis
+    int bar0;                     // first bar of the launch (point weights)
+    float* pts;                   // point weights [bar - bar0][cell][V]
 };
 
 hipError_t launch_pe_forward(const PeLaunch& a, bool exact, hipStream_t stream);
+// Crystal grid scatterers: first-leg weights of the sub-cell centres of bars
+// bar0 .. bar0 + n_bars - 1 (sub-cells = grid cells) into a.pts.
+hipError_t launch_pe_points(const PeLaunch& a, bool exact, int bar0, int n_bars, hipStream_t stream);
 
 // Adjoint scratch: one row of `stride` doubles per (bar d, tile group):
 // [n_det mu_total partials][n_layers layer partials][mu_pe of d][loss].
@@ -56,7 +61,8 @@ struct ScatterLaunch {
     const ScatterPt<double>* pts_d;  // scatter points
     int n_pts;
     int kind;                        // 0: crystal scatter, 1: collimator scatter,
-                                     // 2: collimator scatter from grid points
+                                     // 2: collimator scatter from grid points,
+                                     // 3: crystal scatter from grid points
     int variant;                     // collimator: 0 combined (r = 0.1), 1 standalone (r = 0.2)
     const float* pe;                 // crystal: PE block [N][V]
     const uint32_t* bitmap;          // crystal or collimator bitmap
@@ -72,6 +78,9 @@ struct ScatterLaunch {
     const MuTables* tables;          // mu(E) tables, or nullptr for the power laws
     const float* w1;                 // grid: first-leg weights [n_pts][V] of this chunk
     int accumulate;                  // add to dst instead of writing it
+    const int* rows;                 // detector rows to compute (dst row r = bar rows[r]),
+    int n_rows;                      // or nullptr for all bars
+    int pts_per_bar, pt_bar0;        // kind 3: point s lies in bar pt_bar0 + s / pts_per_bar
 };
 
 // Grid scatterers: first-leg weights w1[s][v] = dv mu_c0 / (4 pi L^2) exp(-att)

@@ -242,28 +242,52 @@ __device__ void narrow_candidates(const PeLaunch& A, const double qd[3], const W
 }
 
 
-template <bool kExact>
+// Point weights of the crystal grid scatterers (kPoints; an extension with no
+// reference counterpart): the sub-cells of bar d are the grid cells, and for
+// each the kernel writes the first-leg weight of its centre,
+//   dv mu_c,d / (4 pi |q - p|^2) exp(-att),
+// att being the PE ray's attenuation up to the point (the sub-cell's own
+// chord in bar d added back), to pts[((d - bar0) n_cells + cell) V + v].
+__device__ __forceinline__ float point_weight(const Bar<double>& bd, int n_cells, const double aux[8]) {
+    const double dv = bd.w * bd.t * bd.h / n_cells;
+    return float(aux[5] * exp(-aux[4] * bd.mut) * dv * bd.muc /
+                 (4.0 * 3.14159265358979323846 * aux[6]));
+}
+
+template <bool kExact, bool kPoints>
 __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
     __shared__ WaveLists L;
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;
     const int lane = threadIdx.x;
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
-    const int d = (int)(blockIdx.x / n_tiles);
+    const int d = A.bar0 + (int)(blockIdx.x / n_tiles);
     const int tile = (int)(blockIdx.x % n_tiles);
     float* out = A.dst + (int64_t)d * g.n_vox;
+    const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
+    float* pts = kPoints ? A.pts + (int64_t)(d - A.bar0) * n_cells * g.n_vox : nullptr;
     const Bar<double>& bd = G.bars_d[d];
     WaveSetup W;
     wave_setup(A, d, tile, lane, W);
     if (!(bd.mut > (double)0.01f)) {  // PESysMatGen.cu:1116, :1194
-        if (W.valid) out[W.v] = 0.0f;
+        if constexpr (kPoints) {
+            if (W.valid)
+                for (int c = 0; c < n_cells; ++c) pts[(int64_t)c * g.n_vox + W.v] = 0.0f;
+        } else {
+            if (W.valid) out[W.v] = 0.0f;
+        }
         return;
     }
     int nb, nh;
     if (!build_candidates<kExact>(A, d, lane, W, L, nb, nh)) {
         // too many candidates for the LDS lists: report, write NaN
         if (lane == 0) atomicAdd(&A.counters[1], 1ull);
-        if (W.valid) out[W.v] = __builtin_nanf("");
+        if constexpr (kPoints) {
+            if (W.valid)
+                for (int c = 0; c < n_cells; ++c) pts[(int64_t)c * g.n_vox + W.v] = __builtin_nanf("");
+        } else {
+            if (W.valid) out[W.v] = __builtin_nanf("");
+        }
         return;
     }
     __syncthreads();
@@ -279,7 +303,6 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
     // loop keeps the loop's register use low.
     __shared__ uint16_t redo_cell[kRedoCap];
     __shared__ uint64_t redo_mask[kRedoCap];
-    const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
     int cell = 0;
     while (cell < n_cells) {
         int n_redo = 0;
@@ -293,8 +316,14 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
             bool near = false;
             if (W.valid) {
                 LdsCands C{&L, nsb, nsh, lane};
-                T val = pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near);
-                if (!near) acc.add(val);
+                if constexpr (kPoints) {
+                    double aux[8];
+                    pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near, nullptr, aux);
+                    if (!near) pts[(int64_t)cell * g.n_vox + W.v] = point_weight(bd, n_cells, aux);
+                } else {
+                    T val = pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near);
+                    if (!near) acc.add(val);
+                }
             }
             if constexpr (!kExact) {
                 const uint64_t m = __ballot(near);
@@ -318,13 +347,21 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
                 if ((redo_mask[i] >> lane) & 1ull) {
                     LdsCands C{&L, rsb, rsh, lane};
                     bool n2 = false;
-                    acc.add((T)pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2));
+                    if constexpr (kPoints) {
+                        double aux[8];
+                        pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2, nullptr, aux);
+                        pts[(int64_t)redo_cell[i] * g.n_vox + W.v] = point_weight(bd, n_cells, aux);
+                    } else {
+                        acc.add((T)pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2));
+                    }
                     ++fallbacks;
                 }
             }
         }
     }
-    if (W.valid) out[W.v] = float(acc.value());
+    if constexpr (!kPoints) {
+        if (W.valid) out[W.v] = float(acc.value());
+    }
     if constexpr (!kExact) {
         unsigned tot = fallbacks;
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
@@ -372,7 +409,7 @@ __device__ bool adjoint_cell(const PeGeom& G, int d, int lane, const WaveSetup& 
     const GeomDesc& g = G.g;
     const Bar<double>& bd = G.bars_d[d];
     const double mut = bd.mut, mupe = bd.mupe;
-    double term = 0.0, dlay[kMaxLayers], aux[4] = {0.0, 0.0, 0.0, 0.0};
+    double term = 0.0, dlay[kMaxLayers], aux[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int m = 0; m < kMaxLayers; ++m) dlay[m] = 0.0;
     bool near = false;
@@ -562,10 +599,25 @@ hipError_t launch_pe_forward(const PeLaunch& A, bool exact, hipStream_t stream) 
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
     const int64_t blocks = (int64_t)n_tiles * A.geom.g.n_det;
     if (blocks <= 0) return hipSuccess;
+    PeLaunch B = A;
+    B.bar0 = 0;
     if (exact)
-        hipLaunchKernelGGL(pe_forward_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A);
+        hipLaunchKernelGGL((pe_forward_kernel<true, false>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
     else
-        hipLaunchKernelGGL(pe_forward_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A);
+        hipLaunchKernelGGL((pe_forward_kernel<false, false>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_pe_points(const PeLaunch& A, bool exact, int bar0, int n_bars, hipStream_t stream) {
+    const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    const int64_t blocks = (int64_t)n_tiles * n_bars;
+    if (blocks <= 0) return hipSuccess;
+    PeLaunch B = A;
+    B.bar0 = bar0;
+    if (exact)
+        hipLaunchKernelGGL((pe_forward_kernel<true, true>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
+    else
+        hipLaunchKernelGGL((pe_forward_kernel<false, true>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
     return hipGetLastError();
 }
 

@@ -535,6 +535,9 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
         aux[1] = double(l1) - double(l2);   // own-bar path
         aux[2] = double(la);                // absorption chord
         aux[3] = double(ea);
+        aux[4] = double(l2);                // the sub-cell's own chord
+        aux[5] = double(eff);               // e^-att
+        aux[6] = dsq;                       // |q - p|^2
     }
     return eff * om * absf;
 }

@@ -33,13 +33,17 @@ __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
     const PeGeom& G = A.geom;
     const int N = G.g.n_det;
     const int ncell = A.sub[0] * A.sub[1] * A.sub[2];
-    const int64_t total = (int64_t)N * A.n_pts * ncell;
+    const int n_rows = A.rows ? A.n_rows : N;
+    const int64_t total = (int64_t)n_rows * A.n_pts * ncell;
     if (tid >= total) return;
     const int cell = (int)(tid % ncell);
     const int s = (int)((tid / ncell) % A.n_pts);
-    const int d = (int)(tid / ((int64_t)ncell * A.n_pts));
+    const int r = (int)(tid / ((int64_t)ncell * A.n_pts));
+    const int d = A.rows ? A.rows[r] : r;
+    // the bar holding the scatter point (crystal scatter)
+    const int sbar = A.kind == 3 ? A.pt_bar0 + s / A.pts_per_bar : s;
     PairTerm out{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (A.kind == 0 && s == d) {
+    if ((A.kind == 0 || A.kind == 3) && sbar == d) {
         A.pair[tid] = out;
         return;
     }
@@ -53,14 +57,14 @@ __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
     // bars flagged between the scatter point and bar d (scatter.cu:1343-1401);
     // grid scatterers: every other bar
     int64_t base = 0;
-    if (A.kind == 0) {
-        int64_t i = s < d ? s : d, j = s < d ? d : s;
+    if (A.kind == 0 || A.kind == 3) {
+        int64_t i = sbar < d ? sbar : d, j = sbar < d ? d : sbar;
         base = (i * N - i * (i + 1) / 2 + j) * N;
     } else if (A.kind == 1) {
         base = ((int64_t)s * N + d) * N;
     }
     for (int k = 0; k < N; ++k) {
-        if (k == d || (A.kind == 0 && k == s)) continue;
+        if (k == d || ((A.kind == 0 || A.kind == 3) && k == sbar)) continue;
         if (A.kind != 2 && !bit_set(A.bitmap, base + k)) continue;
         const Bar<double>& bk = G.bars_d[k];
         double pa[3], pb[3];
@@ -77,6 +81,17 @@ __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
     if (A.kind == 2) {  // collimator material from the grid point to the sub-cell
         double att_t;
         coll_material(G, qs, cc.q, att_t, out.s_wpe, out.s_wc);
+    }
+    if (A.kind == 3) {  // grid point inside bar sbar: its way out of that bar
+        const Bar<double>& bs = G.bars_d[sbar];
+        double sa[3], qa[3];
+        to_frame(bs, qs, sa);
+        to_frame(bs, cc.q, qa);
+        const double lo[3] = {-0.5 * bs.w, -0.5 * bs.t, -0.5 * bs.h};
+        const double hi[3] = {0.5 * bs.w, 0.5 * bs.t, 0.5 * bs.h};
+        const double l = seg_box_geo(sa, qa, lo, hi);
+        out.s_pe += l * bs.mupe;
+        out.s_c += l * bs.muc;
     }
     // own bar from its surface to the sub-cell (scatter.cu:1405-1417)
     double qss[3];
@@ -107,10 +122,11 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A) {
     const GeomDesc& g = G.g;
     const int lane = threadIdx.x;
     const int64_t waves_per_det = (g.n_vox + 63) / 64;
-    const int d = (int)(blockIdx.x / waves_per_det);
+    const int r = (int)(blockIdx.x / waves_per_det);  // output row
+    const int d = A.rows ? A.rows[r] : r;
     const int64_t v = (blockIdx.x % waves_per_det) * 64 + lane;
     const bool valid = v < g.n_vox;
-    float* out = A.dst + (int64_t)d * g.n_vox;
+    float* out = A.dst + (int64_t)r * g.n_vox;
     const Bar<double>& bd = G.bars_d[d];
     if (!A.compton_on || !(bd.mut > (double)0.01f)) {  // scatter.cu:1226-1228
         if (valid) out[v] = 0.0f;
@@ -162,18 +178,19 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A) {
         }
         for (int s = 0; s < A.n_pts; ++s) {
             if (A.kind == 0 && s == d) continue;
+            if (A.kind == 3 && A.pt_bar0 + s / A.pts_per_bar == d) continue;
             const ScatterPt<double>& sp = A.pts_d[s];
             sc.qs[0] = sp.x;
             sc.qs[1] = sp.y;
             sc.qs[2] = sp.z;
             to_frame(bd, sc.qs, sc.qss);
-            sc.pt = A.pair[((int64_t)d * A.n_pts + s) * ncell + cell];
+            sc.pt = A.pair[((int64_t)r * A.n_pts + s) * ncell + cell];
             if (!valid) continue;
             // weight of the scatter point for this voxel
             T wgt;
             if (A.kind == 0) {
                 wgt = T(A.pe[(int64_t)s * g.n_vox + v]) * wscale;
-            } else if (A.kind == 2) {
+            } else if (A.kind == 2 || A.kind == 3) {
                 wgt = T(A.w1[(int64_t)s * g.n_vox + v]);
             } else {
                 double dx = p[0] - sp.x, dy = p[1] - sp.y, dz = p[2] - sp.z;
@@ -247,7 +264,8 @@ hipError_t launch_first_leg(const ScatterLaunch& A, double dv, double muc0, floa
 }
 
 hipError_t launch_pair_terms(const ScatterLaunch& A, hipStream_t stream) {
-    const int64_t total = (int64_t)A.geom.g.n_det * A.n_pts * A.sub[0] * A.sub[1] * A.sub[2];
+    const int64_t rows = A.rows ? A.n_rows : A.geom.g.n_det;
+    const int64_t total = rows * A.n_pts * A.sub[0] * A.sub[1] * A.sub[2];
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(pair_terms_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                        stream, A);
@@ -255,7 +273,8 @@ hipError_t launch_pair_terms(const ScatterLaunch& A, hipStream_t stream) {
 }
 
 hipError_t launch_scatter(const ScatterLaunch& A, bool exact, hipStream_t stream) {
-    const int64_t blocks = (int64_t)A.geom.g.n_det * ((A.geom.g.n_vox + 63) / 64);
+    const int64_t rows = A.rows ? A.n_rows : A.geom.g.n_det;
+    const int64_t blocks = rows * ((A.geom.g.n_vox + 63) / 64);
     if (blocks <= 0) return hipSuccess;
     if (exact)
         hipLaunchKernelGGL(scatter_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A);

@@ -36,7 +36,7 @@ EXPORTS = (
     "smg_bitmap_crystal_words", "smg_bitmap_collimator_words", "smg_bitmap_crystal",
     "smg_bitmap_collimator", "smg_load_bitmap_crystal", "smg_load_bitmap_collimator",
     "smg_scatter_crystal", "smg_scatter_collimator", "smg_set_mu_tables",
-    "smg_scatter_collimator_grid",
+    "smg_scatter_collimator_grid", "smg_scatter_crystal_grid",
 )
 
 
@@ -105,6 +105,7 @@ def lib():
         "smg_scatter_collimator": (i32, [vp, i32, i32, i32, vp, i32, st]),
         "smg_scatter_collimator_grid": (i32, [vp, i32, C.c_double, i32, vp, i32,
                                               C.POINTER(C.c_int64), st]),
+        "smg_scatter_crystal_grid": (i32, [vp, i32, i32, i32, i32, vp, i32, i32, vp, i32, st]),
     }
     missing = [name for name in sig if not hasattr(L, name)]
     if missing:
@@ -298,6 +299,23 @@ class Context:
                                                         C.byref(npts), C.byref(st)))
         return out, st.as_dict(), npts.value
 
+    def scatter_crystal_grid(self, grid, rows=None, rotation: int = 0, precision: str = "fp32"):
+        """Crystal scatter from the centres of a grid (gx, gy, gz) of cells
+        in every bar, for detector rows `rows` (all bars when None).
+        Returns (matrix [len(rows)][V], stats)."""
+        if rows is None:
+            n = self.dims.n_det
+            rp = None
+        else:
+            r = np.ascontiguousarray(rows, np.int32)
+            n = len(r)
+            rp = r.ctypes.data_as(C.c_void_p)
+        out = np.empty((n, self.dims.n_vox), np.float32)
+        st = Stats()
+        self._check(self._L.smg_scatter_crystal_grid(
+            self._h, rotation, *grid, rp, n if rows is not None else 0, PREC[precision],
+            out.ctypes.data_as(C.c_void_p), 0, C.byref(st)))
+        return out, st.as_dict()
 
 def open_context(p, device: int = 0) -> Context:
     """Context with the geometry of ``p`` (a :class:`params.Params`)."""

@@ -158,6 +158,19 @@ smg_status smg_scatter_collimator_grid(smg_ctx* ctx, int rotation, double pitch_
                                        smg_precision prec, float* dst, int dst_is_device,
                                        int64_t* n_points, smg_stats* stats);
 
+/* Crystal scatter with scatter points at the centres of a gx x gy x gz grid
+ * of cells in every bar, for the detector rows `rows` (n_rows of them; dst
+ * row r is bar rows[r]) or for all bars when rows is NULL.  No counterpart in
+ * the reference, whose crystal scatterer is each bar's centre weighted by the
+ * PE matrix (ScatterGen_Crystal/scatter.cu:1061-1421).  Each point weighs
+ * dv mu_c / (4 pi L^2) exp(-att), att being the photoelectric ray's
+ * attenuation to the point; the second leg passes the bars flagged in the
+ * crystal bitmap for the bar pair and leaves the scatter bar; the rest is the
+ * reference's scatter term. */
+smg_status smg_scatter_crystal_grid(smg_ctx* ctx, int rotation, int gx, int gy, int gz,
+                                    const int* rows, int n_rows, smg_precision prec, float* dst,
+                                    int dst_is_device, smg_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,6 +96,10 @@ def lib():
     L.smg_o_coll_grid_weight.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
     L.smg_o_coll_grid_scatter_pairs.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
                                                 C.c_int, _ip, _ip, C.c_int, _dp, C.c_int]
+    L.smg_o_crystal_grid_weight.restype = C.c_double
+    L.smg_o_crystal_grid_weight.argtypes = [C.c_void_p] + [C.c_int] * 7
+    L.smg_o_crystal_grid_scatter_pairs.argtypes = [C.c_void_p, _up] + [C.c_int] * 7 + [
+        _ip, _ip, C.c_int, _dp, C.c_int]
     L.smg_o_set_mu_tables.restype = C.c_int
     L.smg_o_set_mu_tables.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
     L.smg_o_mu_ratio.restype = C.c_double
@@ -272,6 +276,18 @@ class Oracle:
         v = np.ascontiguousarray(v, np.int32)
         out = np.zeros(len(d))
         lib().smg_o_coll_grid_scatter_pairs(self._g, pitch, rot, *sub, d, v, len(d), out, nthreads)
+        return out
+
+    def crystal_grid_weight(self, b, v, grid, cell, rot=0) -> float:
+        return lib().smg_o_crystal_grid_weight(self._g, b, v, rot, *grid, cell)
+
+    def crystal_grid_scatter_pairs(self, bitmap, grid, d, v, rot=0, sub=(1, 1, 1), nthreads=0):
+        d = np.ascontiguousarray(d, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        out = np.zeros(len(d))
+        bm = np.ascontiguousarray(bitmap, np.uint32)
+        lib().smg_o_crystal_grid_scatter_pairs(self._g, bm, rot, *grid, *sub, d, v, len(d), out,
+                                                nthreads)
         return out
 
     def crystal_scatter_pairs(self, pe, bitmap, d, v, rot=0, sub=(1, 1, 1), nthreads=0):

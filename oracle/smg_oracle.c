@@ -513,6 +513,70 @@ void smg_o_pe_cull_counts(const smg_o_geom* g, int d, int v, int rot, int* n_cry
     cull_free(g, &cl);
 }
 
+/* Attenuation of the ray p -> q up to the sub-cell [clo, chi] of bar d that
+ * holds q (PE:1240-1346): plate gates, holes and plate material, the culled
+ * bars, and the own-bar path l1 - l2.  lay_len and bar_len receive the layer
+ * and bar path lengths (for the gradient), *l1o and *l2o the own-bar chords
+ * from the bar's and the sub-cell's surface to q. */
+static double pe_ray_att(const smg_o_geom* g, const bar_t* bdp, const cull_t* cl, const double p[3],
+                         const double ps[3], const double q[3], const double qs[3],
+                         const double clo[3], const double chi[3], double cosang, double* lay_len,
+                         double* bar_len, double* l1o, double* l2o) {
+    const bar_t bd = *bdp;
+    const int Lc = g->n_layers;
+    double blo[3] = {-bd.w / 2.0, -bd.t / 2.0, -bd.h / 2.0};
+    double bhi[3] = {bd.w / 2.0, bd.t / 2.0, bd.h / 2.0};
+    double att = 0.0;
+    /* plate gate: a ray that does not cross a plate box is blocked (PE:1240-1254) */
+    for (int m = 0; m < Lc; ++m) {
+        const float* L = g->coll + (m + 1) * 10;
+        double lo[3] = {-(double)L[1] / 2.0, -(double)L[2] / 2.0 + g->F + L[4], -(double)L[3] / 2.0};
+        double hi[3] = {(double)L[1] / 2.0, (double)L[2] / 2.0 + g->F + L[4], (double)L[3] / 2.0};
+        OPS(10); /* plate box */
+        if (smg_o_box_ray(p, q, lo, hi) < (double)0.1f) att += 100000.0;
+    }
+    /* holes and plate material (PE:1257-1286) */
+    for (int m = 0; m < Lc; ++m) {
+        const float* L = g->coll + (m + 1) * 10;
+        double in_coll = (double)L[2] / cosang;
+        double sum = 0.0;
+        OPS(5); /* plate path and its attenuation */
+        for (int i = 0; i < cl->n_holes[m]; ++i) {
+            const float* hp = g->coll + 100 + 9 * cl->holes[m][i];
+            OPS(7);
+            double l = smg_o_cylinder_ray(p, q, hp[0], (double)hp[1] + g->F,
+                                          (double)hp[2] + g->F, hp[3], hp[4]);
+            sum += l;
+            att += (double)hp[5] * l;
+        }
+        lay_len[m] = 0.0;
+        if (in_coll - sum >= (double)0.00001f) {
+            att += (double)L[5] * (in_coll - sum);
+            lay_len[m] = in_coll - sum;
+        }
+    }
+    /* other bars on the way (PE:1290-1325) */
+    for (int i = 0; i < cl->n_cryst; ++i) {
+        bar_t bk = get_bar(g, cl->cryst[i]);
+        double pa[3], qa[3];
+        to_bar_frame(&bk, p, pa);
+        to_bar_frame(&bk, q, qa);
+        double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
+        double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
+        OPS(31); /* two frame transforms, box, attenuation */
+        double l = smg_o_box_ray(pa, qa, lo, hi);
+        bar_len[i] = l;
+        att += l * bk.mut;
+    }
+    /* own bar up to the sub-cell (PE:1329-1346) */
+    OPS(21 + 2 + 20 + 3); /* cell box, exp, extension, absorption, sum */
+    double l1 = smg_o_box_ray_inside(ps, qs, blo, bhi);
+    double l2 = smg_o_box_ray_inside(ps, qs, clo, chi);
+    *l1o = l1;
+    *l2o = l2;
+    return att + (l1 - l2) * bd.mut;
+}
+
 /* One element A[d][v] of the PE matrix (PE:1193-1362), summed over the
  * (sx, sy, sz) sub-cells of bar d.  When grad != NULL the derivatives of
  * wdv * A[d][v] with respect to the layer mu, the bars' mu_total and the
@@ -525,8 +589,6 @@ static double pe_element_core(const smg_o_geom* g, int d, int v, int rot, int sx
     voxel_pos(g, v, rot, p);
     double ps[3];
     to_bar_frame(&bd, p, ps);
-    double blo[3] = {-bd.w / 2.0, -bd.t / 2.0, -bd.h / 2.0};
-    double bhi[3] = {bd.w / 2.0, bd.t / 2.0, bd.h / 2.0};
     cull_t cl;
     pe_cull(g, d, p, &cl);
     const int Lc = g->n_layers;
@@ -552,53 +614,9 @@ static double pe_element_core(const smg_o_geom* g, int d, int v, int rot, int sx
                 double om_y = (bd.w * bd.h / sx / sz) / (4 * M_PI * dsq) * fabs(D[1]) / dd;
                 double om = om_x >= om_y ? om_x : om_y;
                 om = om >= om_z ? om : om_z;
-                double att = 0.0;
-                /* plate gate: a ray that does not cross a plate box is blocked (PE:1240-1254) */
-                for (int m = 0; m < Lc; ++m) {
-                    const float* L = g->coll + (m + 1) * 10;
-                    double lo[3] = {-(double)L[1] / 2.0, -(double)L[2] / 2.0 + g->F + L[4], -(double)L[3] / 2.0};
-                    double hi[3] = {(double)L[1] / 2.0, (double)L[2] / 2.0 + g->F + L[4], (double)L[3] / 2.0};
-                    OPS(10); /* plate box */
-                    if (smg_o_box_ray(p, q, lo, hi) < (double)0.1f) att += 100000.0;
-                }
-                /* holes and plate material (PE:1257-1286) */
-                for (int m = 0; m < Lc; ++m) {
-                    const float* L = g->coll + (m + 1) * 10;
-                    double in_coll = (double)L[2] / cosang;
-                    double sum = 0.0;
-                    OPS(5); /* plate path and its attenuation */
-                    for (int i = 0; i < cl.n_holes[m]; ++i) {
-                        const float* hp = g->coll + 100 + 9 * cl.holes[m][i];
-                        OPS(7);
-                        double l = smg_o_cylinder_ray(p, q, hp[0], (double)hp[1] + g->F,
-                                                      (double)hp[2] + g->F, hp[3], hp[4]);
-                        sum += l;
-                        att += (double)hp[5] * l;
-                    }
-                    lay_len[m] = 0.0;
-                    if (in_coll - sum >= (double)0.00001f) {
-                        att += (double)L[5] * (in_coll - sum);
-                        lay_len[m] = in_coll - sum;
-                    }
-                }
-                /* other bars on the way (PE:1290-1325) */
-                for (int i = 0; i < cl.n_cryst; ++i) {
-                    bar_t bk = get_bar(g, cl.cryst[i]);
-                    double pa[3], qa[3];
-                    to_bar_frame(&bk, p, pa);
-                    to_bar_frame(&bk, q, qa);
-                    double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
-                    double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
-                    OPS(31); /* two frame transforms, box, attenuation */
-                    double l = smg_o_box_ray(pa, qa, lo, hi);
-                    bar_len[i] = l;
-                    att += l * bk.mut;
-                }
-                /* own bar up to the sub-cell (PE:1329-1346) */
-                OPS(21 + 2 + 20 + 3); /* cell box, exp, extension, absorption, sum */
-                double l1 = smg_o_box_ray_inside(ps, qs, blo, bhi);
-                double l2 = smg_o_box_ray_inside(ps, qs, clo, chi);
-                att += (l1 - l2) * bd.mut;
+                double l1, l2;
+                double att = pe_ray_att(g, &bd, &cl, p, ps, q, qs, clo, chi, cosang, lay_len,
+                                        bar_len, &l1, &l2);
                 double eff = exp(-att);
                 /* absorption along the ray extended 1000 mm through the cell (PE:1350-1357) */
                 double ext[3];
@@ -1079,7 +1097,7 @@ static double scatter_body(const smg_o_geom* g, const bar_t* bd, int d, const do
                 for (int k = 0; k < N; ++k) {
                     if (k == d) continue;
                     int bit;
-                    if (kind == 0) {
+                    if (kind == 0 || kind == 3) {
                         if (k == sidx) continue;
                         bit = smg_o_bit_crystal(bitmap, N, sidx, d, k);
                     } else if (kind == 1) {
@@ -1096,6 +1114,15 @@ static double scatter_body(const smg_o_geom* g, const bar_t* bd, int d, const do
                     double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
                     double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
                     att += smg_o_box_ray(sa, qa, lo, hi) * mut;
+                }
+                if (kind == 3) { /* grid point inside bar sidx: its way out of that bar */
+                    bar_t bs = get_bar(g, sidx);
+                    double sa[3], qa[3];
+                    to_bar_frame(&bs, qsw, sa);
+                    to_bar_frame(&bs, q, qa);
+                    double lo[3] = {-0.5 * bs.w, -0.5 * bs.t, -0.5 * bs.h};
+                    double hi[3] = {0.5 * bs.w, 0.5 * bs.t, 0.5 * bs.h};
+                    att += smg_o_seg_box(sa, qa, lo, hi) * (bs.mupe * e3 + bs.muc * e1);
                 }
                 if (kind == 2) { /* collimator material on the second leg, at E' */
                     double att_t, s_pe, s_c;
@@ -1198,6 +1225,80 @@ void smg_o_coll_grid_scatter_pairs(const smg_o_geom* g, double pitch, int rot, i
     for (int i = 0; i < n; ++i)
         out[i] = smg_o_coll_grid_scatter_element(g, pts, np, dv, d[i], v[i], rot, sx, sy, sz);
     free(pts);
+}
+
+/* First-leg weight of the centre s of grid cell `cell` of bar b (cells of
+ * a gx x gy x gz split in the PE sub-cell order) for voxel v: the chance of
+ * a Compton interaction in that cell, dv mu_c,b / (4 pi |s - p|^2) exp(-att),
+ * att being the PE ray's attenuation to s (PE:1240-1346 with the sub-cell's
+ * own chord added back).  An extension with no reference counterpart. */
+double smg_o_crystal_grid_weight(const smg_o_geom* g, int b, int v, int rot, int gx, int gy,
+                                 int gz, int cell) {
+    bar_t bb = get_bar(g, b);
+    if (!(bb.mut > (double)0.01f)) return 0.0;
+    double p[3], ps[3];
+    voxel_pos(g, v, rot, p);
+    to_bar_frame(&bb, p, ps);
+    cull_t cl;
+    pe_cull(g, b, p, &cl);
+    const int by = cell % gy, a = (cell / gy) % gx, c = cell / (gy * gx);
+    double qs[3], clo[3], chi[3], q[3];
+    cell_of(&bb, a, by, c, gx, gy, gz, qs, clo, chi);
+    from_bar_frame(&bb, qs, q);
+    double D[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+    double dsq = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+    double cosang = D[1] / sqrt(dsq);
+    double* lay_len = (double*)malloc(sizeof(double) * (g->n_layers > 0 ? g->n_layers : 1));
+    double* bar_len = (double*)malloc(sizeof(double) * (cl.n_cryst > 0 ? cl.n_cryst : 1));
+    double l1, l2;
+    double att = pe_ray_att(g, &bb, &cl, p, ps, q, qs, clo, chi, cosang, lay_len, bar_len, &l1, &l2);
+    free(lay_len);
+    free(bar_len);
+    cull_free(g, &cl);
+    double dv = bb.w * bb.t * bb.h / ((double)gx * gy * gz);
+    return exp(-(att + l2 * bb.mut)) * dv * bb.muc / (4.0 * M_PI * dsq);
+}
+
+/* Element of the crystal scatter matrix with scatter points at the grid
+ * cell centres of every other bar (an extension with no reference
+ * counterpart): first-leg weight smg_o_crystal_grid_weight, the second leg
+ * through the bars flagged in the crystal bitmap for the bar pair and out
+ * of the scatter bar itself, the rest as the reference's crystal scatter
+ * element (SC:1077-1437). */
+double smg_o_crystal_grid_scatter_element(const smg_o_geom* g, const uint32_t* bitmap, int d,
+                                          int v, int rot, int gx, int gy, int gz, int sx, int sy,
+                                          int sz) {
+    bar_t bd = get_bar(g, d);
+    phys_t ph = phys_for(g, &bd);
+    if (ph.compton != 1 || !(bd.mut > (double)0.01f)) return 0.0;
+    double p[3];
+    voxel_pos(g, v, rot, p);
+    double acc = 0.0;
+    const int nc = gx * gy * gz;
+    for (int b = 0; b < g->n_det; ++b) {
+        if (b == d) continue;
+        bar_t bb = get_bar(g, b);
+        for (int cell = 0; cell < nc; ++cell) {
+            const int by = cell % gy, a = (cell / gy) % gx, c = cell / (gy * gx);
+            double qs[3], clo[3], chi[3], s[3];
+            cell_of(&bb, a, by, c, gx, gy, gz, qs, clo, chi);
+            from_bar_frame(&bb, qs, s);
+            double w = smg_o_crystal_grid_weight(g, b, v, rot, gx, gy, gz, cell);
+            acc += scatter_body(g, &bd, d, p, s, w, bitmap, 3, b, sx, sy, sz, &ph);
+        }
+    }
+    return acc;
+}
+
+void smg_o_crystal_grid_scatter_pairs(const smg_o_geom* g, const uint32_t* bitmap, int rot,
+                                      int gx, int gy, int gz, int sx, int sy, int sz,
+                                      const int* d, const int* v, int n, double* out,
+                                      int nthreads) {
+    if (nthreads <= 0) nthreads = smg_o_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+    for (int i = 0; i < n; ++i)
+        out[i] = smg_o_crystal_grid_scatter_element(g, bitmap, d[i], v[i], rot, gx, gy, gz, sx,
+                                                    sy, sz);
 }
 
 void smg_o_crystal_scatter_pairs(const smg_o_geom* g, const float* pe, const uint32_t* bitmap,

@@ -120,6 +120,14 @@ void smg_o_coll_grid_scatter_pairs(const smg_o_geom* g, double pitch, int rot, i
                                    int sz, const int* d, const int* v, int n, double* out,
                                    int nthreads);
 
+/* ---- crystal scatter from grid points (extension, no counterpart) -------- */
+double smg_o_crystal_grid_weight(const smg_o_geom* g, int b, int v, int rot, int gx, int gy,
+                                 int gz, int cell);
+void smg_o_crystal_grid_scatter_pairs(const smg_o_geom* g, const uint32_t* bitmap, int rot,
+                                      int gx, int gy, int gz, int sx, int sy, int sz,
+                                      const int* d, const int* v, int n, double* out,
+                                      int nthreads);
+
 int smg_o_max_threads(void);
 
 #ifdef __cplusplus

@@ -227,3 +227,45 @@ def test_collimator_grid_chunks_and_determinism(smg_mod, params_mod):
     assert n > 100000
     assert np.array_equal(a, b)
     assert np.isfinite(a).all() and (a >= 0).all() and (a > 0).mean() > 0.5
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
+def test_crystal_grid_scatter(smg_mod, oracle_mod, params_mod, prec, tol):
+    """Grid scatterers at the centres of 2x4x2 cells of every bar (config 1):
+    first-leg weights from the PE kernel's point mode, second leg through the
+    crystal bitmap and out of the scatter bar, for a subset of detector rows,
+    with and without mu(E) tables, against the oracle."""
+    p = params_mod.config(1)
+    o = oracle_mod.Oracle(p)
+    grid = (2, 4, 2)
+    rows = np.array([0, 9, 27, 40, 63], np.int32)
+    rng = np.random.default_rng(21)
+    ri = rng.integers(0, len(rows), 300)
+    v = rng.integers(0, o.n_vox, 300)
+    with smg_mod.open_context(p) as ctx:
+        got, st = ctx.scatter_crystal_grid(grid, rows, 0, prec)
+        bm = ctx.bitmap_crystal()
+        ctx.set_mu_tables(*mu_tables(False))
+        got_t, _ = ctx.scatter_crystal_grid(grid, rows, 0, prec)
+    assert got.shape == (len(rows), o.n_vox)
+    ref = o.crystal_grid_scatter_pairs(bm, grid, rows[ri], v)
+    assert (ref > 0).mean() > 0.5
+    assert_parity(got[ri, v], ref, tol, f"crystal grid scatter {prec}")
+    o.set_mu_tables(*mu_tables(False))
+    ref_t = o.crystal_grid_scatter_pairs(bm, grid, rows[ri], v)
+    assert_parity(got_t[ri, v], ref_t, tol, f"crystal grid scatter, mu(E) tables, {prec}")
+    assert not np.allclose(ref, ref_t, rtol=1e-3, atol=0.0)
+
+
+def test_crystal_grid_rows_match_full_matrix(smg_mod, params_mod):
+    """A row subset gives the same rows as the full matrix, and a repeat run
+    gives the same bits; the matrix is finite and non-negative."""
+    p = params_mod.config(1)
+    rows = np.array([63, 5, 5, 30], np.int32)
+    with smg_mod.open_context(p) as ctx:
+        full, _ = ctx.scatter_crystal_grid((2, 2, 2), None, 0, "fp32")
+        again, _ = ctx.scatter_crystal_grid((2, 2, 2), None, 0, "fp32")
+        part, _ = ctx.scatter_crystal_grid((2, 2, 2), rows, 0, "fp32")
+    assert np.array_equal(full, again)
+    assert np.isfinite(full).all() and (full >= 0).all() and (full > 0).mean() > 0.5
+    np.testing.assert_allclose(part, full[rows], rtol=2e-6, atol=0.0)

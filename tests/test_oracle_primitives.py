@@ -325,3 +325,32 @@ def test_coll_grid_points_and_weight(oracle_mod, params_mod):
     L = np.linalg.norm(s - src)
     ref = dv * p.coll[17] / (4 * np.pi * L * L) * np.exp(-p.coll[15] * depth)
     assert o.coll_grid_weight(src, s, dv) == pytest.approx(ref, rel=1e-12)
+
+
+def test_crystal_grid_weight_closed_form(oracle_mod, params_mod):
+    """First-leg weight of a crystal grid point with no collimator and no
+    other bar on the way: dv mu_c / (4 pi L^2) exp(-mu_t l), l the path in the
+    bar from its front face to the point (the voxel straight in front)."""
+    p = params_mod.config(1)
+    p.coll[0] = 0  # no collimator layer
+    b = 9
+    xb, zb = float(p.det[12 * b + 1]), float(p.det[12 * b + 3])
+    yb = float(p.det[12 * b + 2]) + float(p.img[11])
+    w, t, h = (float(x) for x in p.det[12 * b + 4:12 * b + 7])
+    mut, muc = float(p.det[12 * b + 7]), float(p.det[12 * b + 9])
+    grid = (2, 4, 2)
+    # grid column (a, c) = (0, 1) of bar b: x = xb - w/4, z = zb + h/4; put the
+    # voxel row z there and pick the voxel column at that x
+    p.img[10] = zb + h / 4
+    o = oracle_mod.Oracle(p)
+    ix = int(round(xb - w / 4 + 15.5))
+    assert abs(ix - 15.5 - (xb - w / 4)) < 1e-9
+    v = ix + 16 * 32  # iy = 16, y = 0.5
+    pv = np.array([ix - 15.5, 0.5, zb + h / 4])
+    dv = w * t * h / 16
+    for by in range(4):
+        cell = (1 * 2 + 0) * 4 + by
+        s = np.array([xb - w / 4, yb - t / 2 + (by + 0.5) * t / 4, zb + h / 4])
+        L = np.linalg.norm(s - pv)
+        ref = dv * muc / (4 * np.pi * L * L) * np.exp(-mut * (s[1] - (yb - t / 2)))
+        assert o.crystal_grid_weight(b, v, grid, cell) == pytest.approx(ref, rel=1e-12)
