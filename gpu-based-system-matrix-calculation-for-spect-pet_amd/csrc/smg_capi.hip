@@ -77,6 +77,7 @@ struct smg_ctx {
     DevBuf<ScatterPt<double>> pts_bar;   // bar centres (crystal scatter)
     DevBuf<ScatterPt<double>> pts_hole;  // layer-0 hole centres (collimator scatter)
     DevBuf<PairTerm> pair;
+    DevBuf<double> sc_partial;   // per-slice scatter sums
     MuTables mu_tab{};                 // host copy; n = 0: power laws
     DevBuf<MuTables> mu_tab_d;
     DevBuf<ScatterPt<double>> pts_grid;  // grid scatterers (one chunk)
@@ -267,6 +268,18 @@ smg_status load_bitmap(smg_ctx* c, bool collimator, const uint32_t* words, int64
     return SMG_OK;
 }
 
+// Partial-sum buffer of a scatter launch whose scatter points are split over
+// several waves per element.  It only grows; hipFree waits for the device.
+hipError_t scatter_partials(smg_ctx* c, ScatterLaunch& A) {
+    const size_t np = scatter_partial_size(A);
+    if (np) {
+        hipError_t e = c->sc_partial.reserve(np);
+        if (e != hipSuccess) return e;
+    }
+    A.partial = c->sc_partial.ptr;
+    return hipSuccess;
+}
+
 smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
                          smg_precision prec, const float* pe, int pe_is_device, float* dst,
                          int dst_is_device, smg_stats* stats) {
@@ -337,6 +350,7 @@ smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
     SMG_HIP(c, launch_pair_terms(A, c->stream));
+    SMG_HIP(c, scatter_partials(c, A));
     SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));
     SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long cnt[4] = {0, 0, 0, 0};
@@ -427,6 +441,7 @@ smg_status smg_destroy(smg_ctx* c) {
     c->pts_bar.release();
     c->pts_hole.release();
     c->pair.release();
+    c->sc_partial.release();
     c->mu_tab_d.release();
     c->pts_grid.release();
     c->w1.release();
@@ -715,6 +730,7 @@ smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm
         A.accumulate = s0 > 0;
         SMG_HIP(c, launch_first_leg(A, dv, muc0, c->w1.ptr, c->stream));
         SMG_HIP(c, launch_pair_terms(A, c->stream));
+        SMG_HIP(c, scatter_partials(c, A));
         SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));
     }
     SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -839,6 +855,7 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
         A.pt_bar0 = b0;
         A.accumulate = b0 > 0;
         SMG_HIP(c, launch_pair_terms(A, c->stream));
+        SMG_HIP(c, scatter_partials(c, A));
         SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));
     }
     SMG_HIP(c, hipEventRecord(c->ev1, c->stream));

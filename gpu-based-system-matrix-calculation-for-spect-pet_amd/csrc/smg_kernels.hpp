@@ -81,6 +81,8 @@ struct ScatterLaunch {
     const int* rows;                 // detector rows to compute (dst row r = bar rows[r]),
     int n_rows;                      // or nullptr for all bars
     int pts_per_bar, pt_bar0;        // kind 3: point s lies in bar pt_bar0 + s / pts_per_bar
+    double* partial;                 // [slices][rows][V] per-slice sums when the scatter points
+                                     // are split over several waves per element
 };
 
 // Grid scatterers: first-leg weights w1[s][v] = dv mu_c0 / (4 pi L^2) exp(-att)
@@ -90,5 +92,8 @@ hipError_t launch_first_leg(const ScatterLaunch& a, double dv, double muc0, floa
 
 hipError_t launch_pair_terms(const ScatterLaunch& a, hipStream_t stream);
 hipError_t launch_scatter(const ScatterLaunch& a, bool exact, hipStream_t stream);
+// Doubles launch_scatter needs in `partial` for this launch (0: one wave per
+// element covers every scatter point, no partial sums).
+size_t scatter_partial_size(const ScatterLaunch& a);
 
 }  // namespace smg

@@ -9,13 +9,18 @@
 //                      voxel-independent second-leg sums S_pe, S_c over the
 //                      bars flagged in the geometry bitmap, the own-bar path
 //                      and the absorption chord, in FP64.
-//   scatter_kernel     one wave per (bar d, 64 consecutive voxels); each lane
-//                      owns one element and loops over scatter points and
-//                      sub-cells, summing with a two-sum compensated sum.
+//   scatter_kernel     one wave per (bar d, 64 consecutive voxels[, slice of
+//                      the scatter points]); each lane owns one element and
+//                      loops over scatter points and sub-cells, summing with
+//                      a two-sum compensated sum.  When the points are split
+//                      into slices, scatter_reduce_kernel adds the slices'
+//                      sums in slice order.
 //                      The reference instead runs one thread per
 //                      (d, v, s) and sums with float atomicAdd, in an order
 //                      that changes from run to run (scatter.cu:1427).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "smg_kernels.hpp"
 #include "smg_scatter_body.hpp"
@@ -115,21 +120,56 @@ __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
     A.pair[tid] = out;
 }
 
+// Waves per element row: the scatter points split into `slices` ranges when
+// rows x voxel waves alone would leave the chip under-filled (a few detector
+// rows, or a small image), each range summed by its own wave.
+struct SliceShape {
+    int64_t rows, waves_per_det;
+    int slices, per;  // per: scatter points per slice
+};
+
+SliceShape slice_shape(const ScatterLaunch& A) {
+    SliceShape s;
+    s.rows = A.rows ? A.n_rows : A.geom.g.n_det;
+    s.waves_per_det = (A.geom.g.n_vox + 63) / 64;
+    const int64_t base = s.rows * s.waves_per_det;
+    constexpr int64_t kTargetWaves = 16384;  // 256 CUs x 4 SIMDs x 16
+    constexpr int64_t kMinPoints = 32;       // scatter points per wave, at least
+    int64_t n = base > 0 ? (kTargetWaves + base - 1) / base : 1;
+    n = std::min<int64_t>(n, std::max<int64_t>(1, (int64_t)A.n_pts / kMinPoints));
+    n = std::max<int64_t>(n, 1);
+    s.per = (int)((A.n_pts + n - 1) / n);
+    s.slices = s.per > 0 ? (int)((A.n_pts + s.per - 1) / s.per) : 1;
+    if (s.slices < 1) s.slices = 1;
+    return s;
+}
+
 template <bool kExact>
-__global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A) {
+__global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A, SliceShape sh) {
     using T = typename std::conditional<kExact, double, float>::type;
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;
     const int lane = threadIdx.x;
-    const int64_t waves_per_det = (g.n_vox + 63) / 64;
-    const int r = (int)(blockIdx.x / waves_per_det);  // output row
+    const int64_t waves_per_det = sh.waves_per_det;
+    const int64_t wrow = blockIdx.x % (sh.rows * waves_per_det);
+    const int slice = (int)(blockIdx.x / (sh.rows * waves_per_det));
+    const int r = (int)(wrow / waves_per_det);  // output row
     const int d = A.rows ? A.rows[r] : r;
-    const int64_t v = (blockIdx.x % waves_per_det) * 64 + lane;
+    const int64_t v = (wrow % waves_per_det) * 64 + lane;
     const bool valid = v < g.n_vox;
+    const bool sliced = sh.slices > 1;
     float* out = A.dst + (int64_t)r * g.n_vox;
+    double* part = A.partial + ((int64_t)slice * sh.rows + r) * g.n_vox;
+    const int s_begin = slice * sh.per;
+    const int s_end = min(A.n_pts, s_begin + sh.per);
     const Bar<double>& bd = G.bars_d[d];
     if (!A.compton_on || !(bd.mut > (double)0.01f)) {  // scatter.cu:1226-1228
-        if (valid) out[v] = 0.0f;
+        if (valid) {
+            if (sliced)
+                part[v] = 0.0;
+            else
+                out[v] = A.accumulate ? out[v] : 0.0f;
+        }
         return;
     }
     const int64_t vv = valid ? v : 0;
@@ -176,7 +216,7 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A) {
             sc.corner[corner][1] = (corner & 2) ? cc.hi[1] : cc.lo[1];
             sc.corner[corner][2] = (corner & 4) ? cc.hi[2] : cc.lo[2];
         }
-        for (int s = 0; s < A.n_pts; ++s) {
+        for (int s = s_begin; s < s_end; ++s) {
             if (A.kind == 0 && s == d) continue;
             if (A.kind == 3 && A.pt_bar0 + s / A.pts_per_bar == d) continue;
             const ScatterPt<double>& sp = A.pts_d[s];
@@ -219,12 +259,30 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A) {
             acc.add(val);
         }
     }
-    if (valid) out[v] = A.accumulate ? out[v] + float(acc.value()) : float(acc.value());
+    if (valid) {
+        if (sliced)
+            part[v] = double(acc.s) + double(acc.c);
+        else
+            out[v] = A.accumulate ? out[v] + float(acc.value()) : float(acc.value());
+    }
     if constexpr (!kExact) {
         unsigned tot = fallbacks;
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
         if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
     }
+}
+
+// Sum of the slices' partial sums of one element, in slice order (a fixed
+// order, so the result does not depend on scheduling), compensated in FP64.
+__global__ __launch_bounds__(256) void scatter_reduce_kernel(ScatterLaunch A, SliceShape sh) {
+    const int64_t n_vox = A.geom.g.n_vox;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = sh.rows * n_vox;
+    if (tid >= n) return;
+    KahanSum<double> acc;
+    for (int k = 0; k < sh.slices; ++k) acc.add(A.partial[(int64_t)k * n + tid]);
+    const float val = float(acc.value());
+    A.dst[tid] = A.accumulate ? A.dst[tid] + val : val;
 }
 
 // Grid scatterers (an extension with no reference counterpart): one thread
@@ -272,14 +330,25 @@ hipError_t launch_pair_terms(const ScatterLaunch& A, hipStream_t stream) {
     return hipGetLastError();
 }
 
+size_t scatter_partial_size(const ScatterLaunch& A) {
+    const SliceShape sh = slice_shape(A);
+    return sh.slices > 1 ? (size_t)sh.slices * (size_t)sh.rows * (size_t)A.geom.g.n_vox : 0;
+}
+
 hipError_t launch_scatter(const ScatterLaunch& A, bool exact, hipStream_t stream) {
-    const int64_t rows = A.rows ? A.n_rows : A.geom.g.n_det;
-    const int64_t blocks = rows * ((A.geom.g.n_vox + 63) / 64);
+    const SliceShape sh = slice_shape(A);
+    const int64_t blocks = sh.slices * sh.rows * sh.waves_per_det;
     if (blocks <= 0) return hipSuccess;
+    if (sh.slices > 1 && !A.partial) return hipErrorInvalidValue;
     if (exact)
-        hipLaunchKernelGGL(scatter_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A);
+        hipLaunchKernelGGL(scatter_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A, sh);
     else
-        hipLaunchKernelGGL(scatter_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A);
+        hipLaunchKernelGGL(scatter_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A, sh);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || sh.slices == 1) return e;
+    const int64_t n = sh.rows * A.geom.g.n_vox;
+    hipLaunchKernelGGL(scatter_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                       A, sh);
     return hipGetLastError();
 }
 
