@@ -157,6 +157,22 @@ def extras(smg, params, pe_ctx, out, precision):
                                                   "a 0.5 mm grid over the 60x5x60 mm tungsten, "
                                                   "1x1x1 cells, power-law mu(E)")
 
+    # crystal scatter from a 0.25 mm grid in every bar (10x32x10 cells of the 2.5x8x2.5 mm
+    # bars), 32x32x1 voxels, 2048 bars, 8 detector rows (every 256th bar)
+    rows = np.arange(0, p4.n_det, 256, dtype=np.int32)
+    with smg.open_context(p4) as c:
+        for prec in ("fp32", "fp64"):
+            _, st = c.scatter_crystal_grid((10, 32, 10), rows, 0, prec)
+            ex[f"crystal_grid_scatter_{prec}"] = {
+                "elements_per_s": round(st["elements"] / st["kernel_ms"] * 1e3, 1),
+                "pairs_per_s": round(st["rays"] / st["kernel_ms"] * 1e3, 1),
+                "kernel_ms": round(st["kernel_ms"], 1), "fp64_fallbacks": st["fp64_fallbacks"]}
+            log(f"crystal grid scatter {prec}: {st['kernel_ms']:.0f} ms")
+        ex["crystal_grid_scatter_workload"] = (
+            f"32x32x1 voxels x {len(rows)} detector rows of 2048 bars, scatter points at the "
+            f"centres of 10x32x10 cells (0.25 mm) of every bar ({p4.n_det * 3200} points), "
+            "1x1x1 cells, power-law mu(E)")
+
     # adjoint: forward + dL/dmu on the 32x32x1-voxel, 64-bar geometry, 4x4x4 sub-cells
     p5 = params.config(5)
     w = np.random.default_rng(2).uniform(0, 1, (p5.n_det, p5.n_vox)).astype(np.float32)

@@ -25,6 +25,7 @@ struct PeLaunch {
     int tiles[3];                 // number of tiles per axis
     int bar0;                     // first bar of the launch (point weights)
     float* pts;                   // point weights [bar - bar0][cell][V]
+    int cell_slices;              // point weights: waves per (bar, tile), each a range of cells
 };
 
 hipError_t launch_pe_forward(const PeLaunch& a, bool exact, hipStream_t stream);

@@ -21,6 +21,8 @@
 //   4. each lane writes its element once: no atomics, deterministic results.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "smg_kernels.hpp"
 #include "smg_pe_body.hpp"
 
@@ -261,10 +263,16 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
     const GeomDesc& g = G.g;
     const int lane = threadIdx.x;
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
-    const int d = A.bar0 + (int)(blockIdx.x / n_tiles);
-    const int tile = (int)(blockIdx.x % n_tiles);
+    // point weights: cells [c_begin, c_end) of the wave's slice; PE: all cells
+    const int slices = kPoints ? A.cell_slices : 1;
+    const int wave = (int)(blockIdx.x / slices);
+    const int d = A.bar0 + wave / n_tiles;
+    const int tile = wave % n_tiles;
     float* out = A.dst + (int64_t)d * g.n_vox;
     const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
+    const int per = (n_cells + slices - 1) / slices;
+    const int c_begin = kPoints ? (int)(blockIdx.x % slices) * per : 0;
+    const int c_end = kPoints ? min(n_cells, c_begin + per) : n_cells;
     float* pts = kPoints ? A.pts + (int64_t)(d - A.bar0) * n_cells * g.n_vox : nullptr;
     const Bar<double>& bd = G.bars_d[d];
     WaveSetup W;
@@ -272,7 +280,7 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
     if (!(bd.mut > (double)0.01f)) {  // PESysMatGen.cu:1116, :1194
         if constexpr (kPoints) {
             if (W.valid)
-                for (int c = 0; c < n_cells; ++c) pts[(int64_t)c * g.n_vox + W.v] = 0.0f;
+                for (int c = c_begin; c < c_end; ++c) pts[(int64_t)c * g.n_vox + W.v] = 0.0f;
         } else {
             if (W.valid) out[W.v] = 0.0f;
         }
@@ -284,7 +292,7 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
         if (lane == 0) atomicAdd(&A.counters[1], 1ull);
         if constexpr (kPoints) {
             if (W.valid)
-                for (int c = 0; c < n_cells; ++c) pts[(int64_t)c * g.n_vox + W.v] = __builtin_nanf("");
+                for (int c = c_begin; c < c_end; ++c) pts[(int64_t)c * g.n_vox + W.v] = __builtin_nanf("");
         } else {
             if (W.valid) out[W.v] = __builtin_nanf("");
         }
@@ -303,10 +311,10 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
     // loop keeps the loop's register use low.
     __shared__ uint16_t redo_cell[kRedoCap];
     __shared__ uint64_t redo_mask[kRedoCap];
-    int cell = 0;
-    while (cell < n_cells) {
+    int cell = c_begin;
+    while (cell < c_end) {
         int n_redo = 0;
-        for (; cell < n_cells && n_redo < kRedoCap; ++cell) {
+        for (; cell < c_end && n_redo < kRedoCap; ++cell) {
             CellCtx cc;
             tile_cell(A, bd, cell, W, lane, cc);
             int nsb, nsh;
@@ -601,6 +609,7 @@ hipError_t launch_pe_forward(const PeLaunch& A, bool exact, hipStream_t stream) 
     if (blocks <= 0) return hipSuccess;
     PeLaunch B = A;
     B.bar0 = 0;
+    B.cell_slices = 1;
     if (exact)
         hipLaunchKernelGGL((pe_forward_kernel<true, false>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
     else
@@ -610,10 +619,17 @@ hipError_t launch_pe_forward(const PeLaunch& A, bool exact, hipStream_t stream) 
 
 hipError_t launch_pe_points(const PeLaunch& A, bool exact, int bar0, int n_bars, hipStream_t stream) {
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
-    const int64_t blocks = (int64_t)n_tiles * n_bars;
-    if (blocks <= 0) return hipSuccess;
+    const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
+    const int64_t waves = (int64_t)n_tiles * n_bars;
+    if (waves <= 0) return hipSuccess;
     PeLaunch B = A;
     B.bar0 = bar0;
+    // cells split over waves until the launch holds about 32768 waves, with
+    // at least 64 cells per wave (each wave builds its own candidate lists)
+    int64_t slices = (32768 + waves - 1) / waves;
+    slices = std::min<int64_t>(slices, std::max(1, n_cells / 64));
+    B.cell_slices = (int)std::max<int64_t>(slices, 1);
+    const int64_t blocks = waves * B.cell_slices;
     if (exact)
         hipLaunchKernelGGL((pe_forward_kernel<true, true>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
     else

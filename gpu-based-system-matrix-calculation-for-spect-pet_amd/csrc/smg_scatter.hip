@@ -199,6 +199,16 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A, SliceShape
     }
     const int ncell = A.sub[0] * A.sub[1] * A.sub[2];
     const T wscale = T(bd.muc / bd.mupe);  // crystal weight uses bar d's mu (scatter.cu:1218-1219)
+    // scatter points inside bar d itself do not count: its centre (kind 0),
+    // or its grid points (kind 3)
+    int skip_lo = 0, skip_hi = 0;
+    if (A.kind == 0) {
+        skip_lo = d;
+        skip_hi = d + 1;
+    } else if (A.kind == 3) {
+        skip_lo = (d - A.pt_bar0) * A.pts_per_bar;
+        skip_hi = skip_lo + A.pts_per_bar;
+    }
     KahanSum<T> acc;
     unsigned fallbacks = 0;
     for (int cell = 0; cell < ncell; ++cell) {
@@ -217,8 +227,7 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A, SliceShape
             sc.corner[corner][2] = (corner & 4) ? cc.hi[2] : cc.lo[2];
         }
         for (int s = s_begin; s < s_end; ++s) {
-            if (A.kind == 0 && s == d) continue;
-            if (A.kind == 3 && A.pt_bar0 + s / A.pts_per_bar == d) continue;
+            if (s >= skip_lo && s < skip_hi) continue;
             const ScatterPt<double>& sp = A.pts_d[s];
             sc.qs[0] = sp.x;
             sc.qs[1] = sp.y;
