@@ -221,11 +221,6 @@ __global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A, SliceShape
             sc.lo[k] = cc.lo[k];
             sc.hi[k] = cc.hi[k];
         }
-        for (int corner = 0; corner < 8; ++corner) {
-            sc.corner[corner][0] = (corner & 1) ? cc.hi[0] : cc.lo[0];
-            sc.corner[corner][1] = (corner & 2) ? cc.hi[1] : cc.lo[1];
-            sc.corner[corner][2] = (corner & 4) ? cc.hi[2] : cc.lo[2];
-        }
         for (int s = s_begin; s < s_end; ++s) {
             if (s >= skip_lo && s < skip_hi) continue;
             const ScatterPt<double>& sp = A.pts_d[s];

@@ -176,9 +176,10 @@ SMG_FN T kn_of_cos(T c, T alpha) {
 // Inclusive rectangle rule of KN sin on lo + i*step, i = 0..n with
 // n = (int)((hi - lo)/step) (scatter.cu:980-993).  FP32: flags a step count
 // within rounding distance of an integer, and takes cos and sin of the nodes
-// from an angle-addition recurrence in FP64 started at lo (four FMAs a step
-// instead of two transcendental calls; the recurrence drifts by about one FP64
-// ulp per step).  FP64: cos and sin of every node, as the reference.
+// from an angle-addition recurrence in FP64 started at the FP32 cos and sin of
+// lo (four FMAs a step instead of two transcendental calls; the rotation keeps
+// the start's ~1e-7 error and drifts by about one FP64 ulp per step).  FP64:
+// cos and sin of every node, as the reference.
 template <class T>
 SMG_FN T kn_sum(T lo, T hi, T alpha, bool& near) {
     const T step = T(kKnStep);
@@ -189,7 +190,11 @@ SMG_FN T kn_sum(T lo, T hi, T alpha, bool& near) {
         near |= (fr < T(2e-4) && n > 0) | (fr > T(1) - T(2e-4));
         // cos and sin of the step (double)0.01f = 0.009999999776482582
         const double cst = 0.99995000041890041468, snt = 0.0099998331106604225523;
-        double c = cos((double)lo), sv = sin((double)lo);
+        // start values in FP32 (the sum is FP32): an FP64 cos/sin here costs
+        // more registers than the whole FP32 term
+        float cf, sf;
+        sincosf(float(lo), &sf, &cf);
+        double c = cf, sv = sf;
         KahanSum<T> s;
         for (int i = 0; i <= n; ++i) {
             s.add(kn_of_cos(T(c), alpha) * T(sv) * step);
@@ -236,8 +241,8 @@ struct ScatterCell {
     double qs[3];        // scatter point, world
     double qss[3];       // scatter point, frame of bar d
     double q[3];         // sub-cell centre, world
-    double lo[3], hi[3]; // sub-cell box, frame of bar d
-    double corner[8][3]; // sub-cell corners, frame of bar d
+    double lo[3], hi[3]; // sub-cell box, frame of bar d; corner c has lo[k] or hi[k]
+                         // on axis k as bit k of c is clear or set
     PairTerm pt;
 };
 
@@ -331,7 +336,7 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
     if constexpr (std::is_same<T, double>::value) {
         for (int c = 0; c < 8; ++c) {
             T wc[3];
-            for (int k = 0; k < 3; ++k) wc[k] = T(sc.corner[c][k] - sc.qss[k]);
+            for (int k = 0; k < 3; ++k) wc[k] = ((c >> k) & 1 ? sc.hi[k] : sc.lo[k]) - sc.qss[k];
             T th = cone_angle<T>(us, wc);
             if (c == 0 || th > tmax) tmax = th;
             if (c == 0 || th < tmin) tmin = th;
@@ -341,9 +346,14 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
         // angles a, b in [0, pi] given as (|u x w|, u.w) = r (sin, cos),
         // a < b exactly when sin(b - a) > 0, i.e. s_b c_a - c_b s_a > 0.
         float s_lo = 0.0f, c_lo = 1.0f, s_hi = 0.0f, c_hi = 1.0f;
+        float wlo[3], whi[3];  // corner offsets from the scatter point
+        for (int k = 0; k < 3; ++k) {
+            wlo[k] = float(sc.lo[k] - sc.qss[k]);
+            whi[k] = float(sc.hi[k] - sc.qss[k]);
+        }
         for (int c = 0; c < 8; ++c) {
             float w[3];
-            for (int k = 0; k < 3; ++k) w[k] = float(sc.corner[c][k] - sc.qss[k]);
+            for (int k = 0; k < 3; ++k) w[k] = (c >> k) & 1 ? whi[k] : wlo[k];
             const float cx = us[1] * w[2] - us[2] * w[1], cy = us[2] * w[0] - us[0] * w[2],
                         cz = us[0] * w[1] - us[1] * w[0];
             // (r sin, r cos) with r = |u| |w| > 0: the sign tests and atan2 do

@@ -369,11 +369,6 @@ double scatter_element(const Ctx& c, const float* pe, const uint32_t* bitmap, in
             sc.lo[k] = cc.lo[k];
             sc.hi[k] = cc.hi[k];
         }
-        for (int corner = 0; corner < 8; ++corner) {
-            sc.corner[corner][0] = (corner & 1) ? cc.hi[0] : cc.lo[0];
-            sc.corner[corner][1] = (corner & 2) ? cc.hi[1] : cc.lo[1];
-            sc.corner[corner][2] = (corner & 4) ? cc.hi[2] : cc.lo[2];
-        }
         for (int s = 0; s < n_pts; ++s) {
             if (kind == 0 && s == d) continue;
             double dyh = 0.0;
