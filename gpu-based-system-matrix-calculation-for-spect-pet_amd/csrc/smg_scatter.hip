@@ -144,8 +144,12 @@ SliceShape slice_shape(const ScatterLaunch& A) {
     return s;
 }
 
+// 3 waves per SIMD: the register cap (168) spills a few values that the
+// inline FP64 fallback holds; measured 14% faster (FP32) and 5% (FP64) than the
+// 2 waves the unconstrained allocation gives.
 template <bool kExact>
-__global__ __launch_bounds__(64) void scatter_kernel(ScatterLaunch A, SliceShape sh) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void scatter_kernel(
+    ScatterLaunch A, SliceShape sh) {
     using T = typename std::conditional<kExact, double, float>::type;
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;

@@ -287,26 +287,41 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
             omc = c < 0.5 ? 1.0 - c : s2 / (1.0 + c);
         }
     }
-    const double ep = ph.e0 / (1.0 + alpha * omc);
-    const double resp = ph.res * sqrt(ep / ph.e0);
-    if (ep * (1 + 2 * resp / kFwhm) <= ph.lo || ep * (1 - 2 * resp / kFwhm) >= ph.hi)
-        return T(0);  // 2-sigma energy-window early out (scatter.cu:1263-1269)
-    T pe_win;
-    {
+    // E' = E0 g with g = 1 / (1 + alpha (1 - cos)); the FP32 path forms the
+    // window from g and one reciprocal, the FP64 path as the reference does
+    double ep, resp, z1, z2;
+    if constexpr (std::is_same<T, double>::value) {
+        ep = ph.e0 / (1.0 + alpha * omc);
+        resp = ph.res * sqrt(ep / ph.e0);
+        if (ep * (1 + 2 * resp / kFwhm) <= ph.lo || ep * (1 - 2 * resp / kFwhm) >= ph.hi)
+            return T(0);  // 2-sigma energy-window early out (scatter.cu:1263-1269)
         double sigma = resp / kFwhm * ep;
         double s2 = sigma * sqrt(2.0);
-        double z1 = (ph.lo - ep) / s2, z2 = (ph.hi - ep) / s2;
-        if constexpr (std::is_same<T, double>::value) {
-            pe_win = 0.5 * (erf(z2) - erf(z1));
-        } else {  // erfc keeps the difference accurate when both z lie on one side
-            float f1 = float(z1), f2 = float(z2);
-            if (z1 >= 0.0)
-                pe_win = 0.5f * (erfcf(f1) - erfcf(f2));
-            else if (z2 <= 0.0)
-                pe_win = 0.5f * (erfcf(-f2) - erfcf(-f1));
-            else
-                pe_win = 0.5f * (erff(f2) - erff(f1));
-        }
+        z1 = (ph.lo - ep) / s2;
+        z2 = (ph.hi - ep) / s2;
+    } else {
+        // sigma sqrt(2) = resp E' sqrt(2) / FWHM
+        constexpr double k2 = 2.0 / kFwhm, ks2 = 1.4142135623730950488 / kFwhm;
+        const double g = 1.0 / (1.0 + alpha * omc);
+        ep = ph.e0 * g;
+        resp = ph.res * sqrt(g);
+        if (ep * (1 + resp * k2) <= ph.lo || ep * (1 - resp * k2) >= ph.hi)
+            return T(0);  // 2-sigma energy-window early out (scatter.cu:1263-1269)
+        const double inv_s2 = 1.0 / (resp * ks2 * ep);  // 1 / (sigma sqrt 2)
+        z1 = (ph.lo - ep) * inv_s2;
+        z2 = (ph.hi - ep) * inv_s2;
+    }
+    T pe_win;
+    if constexpr (std::is_same<T, double>::value) {
+        pe_win = 0.5 * (erf(z2) - erf(z1));
+    } else {  // erfc keeps the difference accurate when both z lie on one side
+        float f1 = float(z1), f2 = float(z2);
+        if (z1 >= 0.0)
+            pe_win = 0.5f * (erfcf(f1) - erfcf(f2));
+        else if (z2 <= 0.0)
+            pe_win = 0.5f * (erfcf(-f2) - erfcf(-f1));
+        else
+            pe_win = 0.5f * (erff(f2) - erff(f1));
     }
 
     // azimuthal range from the sub-cell's bounding sphere (scatter.cu:1275-1287);
@@ -317,7 +332,11 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
         if (L * st * 2.0 <= rdet) {
             phi = T(kTwoPi);
         } else {
-            double r = rdet / 2.0 / L / st;
+            double r;
+            if constexpr (std::is_same<T, double>::value)
+                r = rdet / 2.0 / L / st;
+            else
+                r = rdet / (2.0 * L * st);
             if constexpr (std::is_same<T, double>::value) {
                 phi = 4.0 * asin(r <= 1.0 ? r : 1.0);
             } else if (r < 0.75) {
@@ -380,31 +399,44 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
         for (int k = 0; k < 3; ++k) D[k] = sc.qss[k] - ps[k];
         dis2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
         double dis = sqrt(dis2);
-        double De[3];
-        for (int k = 0; k < 3; ++k) De[k] = (sc.qss[k] + 1000.0 * D[k] / dis) - ps[k];
         if constexpr (std::is_same<T, float>::value) {
+            // De = D (1 + 1000 / |D|), so |De| - |D| = 1000 up to rounding,
+            // which the FP32 end parameter does not see
+            const double f = 1.0 + 1000.0 / dis;
+            const float De[3] = {float(D[0] * f), float(D[1] * f), float(D[2] * f)};
             SegDir<float> se;
-            float tde = 4.0f * kUnit<float> * (tabs(float(De[0])) + tabs(float(De[1])) + tabs(float(De[2])));
-            segdir_init<float>(se, float(De[0]), float(De[1]), float(De[2]), tde, near);
-            double le = sqrt(De[0] * De[0] + De[1] * De[1] + De[2] * De[2]);
+            float tde = 4.0f * kUnit<float> * (tabs(De[0]) + tabs(De[1]) + tabs(De[2]));
+            segdir_init<float>(se, De[0], De[1], De[2], tde, near);
             float of[3] = {float(sc.qss[0]), float(sc.qss[1]), float(sc.qss[2])};
             float lo[3] = {float(sc.lo[0]), float(sc.lo[1]), float(sc.lo[2])};
             float hi[3] = {float(sc.hi[0]), float(sc.hi[1]), float(sc.hi[2])};
             const float af[3] = {float(ps[0]), float(ps[1]), float(ps[2])};
-            lthru = seg_box_local<true>(se, of, float(double(kSlabEps<float>) - dis), float(le - dis),
-                                        lo, hi, near, af);
+            lthru = seg_box_local<true>(se, of, float(double(kSlabEps<float>) - dis), 1000.0f, lo, hi,
+                                        near, af);
         } else {
+            double De[3];
+            for (int k = 0; k < 3; ++k) De[k] = (sc.qss[k] + 1000.0 * D[k] / dis) - ps[k];
             SegDir<double> se;
             segdir_init<double>(se, De[0], De[1], De[2], 0.0, near);
             lthru = seg_box<double>(se, ps, sc.lo, sc.hi, near);
         }
     }
     if (g_gt(lthru, T(0.001f), T(1e-5), near)) tmin = T(0);
-    T K = kn_sum<T>(tmin, tmax, alpha_t, near) / T(ph.norm);
+    T K;
+    if constexpr (std::is_same<T, double>::value)
+        K = kn_sum<T>(tmin, tmax, alpha_t, near) / T(ph.norm);
+    else
+        K = kn_sum<T>(tmin, tmax, alpha_t, near) * float(1.0 / ph.norm);
 
-    // second leg and absorption at E' (scatter.cu:1371-1427)
+    // second leg and absorption at E' (scatter.cu:1371-1427); the FP32 path
+    // takes E0/E' = 1 + alpha (1 - cos) for the power laws
     double e3, e1;
-    mu_ratios(ph.tab, ph.e0, ep, e3, e1);
+    if (std::is_same<T, float>::value && !(ph.tab && ph.tab->n > 0)) {
+        e1 = 1.0 + alpha * omc;
+        e3 = e1 * e1 * e1;
+    } else {
+        mu_ratios(ph.tab, ph.e0, ep, e3, e1);
+    }
     double att2 = e3 * sc.pt.s_pe + e1 * sc.pt.s_c;
     if (sc.pt.s_wpe != 0.0 || sc.pt.s_wc != 0.0) {  // collimator material (grid scatterers)
         double w3 = e3, w1 = e1;
@@ -418,8 +450,13 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
     double mupe = ph.mupe * e3;
     double mut = mupe + ph.muc * e1;
     T ea = exp_neg<T>(sc.pt.la * mut);
-    T absf = (T(1) - ea) * T(mupe / mut);
-    return wgt * phi / T(kTwoPi) * K * pe_win * exp_neg<T>(att2) * absf;
+    if constexpr (std::is_same<T, double>::value) {
+        T absf = (T(1) - ea) * T(mupe / mut);
+        return wgt * phi / T(kTwoPi) * K * pe_win * exp_neg<T>(att2) * absf;
+    } else {
+        T absf = (T(1) - ea) * tdiv(float(mupe), float(mut));
+        return wgt * phi * float(1.0 / kTwoPi) * K * pe_win * exp_neg<T>(att2) * absf;
+    }
 }
 
 }  // namespace smg
