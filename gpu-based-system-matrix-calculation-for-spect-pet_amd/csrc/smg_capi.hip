@@ -78,6 +78,10 @@ struct smg_ctx {
     DevBuf<ScatterPt<double>> pts_hole;  // layer-0 hole centres (collimator scatter)
     DevBuf<PairTerm> pair;
     DevBuf<double> sc_partial;   // per-slice scatter sums
+    DevBuf<CellCtx> cell_tab;    // PE sub-cell table of every bar
+    DevBuf<CellCtx> cell_tab_pts;  // sub-cell table of one chunk of crystal grid points
+    int cell_tab_sub[3] = {0, 0, 0};
+    uint64_t cell_tab_gen = 0, geom_gen = 0;
     MuTables mu_tab{};                 // host copy; n = 0: power laws
     DevBuf<MuTables> mu_tab_d;
     DevBuf<ScatterPt<double>> pts_grid;  // grid scatterers (one chunk)
@@ -280,6 +284,29 @@ hipError_t scatter_partials(smg_ctx* c, ScatterLaunch& A) {
     return hipSuccess;
 }
 
+// Sub-cell table of every bar for the PE sub-cells of A (built on the
+// context's stream when the geometry or the sub-cell counts change), or none
+// when it would pass 1 GiB: the kernels then form each sub-cell themselves.
+hipError_t pe_cell_table(smg_ctx* c, PeLaunch& A) {
+    A.cells = nullptr;
+    A.cells_bar0 = 0;
+    const int64_t n = (int64_t)c->hg.g.n_det * A.sub[0] * A.sub[1] * A.sub[2];
+    if (n * (int64_t)sizeof(CellCtx) > ((int64_t)1 << 30)) return hipSuccess;
+    const bool same = c->cell_tab_gen == c->geom_gen && c->cell_tab_sub[0] == A.sub[0] &&
+                      c->cell_tab_sub[1] == A.sub[1] && c->cell_tab_sub[2] == A.sub[2];
+    if (!same) {
+        c->cell_tab_gen = 0;
+        hipError_t e = c->cell_tab.reserve((size_t)n);
+        if (e != hipSuccess) return e;
+        e = launch_cell_table(A.geom, 0, c->hg.g.n_det, A.sub, c->cell_tab.ptr, c->stream);
+        if (e != hipSuccess) return e;
+        c->cell_tab_gen = c->geom_gen;
+        for (int k = 0; k < 3; ++k) c->cell_tab_sub[k] = A.sub[k];
+    }
+    A.cells = c->cell_tab.ptr;
+    return hipSuccess;
+}
+
 smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
                          smg_precision prec, const float* pe, int pe_is_device, float* dst,
                          int dst_is_device, smg_stats* stats) {
@@ -442,6 +469,8 @@ smg_status smg_destroy(smg_ctx* c) {
     c->pts_hole.release();
     c->pair.release();
     c->sc_partial.release();
+    c->cell_tab.release();
+    c->cell_tab_pts.release();
     c->mu_tab_d.release();
     c->pts_grid.release();
     c->w1.release();
@@ -503,6 +532,7 @@ smg_status smg_set_geometry(smg_ctx* c, const float* coll, size_t n_coll, const 
     c->hg = std::move(hg);
     c->have_geom = true;
     c->have_bm_crystal = c->have_bm_coll = false;
+    ++c->geom_gen;
     return SMG_OK;
 }
 
@@ -579,7 +609,7 @@ smg_status smg_pe_forward(smg_ctx* c, int rotation, smg_precision prec, float* d
         SMG_HIP(c, c->out.reserve(n_el));
         out = c->out.ptr;
     }
-    PeLaunch A;
+    PeLaunch A{};
     A.geom = device_geom(c);
     A.dst = out;
     A.counters = c->counters.ptr;
@@ -588,6 +618,7 @@ smg_status smg_pe_forward(smg_ctx* c, int rotation, smg_precision prec, float* d
     A.sphi = std::sin(phi);
     for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
     pick_tile(g, A.tile, A.tiles);
+    SMG_HIP(c, pe_cell_table(c, A));
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
     SMG_HIP(c, launch_pe_forward(A, prec == SMG_PREC_FP64, c->stream));
@@ -803,6 +834,9 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
     P.sub[2] = gz;
     pick_tile(g, P.tile, P.tiles);
     P.pts = c->w1.ptr;
+    // sub-cell table of each chunk's bars (grid cells), up to 1 GiB
+    const bool cell_table = (int64_t)nbc * npb * (int64_t)sizeof(CellCtx) <= ((int64_t)1 << 30);
+    if (cell_table) SMG_HIP(c, c->cell_tab_pts.reserve((size_t)nbc * npb));
     ScatterLaunch A{};
     A.geom = P.geom;
     A.kind = 3;
@@ -850,6 +884,12 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
             }
         SMG_HIP(c, hipMemcpyAsync(c->pts_grid.ptr, pts.data(), sizeof(ScatterPt<double>) * nb * npb,
                                   hipMemcpyHostToDevice, c->stream));
+        P.cells = nullptr;
+        P.cells_bar0 = b0;
+        if (cell_table) {
+            SMG_HIP(c, launch_cell_table(P.geom, b0, nb, P.sub, c->cell_tab_pts.ptr, c->stream));
+            P.cells = c->cell_tab_pts.ptr;
+        }
         SMG_HIP(c, launch_pe_points(P, prec == SMG_PREC_FP64, b0, nb, c->stream));
         A.n_pts = nb * npb;
         A.pt_bar0 = b0;
@@ -885,7 +925,7 @@ smg_status smg_pe_backward(smg_ctx* c, int rotation, const float* w, int w_is_de
     if (!w || !grad) return fail(c, SMG_ERR_BAD_PARAM, "w and grad must not be NULL");
     const GeomDesc& g = c->hg.g;
     const size_t n_el = (size_t)g.n_det * (size_t)g.n_vox;
-    PeLaunch A;
+    PeLaunch A{};
     A.geom = device_geom(c);
     A.dst = nullptr;
     A.counters = c->counters.ptr;
@@ -894,6 +934,7 @@ smg_status smg_pe_backward(smg_ctx* c, int rotation, const float* w, int w_is_de
     A.sphi = std::sin(phi);
     for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
     pick_tile(g, A.tile, A.tiles);
+    SMG_HIP(c, pe_cell_table(c, A));
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
     PeGradScratch S;
     // enough waves to fill the GPU: about 8192 (bar, tile group) pairs

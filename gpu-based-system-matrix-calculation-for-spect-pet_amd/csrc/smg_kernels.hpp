@@ -26,7 +26,14 @@ struct PeLaunch {
     int bar0;                     // first bar of the launch (point weights)
     float* pts;                   // point weights [bar - bar0][cell][V]
     int cell_slices;              // point weights: waves per (bar, tile), each a range of cells
+    const CellCtx* cells;         // sub-cell table [bar - cells_bar0][cell] (cell_table), or
+    int cells_bar0;               // nullptr: the kernel forms each sub-cell itself
 };
+
+// Sub-cell table of bars bar0 .. bar0 + n_bars - 1 for sub-cells sub[3], in
+// the PE kernels' cell order: the same values those kernels would compute.
+hipError_t launch_cell_table(const PeGeom& geom, int bar0, int n_bars, const int sub[3], CellCtx* out,
+                             hipStream_t stream);
 
 hipError_t launch_pe_forward(const PeLaunch& a, bool exact, hipStream_t stream);
 // Crystal grid scatterers: first-leg weights of the sub-cell centres of bars

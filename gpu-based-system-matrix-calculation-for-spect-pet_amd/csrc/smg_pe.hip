@@ -73,6 +73,7 @@ struct WaveSetup {
     float udf[3], critf;
     double tc[3], rho;
     float tcf[3], rhof;
+    uint32_t bar_gate;  // bit m: no ray from bar d to the tile can be stopped by layer m's gate
 };
 
 __device__ __forceinline__ void wave_setup(const PeLaunch& A, int d, int tile, int lane, WaveSetup& W) {
@@ -97,6 +98,13 @@ __device__ __forceinline__ void wave_setup(const PeLaunch& A, int d, int tile, i
     W.critf = float(W.sc.crit);
     tile_ball(g, ix0, iy0, iz0, A.tile, A.cphi, A.sphi, W.tc, W.rho);
     tile_ball_f(W.tc, W.rho, W.tcf, W.rhof);
+    // plate gates that no ray from any point of bar d (inside its bounding
+    // ball, the half diagonal with a rounding margin) to the tile can trigger
+    const double qc[3] = {bd.cx, bd.cy, bd.cz};
+    const double rb = 0.5 * bd.rdiag * (1.0 + 1e-12) + 1e-9;
+    const bool clear = lane < g.n_layers &&
+                       plate_gate_clear_ball(G.layers_d[lane < g.n_layers ? lane : 0], qc, rb, W.tc, W.rho);
+    W.bar_gate = (uint32_t)__ballot(clear);
 }
 
 // Step 1: candidate bars and holes of the tile: those whose box a segment from
@@ -190,15 +198,25 @@ __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveS
     return nb <= kCapBars && nh <= kCapHoles;
 }
 
-// Sub-cell `cell` of bar d for this wave's tile, with the plate gates that no
-// ray of the tile can trigger (one layer per lane, then a ballot).
-__device__ __forceinline__ void tile_cell(const PeLaunch& A, const Bar<double>& bd, int cell,
+// Sub-cell `cell` of bar d for this wave's tile (from the sub-cell table when
+// there is one), with the plate gates that no ray of the tile can trigger:
+// those cleared for the whole bar, else tested for the sub-cell point (one
+// layer per lane, then a ballot).
+__device__ __forceinline__ void tile_cell(const PeLaunch& A, const Bar<double>& bd, int d, int cell,
                                           const WaveSetup& W, int lane, CellCtx& cc) {
-    make_cell_ctx_linear(bd, cell, A.sub, cc);
+    if (A.cells)
+        cc = A.cells[(int64_t)(d - A.cells_bar0) * (A.sub[0] * A.sub[1] * A.sub[2]) + cell];
+    else
+        make_cell_ctx_linear(bd, cell, A.sub, cc);
     const GeomDesc& g = A.geom.g;
-    bool clear = lane < g.n_layers && plate_gate_clear(A.geom.layers_d[lane < g.n_layers ? lane : 0],
-                                                       cc.q, W.tc, W.rho);
-    cc.gate_clear = (uint32_t)__ballot(clear);
+    const uint32_t all = (uint32_t)((1ull << g.n_layers) - 1ull);
+    if ((W.bar_gate & all) == all) {  // wave-uniform
+        cc.gate_clear = W.bar_gate;
+        return;
+    }
+    const bool clear = lane < g.n_layers && !((W.bar_gate >> lane) & 1u) &&
+                       plate_gate_clear(A.geom.layers_d[lane < g.n_layers ? lane : 0], cc.q, W.tc, W.rho);
+    cc.gate_clear = W.bar_gate | (uint32_t)__ballot(clear);
 }
 
 // Step 2: candidates whose box a segment from the sub-cell centre qd to the
@@ -316,7 +334,7 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
         int n_redo = 0;
         for (; cell < c_end && n_redo < kRedoCap; ++cell) {
             CellCtx cc;
-            tile_cell(A, bd, cell, W, lane, cc);
+            tile_cell(A, bd, d, cell, W, lane, cc);
             int nsb, nsh;
             __syncthreads();
             narrow_candidates(A, cc.q, W, lane, L, nb, nh, nsb, nsh);
@@ -347,7 +365,7 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
         if constexpr (!kExact) {
             for (int i = 0; i < n_redo; ++i) {
                 CellCtx cc;
-                tile_cell(A, bd, redo_cell[i], W, lane, cc);
+                tile_cell(A, bd, d, redo_cell[i], W, lane, cc);
                 int rsb, rsh;
                 __syncthreads();
                 narrow_candidates(A, cc.q, W, lane, L, nb, nh, rsb, rsh);
@@ -501,7 +519,7 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                 int n_redo = 0;
                 for (; cell < n_cells && n_redo < kRedoCap; ++cell) {
                     CellCtx cc;
-                    tile_cell(A, bd, cell, W, lane, cc);
+                    tile_cell(A, bd, d, cell, W, lane, cc);
                     int nsb, nsh;
                     __syncthreads();
                     narrow_candidates(A, cc.q, W, lane, L, nb, nh, nsb, nsh);
@@ -524,7 +542,7 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                 if constexpr (!kExact) {
                     for (int i = 0; i < n_redo; ++i) {
                         CellCtx cc;
-                        tile_cell(A, bd, redo_cell[i], W, lane, cc);
+                        tile_cell(A, bd, d, redo_cell[i], W, lane, cc);
                         int rsb, rsh;
                         __syncthreads();
                         narrow_candidates(A, cc.q, W, lane, L, nb, nh, rsb, rsh);
@@ -600,6 +618,34 @@ hipError_t launch_pe_backward(const PeLaunch& A, const PeGradScratch& S, bool ex
     const int n_out = A.geom.g.n_layers + 2 * A.geom.g.n_det + 1;
     hipLaunchKernelGGL(pe_grad_reduce, dim3((n_out + 255) / 256), dim3(256), 0, stream, S,
                        A.geom.g.n_det, A.geom.g.n_layers, grad);
+    return hipGetLastError();
+}
+
+namespace {
+
+// One thread per (bar, sub-cell): the sub-cell context the PE kernels would
+// form with make_cell_ctx_linear (same code, same values).
+__global__ __launch_bounds__(256) void cell_table_kernel(PeGeom G, int bar0, int n_bars, int s0, int s1,
+                                                         int s2, CellCtx* out) {
+    const int n_cells = s0 * s1 * s2;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)n_bars * n_cells) return;
+    const int b = (int)(tid / n_cells), cell = (int)(tid % n_cells);
+    const int sub[3] = {s0, s1, s2};
+    CellCtx cc;
+    make_cell_ctx_linear(G.bars_d[bar0 + b], cell, sub, cc);
+    cc.gate_clear = 0;
+    out[tid] = cc;
+}
+
+}  // namespace
+
+hipError_t launch_cell_table(const PeGeom& geom, int bar0, int n_bars, const int sub[3], CellCtx* out,
+                             hipStream_t stream) {
+    const int64_t total = (int64_t)n_bars * sub[0] * sub[1] * sub[2];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cell_table_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
+                       geom, bar0, n_bars, sub[0], sub[1], sub[2], out);
     return hipGetLastError();
 }
 

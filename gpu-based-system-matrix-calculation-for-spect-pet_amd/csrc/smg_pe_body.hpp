@@ -318,6 +318,49 @@ SMG_FN bool plate_gate_clear(const Layer<double>& L, const double q[3], const do
     return true;
 }
 
+// The same test for every segment from a point of the ball (qc, rb) to a
+// point of the ball (tc, rho): true when each such segment crosses the
+// plate's two y faces inside its x and z extents, more than 0.01 from either
+// end point.  With endpoints qc + f and tc + e (|f| <= rb, |e| <= rho) the
+// y faces are met at s = (Y - qc_y - f_y) / (u_y + e_y - f_y), u = tc - qc;
+// with numerator and denominator of fixed sign, s is monotone in each, so its
+// range is spanned by their extremes.  A point at s lies within
+// (1 - s) rb + s rho of qc + s u, a bound linear in s, so the x and z bounds
+// over an s range are taken at its ends.  Applied to a whole bar (qc its
+// centre, rb half its diagonal), it clears the gate for every sub-cell point
+// of the bar at once; rb = 0 gives the conditions of plate_gate_clear.
+SMG_FN bool plate_gate_clear_ball(const Layer<double>& L, const double qc[3], double rb,
+                                  const double tc[3], double rho) {
+    const double m = 0.01;
+    if (!(L.hi[1] - L.lo[1] >= 0.11)) return false;
+    const double u[3] = {tc[0] - qc[0], tc[1] - qc[1], tc[2] - qc[2]};
+    const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double R = rho + rb;
+    if (!(tabs(u[1]) > R + 1e-6) || !(lu > R + 1.0)) return false;
+    double s_lo = 1e300, s_hi = -1e300;
+    for (int f = 0; f < 2; ++f) {
+        const double num = (f ? L.hi[1] : L.lo[1]) - qc[1];
+        if (!(tabs(num) > rb + 1e-6)) return false;  // a face within reach of the bar ball
+        for (int en = -1; en <= 1; en += 2)
+            for (int ed = -1; ed <= 1; ed += 2) {
+                const double s = (num + en * rb) / (u[1] + ed * R);
+                s_lo = tmin2(s_lo, s);
+                s_hi = tmax2(s_hi, s);
+            }
+    }
+    // strictly inside the segment: more than 0.01 from either end point (the
+    // segment is at least lu - R long)
+    const double ds = m / (lu - R);
+    if (!(s_lo > ds) || !(s_hi < 1.0 - ds)) return false;
+    for (int k = 0; k < 3; k += 2) {
+        // qc_k + s u_k -+ ((1 - s) rb + s rho)
+        const double a = tmin2(qc[k] - rb + s_lo * (u[k] + rb - rho), qc[k] - rb + s_hi * (u[k] + rb - rho));
+        const double z = tmax2(qc[k] + rb + s_lo * (u[k] - rb + rho), qc[k] + rb + s_hi * (u[k] - rb + rho));
+        if (!(a > L.lo[k] + m) || !(z < L.hi[k] - m)) return false;
+    }
+    return true;
+}
+
 // Sub-cell number `cell` in the reference's loop order: z outermost, then x,
 // then y (PESysMatGen.cu:1193-1197).
 SMG_FN void make_cell_ctx_linear(const Bar<double>& bd, int cell, const int sub[3], CellCtx& cc) {

@@ -117,7 +117,8 @@ extern "C" {
 // stats[4] = mean bars in the full sets, stats[5] = mean tile candidates
 // (bars) kept by a culling test of some lane, stats[6] = share of (sub-cell,
 // layer) plate gates skipped; the narrowed terms are computed with those
-// gates skipped.
+// gates skipped; stats[7] = share of (wave, layer) gates cleared for the
+// whole bar (plate_gate_clear_ball), the rest tested per sub-cell.
 int lane_narrow_check(const float* coll, const float* det, const float* img, const float* phys,
                       int rot, int sx, int sy, int sz, int tx, int ty, int tz, int n_waves,
                       unsigned seed, double* stats) {
@@ -135,7 +136,8 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
         st ^= st << 17;
         return st;
     };
-    double bad = 0, checked = 0, nsb = 0, nsh = 0, nfull = 0, cells = 0, ntile = 0, nwav = 0, gates = 0;
+    double bad = 0, checked = 0, nsb = 0, nsh = 0, nfull = 0, cells = 0, ntile = 0, nwav = 0, gates = 0,
+           bar_gates = 0;
     for (int w = 0; w < n_waves; ++w) {
         int d = (int)(next() % g.n_det);
         int t = (int)(next() % ((uint64_t)nt[0] * nt[1] * nt[2]));
@@ -147,6 +149,15 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
         tile_ball(g, ix0, iy0, iz0, tile, cphi, sphi, tc, rho);
         float tcf[3], rhof;
         tile_ball_f(tc, rho, tcf, rhof);
+        // plate gates cleared for the whole bar (as the kernel's wave set-up)
+        uint32_t bar_gate = 0;
+        {
+            const double qc[3] = {bd.cx, bd.cy, bd.cz};
+            const double rb = 0.5 * bd.rdiag * (1.0 + 1e-12) + 1e-9;
+            for (int m = 0; m < g.n_layers; ++m)
+                if (plate_gate_clear_ball(G.layers_d[m], qc, rb, tc, rho)) bar_gate |= 1u << m;
+            bar_gates += __builtin_popcount(bar_gate);
+        }
         // full culling sets (every bar and hole), and the kernel's tile
         // candidates (bar_tile_may_hit first, then the culling test)
         const Bar<float>& bdf = G.bars_f[d];
@@ -224,8 +235,10 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
                     cells += 1;
                     // the same sub-cell with the tile's plate gates skipped
                     CellCtx cg = cc;
+                    cg.gate_clear = bar_gate;
                     for (int m = 0; m < g.n_layers; ++m)
-                        if (plate_gate_clear(G.layers_d[m], cc.q, tc, rho)) cg.gate_clear |= 1u << m;
+                        if (!((bar_gate >> m) & 1u) && plate_gate_clear(G.layers_d[m], cc.q, tc, rho))
+                            cg.gate_clear |= 1u << m;
                     gates += __builtin_popcount(cg.gate_clear);
                     for (int l = 0; l < tx * ty * tz; ++l) {
                         if (!valid[l]) continue;
@@ -245,6 +258,7 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
     stats[4] = checked > 0 ? nfull / (checked / (sx * sy * sz)) : 0;
     stats[5] = nwav > 0 ? ntile / nwav : 0;
     stats[6] = cells > 0 ? gates / (cells * (g.n_layers > 0 ? g.n_layers : 1)) : 0;
+    stats[7] = nwav > 0 ? bar_gates / (nwav * (g.n_layers > 0 ? g.n_layers : 1)) : 0;
     return 0;
 }
 

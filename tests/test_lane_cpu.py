@@ -87,11 +87,11 @@ def test_lane_narrowing_is_exact(lane_lib, params_mod, case):
         rot = 2
     else:
         p, sub, tile, waves = narrow_plate_geometry(params_mod), (3, 4, 2), (8, 4, 2), 60
-    st = np.zeros(7)
+    st = np.zeros(8)
     assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, rot, *sub, *tile, waves, 5, st) == 0
     print(f"narrowing {case}: {int(st[1])} terms, {st[2]:.2f} bars and {st[3]:.2f} holes per "
           f"sub-cell (full sets {st[4]:.1f} bars, tile candidates {st[5]:.1f}), "
-          f"{100 * st[6]:.1f}% of plate gates skipped")
+          f"{100 * st[6]:.1f}% of plate gates skipped, {100 * st[7]:.1f}% for the whole bar")
     assert st[1] > 0
     assert st[0] == 0
     if case == "narrowplate":
