@@ -11,9 +11,10 @@ the GPU.  Prints one JSON line.
 Unless --no-extras is given, the line also carries (under "extras") the
 all-FP64 time of the same step, the primary-Compton scatter throughputs with
 the reference's scatter points (crystal scatter on 32x32x1 voxels x 2048 bars,
-collimator scatter on 32x32x16 voxels x 64 bars) and with scatter points on a
-0.5 mm grid over the collimator tungsten, and the forward and adjoint
-(dL/dmu) times on the 32x32x1-voxel, 64-bar geometry.
+collimator scatter on 32x32x16 voxels x 64 bars), with scatter points on a
+0.5 mm grid over the collimator tungsten and on a 0.25 mm grid in every bar
+(8 detector rows of 2048 bars), and the forward and adjoint (dL/dmu) times on
+the 32x32x1-voxel, 64-bar geometry.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision fp32|fp64]
 """
@@ -33,8 +34,9 @@ METRIC = "matrix elements/s (PE, scatter) at ≤1e-5 rel err vs FP64; fwd+grad w
 # MI355X published peaks
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# measured HBM traffic of one FP32 launch of the full-size PE step (rocprofv3 PMC)
-TRAFFIC_BYTES = 2.18e9
+# measured HBM traffic of one FP32 launch of the full-size PE step (rocprofv3 PMC
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a plain 1 GiB copy)
+TRAFFIC_BYTES = 2.24e9
 
 
 def log(*a):
