@@ -269,3 +269,40 @@ def test_crystal_grid_rows_match_full_matrix(smg_mod, params_mod):
     assert np.array_equal(full, again)
     assert np.isfinite(full).all() and (full >= 0).all() and (full > 0).mean() > 0.5
     np.testing.assert_allclose(part, full[rows], rtol=2e-6, atol=0.0)
+
+
+def test_collimator_grid_scatter_half_mm_pitch(smg_mod, oracle_mod, params_mod):
+    """The benchmark's collimator grid: 0.5 mm pitch over the 60x5x60 mm
+    tungsten (143,440 points), config-3 image (32x32x16) and 64 bars, FP32
+    path, sampled elements against the oracle."""
+    p = params_mod.config(3)
+    o = oracle_mod.Oracle(p)
+    d, v = _sample(o, 48, 17)
+    with smg_mod.open_context(p) as ctx:
+        got, st, n = ctx.scatter_collimator_grid(0.5, 0, "fp32")
+    assert n > 140000
+    ref = o.coll_grid_scatter_pairs(0.5, d, v)
+    assert (ref > 0).all()
+    assert_parity(got[d, v], ref, TOL_FP32,
+                  f"collimator grid 0.5 mm fp32 (fallbacks {st['fp64_fallbacks']})")
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
+def test_crystal_grid_scatter_2048_bars(smg_mod, oracle_mod, params_mod, prec, tol):
+    """Crystal grid scatter on the benchmark's detector (2 layers of 32x32
+    bars, 32x32x1 voxels) with 2x8x2 cells per bar (65,536 points), four
+    detector rows, sampled elements against the oracle (the GPU crystal
+    bitmap, bit-exact elsewhere, gates both)."""
+    p = params_mod.config(4)
+    o = oracle_mod.Oracle(p)
+    rows = np.array([0, 700, 1400, 2047], np.int32)
+    rng = np.random.default_rng(23)
+    ri = rng.integers(0, len(rows), 48)
+    v = rng.integers(0, o.n_vox, 48)
+    with smg_mod.open_context(p) as ctx:
+        got, st = ctx.scatter_crystal_grid((2, 8, 2), rows, 0, prec)
+        bm = ctx.bitmap_crystal()
+    ref = o.crystal_grid_scatter_pairs(bm, (2, 8, 2), rows[ri], v)
+    assert (ref > 0).all()
+    assert_parity(got[ri, v], ref, tol,
+                  f"crystal grid 2x8x2, 2048 bars, {prec} (fallbacks {st['fp64_fallbacks']})")
