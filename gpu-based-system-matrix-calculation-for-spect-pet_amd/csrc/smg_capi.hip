@@ -917,8 +917,8 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
     return SMG_OK;
 }
 
-smg_status smg_pe_backward(smg_ctx* c, int rotation, const float* w, int w_is_device, double* grad,
-                           double* loss, smg_stats* stats) {
+smg_status smg_pe_backward(smg_ctx* c, int rotation, smg_precision prec, const float* w,
+                           int w_is_device, double* grad, double* loss, smg_stats* stats) {
     double t_start = now_ms();
     smg_status s = check_ready(c, rotation);
     if (s != SMG_OK) return s;
@@ -955,7 +955,7 @@ smg_status smg_pe_backward(smg_ctx* c, int rotation, const float* w, int w_is_de
     }
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
-    SMG_HIP(c, launch_pe_backward(A, S, false, c->grad.ptr, c->stream));
+    SMG_HIP(c, launch_pe_backward(A, S, prec == SMG_PREC_FP64, c->grad.ptr, c->stream));
     SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
     std::vector<double> out((size_t)n_out);
     unsigned long long cnt[4] = {0, 0, 0, 0};

@@ -94,7 +94,7 @@ def lib():
         "smg_set_scatter_subsamples": (i32, [vp, i32, i32, i32]),
         "smg_set_mu_tables": (i32, [vp, i32, dp, dp, dp, dp, dp]),
         "smg_pe_forward": (i32, [vp, i32, i32, vp, i32, st]),
-        "smg_pe_backward": (i32, [vp, i32, vp, i32, dp, dp, st]),
+        "smg_pe_backward": (i32, [vp, i32, i32, vp, i32, dp, dp, st]),
         "smg_bitmap_crystal_words": (i64, [i32]),
         "smg_bitmap_collimator_words": (i64, [i32, i32]),
         "smg_bitmap_crystal": (i32, [vp, vp, i64, i32]),
@@ -231,14 +231,16 @@ class Context:
                                            out.ctypes.data_as(C.c_void_p), 0, C.byref(st)))
         return out, st.as_dict()
 
-    def pe_backward(self, w: np.ndarray, rotation: int = 0) -> tuple[float, np.ndarray, dict]:
+    def pe_backward(self, w: np.ndarray, rotation: int = 0,
+                    precision: str = "fp32") -> tuple[float, np.ndarray, dict]:
         """Loss sum(w * A) and its gradient: [layer mu][bar mu_total][bar mu_pe]."""
         w = np.ascontiguousarray(w, dtype=np.float32).reshape(self.dims.n_det, self.dims.n_vox)
         n_par = self._n_layers + 2 * self.dims.n_det
         grad = np.zeros(n_par, np.float64)
         loss = C.c_double()
         st = Stats()
-        self._check(self._L.smg_pe_backward(self._h, rotation, w.ctypes.data_as(C.c_void_p), 0,
+        self._check(self._L.smg_pe_backward(self._h, rotation, PREC[precision],
+                                            w.ctypes.data_as(C.c_void_p), 0,
                                             grad.ctypes.data_as(C.POINTER(C.c_double)),
                                             C.byref(loss), C.byref(st)))
         return loss.value, grad, st.as_dict()

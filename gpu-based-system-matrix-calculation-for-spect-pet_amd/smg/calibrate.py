@@ -2,14 +2,30 @@
 
 Recovers the collimator layers' mu and the bars' mu_total / mu_pe from
 point-source projections: each source is one voxel v, its projection is the
-column A[:, v].  The loss is the relative squared residual over the chosen
-sources,
+column A[:, v].  The residuals are relative,
 
-    L(mu) = sum_{d, v in S} ((A[d][v](mu) - Y[d][v]) / Y[d][v])^2,
+    r[d][v] = (A[d][v](mu) - Y[d][v]) / Y[d][v]  for v in the sources,
+    L(mu) = sum r^2,
 
-so dL/dmu = backward(w) with w = 2 (A - Y) / Y^2 on the sources (0 elsewhere).
-The parameters are optimised in log space with Adam.  Everything runs through
-the C ABI (smg_pe_forward / smg_pe_backward) on the GPU.
+so dL/dmu = backward(w) with w = 2 r / Y on the sources (0 elsewhere).  The
+parameters are theta = log mu.  Everything runs through the C ABI
+(smg_pe_forward / smg_pe_backward) on the GPU.
+
+Methods:
+  "levenberg-marquardt" (default): damped Gauss-Newton steps
+      (J^T J + lam diag(J^T J)) delta = -g / 2,
+  with g = dL/dtheta from the adjoint and J^T J from forward differences of
+  the GPU forward (one forward per parameter); a step is taken when it lowers
+  L (lam / 3), else lam * 4 and retry.  The problem is ill-conditioned (the
+  Jacobian's condition number is ~1.6e4 on the 64-bar geometry: a common
+  scale of the bars' mu_pe against the plate mu is the weakest direction), so
+  first-order methods crawl along that valley; the Gauss-Newton curvature
+  removes it.
+  "adam": first-order steps on theta with Adam (lr per step).
+  "lbfgs": scipy's L-BFGS-B on theta with the adjoint gradient.
+The mu live in float32 arrays (the parameter-file format), so theta is kept
+at the log of float32 values and each finite-difference step is divided by
+the float32 step actually taken.
 """
 from __future__ import annotations
 
@@ -49,42 +65,133 @@ class CalibrationResult:
     losses: list = field(default_factory=list)
     forward_ms: float = 0.0
     backward_ms: float = 0.0
+    n_forward: int = 0
+    n_backward: int = 0
+
+
+class _Problem:
+    """Forward / adjoint evaluations of the calibration loss on one context."""
+
+    def __init__(self, p_start: Params, target: np.ndarray, sources: np.ndarray, sub, precision,
+                 device: int, res: CalibrationResult):
+        self.p = p_start
+        self.target = np.asarray(target, np.float64)
+        self.sources = np.asarray(sources)
+        self.prec = precision
+        self.res = res
+        self.ctx = open_context(p_start, device)
+        self.ctx.set_pe_subsamples(*sub)
+        self.w = np.zeros((self.ctx.dims.n_det, self.ctx.dims.n_vox), np.float32)
+
+    def close(self):
+        self.ctx.close()
+
+    def residuals(self, mu32: np.ndarray) -> np.ndarray:
+        self.ctx.set_geometry(set_mu(self.p, mu32))
+        A, st = self.ctx.pe_forward(0, self.prec)
+        self.res.forward_ms += st["kernel_ms"]
+        self.res.n_forward += 1
+        return (A[:, self.sources].astype(np.float64) - self.target) / self.target
+
+    def grad_theta(self, mu32: np.ndarray, r: np.ndarray) -> np.ndarray:
+        """dL/dtheta at the mu of the last residuals call (the context's geometry)."""
+        self.w[:] = 0.0
+        self.w[:, self.sources] = (2.0 * r / self.target).astype(np.float32)
+        _, g, st = self.ctx.pe_backward(self.w, precision=self.prec)
+        self.res.backward_ms += st["kernel_ms"]
+        self.res.n_backward += 1
+        return g * mu32
+
+
+def _f32(mu: np.ndarray) -> np.ndarray:
+    return mu.astype(np.float32).astype(np.float64)
 
 
 def calibrate(p_start: Params, target: np.ndarray, sources: np.ndarray, sub=(4, 4, 4),
-              steps: int = 200, lr: float = 0.02, device: int = 0) -> CalibrationResult:
+              steps: int = 20, lr: float = 0.02, device: int = 0,
+              method: str = "levenberg-marquardt", precision: str = "fp32",
+              fd_step: float = 1e-4, lam0: float = 0.1) -> CalibrationResult:
     """target: [N][n_sources] projections of the sources (columns of the true
-    matrix).  Returns the fitted mu (gradient order) and the loss history."""
-    mu = get_mu(p_start)
+    matrix).  steps: iterations of the method (for "lbfgs", evaluations).
+    Returns the fitted mu (gradient order) and the loss after each step."""
+    mu = _f32(get_mu(p_start))
+    res = CalibrationResult(mu=mu.copy())
+    prob = _Problem(p_start, target, sources, sub, precision, device, res)
+    try:
+        if method == "levenberg-marquardt":
+            mu = _lm(prob, mu, steps, fd_step, lam0)
+        elif method == "adam":
+            mu = _adam(prob, mu, steps, lr)
+        elif method == "lbfgs":
+            mu = _lbfgs(prob, mu, steps)
+        else:
+            raise ValueError(f"unknown method {method!r}")
+    finally:
+        prob.close()
+    res.mu = mu
+    return res
+
+
+def _lm(prob: _Problem, mu: np.ndarray, steps: int, h: float, lam: float) -> np.ndarray:
+    r = prob.residuals(mu)
+    L = float(np.sum(r * r))
+    prob.res.losses.append(L)
+    n = mu.size
+    for _ in range(steps):
+        g = prob.grad_theta(mu, r)
+        # J^T J from forward differences in theta, each divided by the float32 step taken
+        J = np.empty((r.size, n))
+        for j in range(n):
+            mj = mu.copy()
+            mj[j] = _f32(np.array([mu[j] * np.exp(h)]))[0]
+            J[:, j] = ((prob.residuals(mj) - r) / np.log(mj[j] / mu[j])).ravel()
+        H = J.T @ J
+        D = np.diag(np.diag(H))
+        while True:
+            delta = np.linalg.solve(H + lam * D, -0.5 * g)
+            mu_new = _f32(mu * np.exp(delta))
+            r_new = prob.residuals(mu_new)
+            L_new = float(np.sum(r_new * r_new))
+            if L_new < L:
+                mu, r, L = mu_new, r_new, L_new
+                lam = max(lam / 3.0, 1e-12)
+                break
+            lam *= 4.0
+            if lam > 1e8:  # no step lowers L: converged to the arithmetic's resolution
+                prob.res.losses.append(L)
+                return mu
+        prob.res.losses.append(L)
+    return mu
+
+
+def _adam(prob: _Problem, mu: np.ndarray, steps: int, lr: float) -> np.ndarray:
     theta = np.log(mu)
     m = np.zeros_like(theta)
     v = np.zeros_like(theta)
     b1, b2, eps = 0.9, 0.999, 1e-12
-    res = CalibrationResult(mu=mu.copy())
-    ctx = open_context(p_start, device)
-    ctx.set_pe_subsamples(*sub)
-    n_det, n_vox = ctx.dims.n_det, ctx.dims.n_vox
-    w = np.zeros((n_det, n_vox), np.float32)
-    try:
-        for step in range(1, steps + 1):
-            p = set_mu(p_start, np.exp(theta))
-            ctx.set_geometry(p)
-            A, st_f = ctx.pe_forward(0, "fp32")
-            pred = A[:, sources].astype(np.float64)
-            r = (pred - target) / target
-            res.losses.append(float(np.sum(r * r)))
-            w[:] = 0.0
-            w[:, sources] = (2.0 * r / target).astype(np.float32)
-            _, grad, st_b = ctx.pe_backward(w)
-            res.forward_ms += st_f["kernel_ms"]
-            res.backward_ms += st_b["kernel_ms"]
-            g = grad * np.exp(theta)  # d/d(log mu)
-            m = b1 * m + (1 - b1) * g
-            v = b2 * v + (1 - b2) * g * g
-            mh = m / (1 - b1 ** step)
-            vh = v / (1 - b2 ** step)
-            theta = theta - lr * mh / (np.sqrt(vh) + eps)
-    finally:
-        ctx.close()
-    res.mu = np.exp(theta)
-    return res
+    for step in range(1, steps + 1):
+        mu32 = _f32(np.exp(theta))
+        r = prob.residuals(mu32)
+        prob.res.losses.append(float(np.sum(r * r)))
+        g = prob.grad_theta(mu32, r)
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        mh = m / (1 - b1 ** step)
+        vh = v / (1 - b2 ** step)
+        theta = theta - lr * mh / (np.sqrt(vh) + eps)
+    return _f32(np.exp(theta))
+
+
+def _lbfgs(prob: _Problem, mu: np.ndarray, evals: int) -> np.ndarray:
+    from scipy.optimize import minimize
+
+    def fg(theta):
+        mu32 = _f32(np.exp(theta))
+        r = prob.residuals(mu32)
+        L = float(np.sum(r * r))
+        prob.res.losses.append(L)
+        return L, prob.grad_theta(mu32, r)
+
+    out = minimize(fg, np.log(mu), jac=True, method="L-BFGS-B",
+                   options=dict(maxiter=evals, maxfun=evals, ftol=0.0, gtol=0.0, maxcor=30))
+    return _f32(np.exp(out.x))

@@ -115,9 +115,12 @@ smg_status smg_pe_forward(smg_ctx* ctx, int rotation, smg_precision prec, float*
 /* dL/dmu of L = sum_dv w[d][v] A[d][v] (adjoint of smg_pe_forward), for the
  * mu of each collimator layer, mu_total and mu_pe of each bar:
  * grad[0 .. Lc-1], grad[Lc .. Lc+N-1], grad[Lc+N .. Lc+2N-1].  w is N x V.
- * loss (optional) receives L.  Deterministic: fixed-order reductions. */
-smg_status smg_pe_backward(smg_ctx* ctx, int rotation, const float* w, int w_is_device,
-                           double* grad, double* loss, smg_stats* stats);
+ * loss (optional) receives L.  prec selects the ray arithmetic as in
+ * smg_pe_forward (FP32 with FP64 re-computation near thresholds, or all
+ * FP64); the sums are FP64 either way.  Deterministic: fixed-order
+ * reductions.  No counterpart in the reference, which has no gradient. */
+smg_status smg_pe_backward(smg_ctx* ctx, int rotation, smg_precision prec, const float* w,
+                           int w_is_device, double* grad, double* loss, smg_stats* stats);
 
 /* ---- geometry bitmaps (the scatter kernels' attenuation culling) ------------
  * Crystal bitmap: N(N+1)/2 * N bits; collimator bitmap: H0 * N * N bits,

@@ -15,24 +15,29 @@ def cfg5(params_mod):
     return p, w
 
 
-def test_gradient_matches_fp64_host(smg_mod, oracle_mod, cfg5):
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("fp64", 1e-7)])
+def test_gradient_matches_fp64_host(smg_mod, oracle_mod, cfg5, prec, tol):
+    """FP32 path: components within 1e-5 of the largest and within 1e-4
+    relative where above 1e-3 of it; all-FP64 path: 1e-7 and 1e-6."""
     p, w = cfg5
     sub = p.meta["subsamples"]
     loss_ref, g_ref = oracle_mod.Oracle(p).pe_grad(w, sub=sub)
     with smg_mod.open_context(p) as ctx:
         ctx.set_pe_subsamples(*sub)
-        loss, g, st = ctx.pe_backward(w.astype(np.float32))
-        A, _ = ctx.pe_forward(0, "fp32")
+        loss, g, st = ctx.pe_backward(w.astype(np.float32), precision=prec)
+        A, _ = ctx.pe_forward(0, prec)
     w32 = w.astype(np.float32).astype(np.float64)
     assert loss == pytest.approx(float(np.sum(w32 * A.astype(np.float64))), rel=1e-6)
     _, g_ref32 = oracle_mod.Oracle(p).pe_grad(w32, sub=sub)
     scale = np.abs(g_ref32).max()
     err = np.abs(g - g_ref32)
-    print(f"adjoint: max |dg| / max |g| = {err.max() / scale:.2e}, kernel {st['kernel_ms']:.2f} ms", flush=True)
-    assert err.max() <= 1e-5 * scale
     big = np.abs(g_ref32) > 1e-3 * scale
-    assert np.all(err[big] <= 1e-4 * np.abs(g_ref32[big]))
-    assert np.abs(loss_ref - loss) <= 1e-5 * abs(loss_ref)
+    print(f"adjoint {prec}: max |dg| / max |g| = {err.max() / scale:.2e}, max relative on the "
+          f"large components {np.max(err[big] / np.abs(g_ref32[big])):.2e}, "
+          f"kernel {st['kernel_ms']:.2f} ms", flush=True)
+    assert err.max() <= tol * scale
+    assert np.all(err[big] <= 10 * tol * np.abs(g_ref32[big]))
+    assert np.abs(loss_ref - loss) <= tol * abs(loss_ref)
 
 
 def test_gradient_matches_finite_differences(smg_mod, params_mod, cfg5):
@@ -73,25 +78,50 @@ def test_backward_deterministic(smg_mod, cfg5):
     assert np.array_equal(a, b)
 
 
-def test_calibration_recovers_mu(smg_mod, params_mod, cfg5):
-    """200 Adam steps from mu perturbed by +-5% fit 1000 point-source
-    projections (distinct voxels, seed 2).  The collimator mu is well
-    determined and is recovered; per-bar mu_total / mu_pe are weakly
-    identifiable from these projections (Jacobian condition number ~1.6e4:
-    the weakest direction trades a common scale of all mu_pe against the
-    plate mu), so the fit is checked in output space."""
+def _calibration_case(smg_mod, p, prec="fp32"):
+    """1000 point sources (distinct voxels, seed 2), their projections at the
+    true mu, and a start with every mu off by +-5%."""
     from smg import calibrate
-    p, _ = cfg5
     sub = (2, 2, 2)
     rng = np.random.default_rng(2)
     sources = np.sort(rng.choice(p.n_vox, 1000, replace=False))
     mu_true = calibrate.get_mu(p)
     with smg_mod.open_context(p) as ctx:
         ctx.set_pe_subsamples(*sub)
-        A, _ = ctx.pe_forward(0, "fp32")
+        A, _ = ctx.pe_forward(0, prec)
     target = A[:, sources].astype(np.float64)
     mu_start = mu_true * (1 + rng.choice([-0.05, 0.05], mu_true.size))
-    res = calibrate.calibrate(calibrate.set_mu(p, mu_start), target, sources, sub=sub, steps=200)
+    return sub, sources, mu_true, target, calibrate.set_mu(p, mu_start)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_calibration_recovers_every_mu(smg_mod, params_mod, cfg5, prec):
+    """Levenberg-Marquardt (adjoint gradient, Gauss-Newton curvature from
+    forward differences) from mu perturbed by +-5%: all 129 mu (plate,
+    mu_total and mu_pe of 64 bars) recovered to 1e-3 within 20 steps."""
+    from smg import calibrate
+    p, _ = cfg5
+    sub, sources, mu_true, target, p_start = _calibration_case(smg_mod, p, prec)
+    res = calibrate.calibrate(p_start, target, sources, sub=sub, steps=20, precision=prec)
+    err = np.abs(res.mu / mu_true - 1)
+    print(f"calibration LM {prec}: loss {res.losses[0]:.3e} -> {res.losses[-1]:.3e} in "
+          f"{len(res.losses) - 1} steps ({res.n_forward} forward, {res.n_backward} backward "
+          f"passes), max mu error 0.050 -> {err.max():.2e} (plate {err[0]:.2e})", flush=True)
+    assert all(b <= a for a, b in zip(res.losses, res.losses[1:]))
+    assert err.max() < 1e-3
+
+
+def test_calibration_adam_fits_projections(smg_mod, params_mod, cfg5):
+    """200 Adam steps from the same start fit the projections and recover
+    the plate mu; per-bar mu_total / mu_pe are weakly identifiable from these
+    projections (Jacobian condition number ~1.6e4: the weakest direction
+    trades a common scale of all mu_pe against the plate mu) and first-order
+    steps stay within ~10% of them, so Adam's fit is checked in output
+    space."""
+    from smg import calibrate
+    p, _ = cfg5
+    sub, sources, mu_true, target, p_start = _calibration_case(smg_mod, p)
+    res = calibrate.calibrate(p_start, target, sources, sub=sub, steps=200, method="adam")
     err = np.abs(res.mu / mu_true - 1)
     rms = np.sqrt(res.losses[-1] / target.size)
     print(f"calibration: loss {res.losses[0]:.3e} -> {res.losses[-1]:.3e} (rms rel residual "
