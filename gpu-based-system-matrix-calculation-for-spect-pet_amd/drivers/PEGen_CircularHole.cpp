@@ -8,7 +8,9 @@
 // PESysMatGen() and writes PE_SysMat_shift_<x>_<y>_<z>.sysmat, FP32
 // [rotation][bar][voxel].
 // Additions: -sub SX SY SZ (sub-cells per bar, default 8 16 8) and -fp64
-// (all-FP64 kernels); they set SMG_PE_SUBSAMPLES / SMG_PRECISION.
+// (all-FP64 kernels); they set SMG_PE_SUBSAMPLES / SMG_PRECISION.  The file is
+// written one rotation at a time (the reference keeps all rotations in host
+// memory), so its size is not bounded by host memory.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -46,14 +48,15 @@ int main(int argc, char* argv[]) {
     drv::Dims d = drv::dims_of(p);
     std::printf("FOV dimension: %d %d %d, %d bars, %d rotations\n", d.nx, d.ny, d.nz, d.n_det,
                 d.n_rot);
-    std::vector<float> out(d.block() * (size_t)d.n_rot, 0.0f);
+    // one rotation block in memory at a time; block r is written while r + 1 is computed
+    drv::BlockWriter out(drv::sysmat_name("PE_SysMat", p), d.block());
     for (int r = 0; r < d.n_rot; ++r) {
         std::printf("rotation %d\n", r);
         p.img[20] = (float)r;
-        int q = PESysMatGen(p.coll.data(), p.det.data(), p.img.data(), out.data() + (size_t)r * d.block(),
-                            cuda_id);
+        int q = PESysMatGen(p.coll.data(), p.det.data(), p.img.data(), out.buffer(), cuda_id);
         std::printf("numImagebin = %d\n", q);
+        out.commit();
     }
-    drv::write_matrix(drv::sysmat_name("PE_SysMat", p), out);
+    out.close();
     return 0;
 }

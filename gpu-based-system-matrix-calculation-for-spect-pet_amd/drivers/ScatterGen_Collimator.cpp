@@ -13,21 +13,11 @@ int main(int argc, char* argv[]) {
     drv::print_layers(p);
     drv::ScatterArgs a = drv::parse_scatter_args(argc, argv, p, 2);
     drv::Dims d = drv::dims_of(p);
-    std::vector<float> pe(d.block() * (size_t)d.n_rot), out(d.block() * (size_t)d.n_rot, 0.0f);
-    drv::read_matrix(a.pe_file, pe);
-    for (int r = 0; r < d.n_rot; ++r) {
-        std::printf("rotation %d\n", r);
-        p.img[20] = (float)r;
-        float* pe_block = pe.data() + (a.pe_per_rotation ? (size_t)r * d.block() : 0);
-        int q = scatter(p.coll.data(), p.det.data(), p.img.data(), p.phys.data(), pe_block,
-                        a.geo_coll.c_str(), out.data() + (size_t)r * d.block(), a.cuda_id);
-        std::printf("numImagebin = %d\n", q);
-    }
-    drv::write_matrix(drv::sysmat_name("Scatter_SysMat", p), out);
-    if (p.phys[3] == 1) {
-        std::vector<float> sum(out.size());
-        for (size_t i = 0; i < out.size(); ++i) sum[i] = pe[i] + out[i];
-        drv::write_matrix(drv::sysmat_name("SysMat", p), sum);
-    }
+    drv::scatter_rotations(p, d, a, "Scatter_SysMat", true, "SysMat",
+                           [&](int, float* pe_block, float* dst) {
+                               int q = scatter(p.coll.data(), p.det.data(), p.img.data(), p.phys.data(),
+                                               pe_block, a.geo_coll.c_str(), dst, a.cuda_id);
+                               std::printf("numImagebin = %d\n", q);
+                           });
     return 0;
 }

@@ -7,11 +7,14 @@
 // reference read them unchecked.
 #pragma once
 
+#include <sys/types.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace drv {
@@ -80,28 +83,97 @@ inline std::string sysmat_name(const char* prefix, const Params& p) {
     return buf;
 }
 
-inline void write_matrix(const std::string& name, const std::vector<float>& m) {
-    FILE* f = std::fopen(name.c_str(), "wb+");
-    if (!f || std::fwrite(m.data(), sizeof(float), m.size(), f) != m.size()) {
-        std::fprintf(stderr, "cannot write %s\n", name.c_str());
-        std::exit(EXIT_FAILURE);
+// Sequential writer of a [rotation][bar][voxel] matrix file, one rotation
+// block at a time, so a driver holds two blocks in memory whatever the number
+// of rotations (the reference drivers keep the whole matrix, R x N x V floats,
+// and write it at the end).  A committed block is written by a background
+// thread while the caller fills the other buffer; the file is built as
+// "<name>.part" and renamed to <name> once complete.  The bytes are those of
+// the whole-matrix write.
+class BlockWriter {
+  public:
+    BlockWriter(const std::string& name, size_t block)
+        : name_(name), part_(name + ".part"), block_(block) {
+        f_ = std::fopen(part_.c_str(), "wb");
+        if (!f_) fail("cannot write");
+        buf_[0].assign(block, 0.0f);
+        buf_[1].assign(block, 0.0f);
     }
-    std::fclose(f);
-    std::printf("%s written\n", name.c_str());
-}
+    BlockWriter(const BlockWriter&) = delete;
+    BlockWriter& operator=(const BlockWriter&) = delete;
+    ~BlockWriter() {
+        if (th_.joinable()) th_.join();
+        if (f_) std::fclose(f_);
+    }
+    // the buffer of the next block
+    float* buffer() { return buf_[cur_].data(); }
+    // hand the buffer's block to the writer thread (after the previous write)
+    void commit() {
+        if (th_.joinable()) th_.join();
+        if (!ok_) fail("cannot write");
+        const float* b = buf_[cur_].data();
+        th_ = std::thread([this, b] { ok_ = std::fwrite(b, sizeof(float), block_, f_) == block_; });
+        cur_ ^= 1;
+    }
+    void close() {
+        if (th_.joinable()) th_.join();
+        if (!ok_) fail("cannot write");
+        if (std::fclose(f_) != 0) {
+            f_ = nullptr;
+            fail("cannot write");
+        }
+        f_ = nullptr;
+        if (std::rename(part_.c_str(), name_.c_str()) != 0) fail("cannot rename to");
+        std::printf("%s written\n", name_.c_str());
+    }
 
-inline void read_matrix(const std::string& name, std::vector<float>& m) {
-    FILE* f = std::fopen(name.c_str(), "rb");
-    if (!f) {
-        std::fprintf(stderr, "cannot open PE matrix %s\n", name.c_str());
+  private:
+    [[noreturn]] void fail(const char* what) {
+        std::fprintf(stderr, "%s %s\n", what, name_.c_str());
         std::exit(EXIT_FAILURE);
     }
-    size_t got = std::fread(m.data(), sizeof(float), m.size(), f);
-    std::fclose(f);
-    if (got != m.size()) {
-        std::fprintf(stderr, "%s holds %zu floats, %zu expected\n", name.c_str(), got, m.size());
-        std::exit(EXIT_FAILURE);
+    std::string name_, part_;
+    size_t block_;
+    FILE* f_ = nullptr;
+    std::vector<float> buf_[2];
+    int cur_ = 0;
+    bool ok_ = true;  // set by the writer thread, read after joining it
+    std::thread th_;
+};
+
+// Rotation blocks of a [rotation][bar][voxel] matrix file, read one at a time;
+// the file must hold at least n_blocks blocks (the reference reads them all).
+class BlockReader {
+  public:
+    BlockReader(const std::string& name, size_t block, int n_blocks) : name_(name), block_(block) {
+        f_ = std::fopen(name.c_str(), "rb");
+        if (!f_) {
+            std::fprintf(stderr, "cannot open PE matrix %s\n", name.c_str());
+            std::exit(EXIT_FAILURE);
+        }
+        const size_t want = block * (size_t)n_blocks;
+        size_t have = 0;
+        if (fseeko(f_, 0, SEEK_END) == 0) have = (size_t)ftello(f_) / sizeof(float);
+        if (have < want) {
+            std::fprintf(stderr, "%s holds %zu floats, %zu expected\n", name.c_str(), have, want);
+            std::exit(EXIT_FAILURE);
+        }
     }
-}
+    BlockReader(const BlockReader&) = delete;
+    BlockReader& operator=(const BlockReader&) = delete;
+    ~BlockReader() { std::fclose(f_); }
+    void read(int r, float* dst) {
+        if (fseeko(f_, (off_t)r * (off_t)block_ * (off_t)sizeof(float), SEEK_SET) != 0 ||
+            std::fread(dst, sizeof(float), block_, f_) != block_) {
+            std::fprintf(stderr, "cannot read rotation %d of %s\n", r, name_.c_str());
+            std::exit(EXIT_FAILURE);
+        }
+    }
+
+  private:
+    std::string name_;
+    size_t block_;
+    FILE* f_ = nullptr;
+};
 
 }  // namespace drv

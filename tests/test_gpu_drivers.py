@@ -86,3 +86,19 @@ def test_standalone_scatter_drivers(run_dir, smg_mod):
     assert np.array_equal(cry[1], ref_cry)
     total = np.fromfile(d / shifted("SysMat", p), np.float32).reshape(2, 64, 1024)
     np.testing.assert_array_equal(total, pe + cry)  # the crystal driver ran last
+
+
+def test_scatter_driver_pe_per_rotation(run_dir, smg_mod):
+    """-PErot drives each rotation with its own PE block (the reference passes
+    rotation 0's for all); the files are written one rotation block at a time
+    and no partial file is left behind."""
+    d, p = run_dir
+    run("ScatterGen_Crystal", ["-PErot"], d)
+    pe = np.fromfile(d / shifted("PE_SysMat", p), np.float32).reshape(2, 64, 1024)
+    cry = np.fromfile(d / shifted("Scatter_Crystal_SysMat", p), np.float32).reshape(2, 64, 1024)
+    with smg_mod.open_context(p) as ctx:
+        for r in range(2):
+            ref, _ = ctx.scatter_crystal(pe[r], r, "fp32")
+            assert np.array_equal(cry[r], ref)
+    assert not np.array_equal(pe[0], pe[1])
+    assert not list(d.glob("*.part"))

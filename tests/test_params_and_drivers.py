@@ -81,3 +81,17 @@ def test_scatter_driver_missing_pe_file(exe, tmp_path, params_mod):
     params_mod.config(1).write(str(tmp_path))
     r = run(exe, ["-PE", "nope.sysmat"], tmp_path)
     assert r.returncode == 1 and "nope.sysmat" in r.stderr
+
+
+@pytest.mark.parametrize("exe", DRIVERS[1:])
+def test_scatter_driver_short_pe_file(exe, tmp_path, params_mod):
+    """A PE matrix with fewer rotation blocks than the image asks for is an
+    error before any output file is opened."""
+    p = params_mod.config(1)
+    params_mod.set_image(p, 32, 32, 1, 1.0, 1.0, 1.0, n_rot=2, dphi=0.2)
+    p.write(str(tmp_path))
+    np.zeros(64 * 1024 + 5, np.float32).tofile(tmp_path / "short.sysmat")  # one block and a bit
+    r = run(exe, ["-PE", "short.sysmat"], tmp_path)
+    assert r.returncode == 1 and "short.sysmat holds 65541 floats, 131072 expected" in r.stderr
+    assert sorted(x.name for x in tmp_path.iterdir() if x.suffix in (".sysmat", ".part")) == \
+        ["short.sysmat"]
