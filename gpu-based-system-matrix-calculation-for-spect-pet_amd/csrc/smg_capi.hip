@@ -78,6 +78,7 @@ struct smg_ctx {
     DevBuf<ScatterPt<double>> pts_hole;  // layer-0 hole centres (collimator scatter)
     DevBuf<PairTerm> pair;
     DevBuf<double> sc_partial;   // per-slice scatter sums
+    DevBuf<double> proj;         // projection partial sums
     DevBuf<CellCtx> cell_tab;    // PE sub-cell table of every bar
     DevBuf<CellCtx> cell_tab_pts;  // sub-cell table of one chunk of crystal grid points
     int cell_tab_sub[3] = {0, 0, 0};
@@ -469,6 +470,7 @@ smg_status smg_destroy(smg_ctx* c) {
     c->pts_hole.release();
     c->pair.release();
     c->sc_partial.release();
+    c->proj.release();
     c->cell_tab.release();
     c->cell_tab_pts.release();
     c->mu_tab_d.release();
@@ -643,6 +645,78 @@ smg_status smg_pe_forward(smg_ctx* c, int rotation, smg_precision prec, float* d
                     "%llu waves exceeded the candidate capacity (%d bars, %d holes)", cnt[1],
                     kCapBars, kCapHoles);
     return SMG_OK;
+}
+
+// Matrix-free projections (smg_pe_project / smg_pe_backproject).
+static smg_status project_entry(smg_ctx* c, int rotation, smg_precision prec, bool back, const float* in,
+                                int in_is_device, float* out, int out_is_device, smg_stats* stats) {
+    double t_start = now_ms();
+    smg_status s = check_ready(c, rotation);
+    if (s != SMG_OK) return s;
+    if (!in || !out) return fail(c, SMG_ERR_BAD_PARAM, "input and output must not be NULL");
+    const GeomDesc& g = c->hg.g;
+    const size_t n_in = back ? (size_t)g.n_det : (size_t)g.n_vox;
+    const size_t n_out = back ? (size_t)g.n_vox : (size_t)g.n_det;
+    PeLaunch A{};
+    A.geom = device_geom(c);
+    A.counters = c->counters.ptr;
+    const double phi = (double)rotation * g.dphi;
+    A.cphi = std::cos(phi);
+    A.sphi = std::sin(phi);
+    for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
+    A.cell_slices = 1;
+    pick_tile(g, A.tile, A.tiles);
+    SMG_HIP(c, pe_cell_table(c, A));
+    const float* in_d = in;
+    if (!in_is_device) {
+        SMG_HIP(c, c->in.reserve(n_in));
+        SMG_HIP(c, hipMemcpyAsync(c->in.ptr, in, n_in * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        in_d = c->in.ptr;
+    }
+    float* out_d = out;
+    if (!out_is_device) {
+        SMG_HIP(c, c->out.reserve(n_out));
+        out_d = c->out.ptr;
+    }
+    SMG_HIP(c, c->proj.reserve(project_scratch(A, back)));
+    SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    if (back)
+        SMG_HIP(c, launch_pe_backproject(A, prec == SMG_PREC_FP64, in_d, c->proj.ptr, out_d, c->stream));
+    else
+        SMG_HIP(c, launch_pe_project(A, prec == SMG_PREC_FP64, in_d, c->proj.ptr, out_d, c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    if (!out_is_device)
+        SMG_HIP(c, hipMemcpyAsync(out, out_d, n_out * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    SMG_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stats) {
+        const uint64_t n_el = (uint64_t)g.n_det * (uint64_t)g.n_vox;
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t_start;
+        stats->elements = n_el;
+        stats->rays = n_el * (uint64_t)(c->pe_sub[0] * c->pe_sub[1] * c->pe_sub[2]);
+        stats->fp64_fallbacks = cnt[0];
+        stats->overflow_waves = cnt[1];
+    }
+    if (cnt[1])
+        return fail(c, SMG_ERR_CAPACITY,
+                    "%llu waves exceeded the candidate capacity (%d bars, %d holes)", cnt[1],
+                    kCapBars, kCapHoles);
+    return SMG_OK;
+}
+
+smg_status smg_pe_project(smg_ctx* c, int rotation, smg_precision prec, const float* x, int x_is_device,
+                          float* y, int y_is_device, smg_stats* stats) {
+    return project_entry(c, rotation, prec, false, x, x_is_device, y, y_is_device, stats);
+}
+
+smg_status smg_pe_backproject(smg_ctx* c, int rotation, smg_precision prec, const float* y,
+                              int y_is_device, float* x, int x_is_device, smg_stats* stats) {
+    return project_entry(c, rotation, prec, true, y, y_is_device, x, x_is_device, stats);
 }
 
 int64_t smg_bitmap_crystal_words(int n_det) {

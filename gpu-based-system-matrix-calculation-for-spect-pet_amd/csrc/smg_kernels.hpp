@@ -30,6 +30,15 @@ struct PeLaunch {
     int cells_bar0;               // nullptr: the kernel forms each sub-cell itself
 };
 
+// Matrix-free projections with the PE matrix of A's rotation: y = A x
+// (x: V floats, y: N floats) and x = A^T y.  partial: project_scratch(A, back)
+// doubles.
+size_t project_scratch(const PeLaunch& a, bool back);
+hipError_t launch_pe_project(const PeLaunch& a, bool exact, const float* x, double* partial, float* y,
+                             hipStream_t stream);
+hipError_t launch_pe_backproject(const PeLaunch& a, bool exact, const float* y, double* partial, float* x,
+                                 hipStream_t stream);
+
 // Sub-cell table of bars bar0 .. bar0 + n_bars - 1 for sub-cells sub[3], in
 // the PE kernels' cell order: the same values those kernels would compute.
 hipError_t launch_cell_table(const PeGeom& geom, int bar0, int n_bars, const int sub[3], CellCtx* out,

@@ -602,6 +602,156 @@ __global__ __launch_bounds__(256) void pe_grad_reduce(PeGradScratch S, int n_det
     grad[i] = s;
 }
 
+// ---- matrix-free projections -----------------------------------------------------------
+//
+// (No counterpart in the reference, which writes the matrix.)  The element
+// A[d][v] of one lane, formed as in pe_forward_kernel (same sub-cell order,
+// same FP64 redo of near rays), is used at once instead of being stored:
+// forward projection y[d] = sum_v A[d][v] x[v], back-projection
+// x[v] = sum_d A[d][v] y[d].  Sums are FP64 in a fixed order.
+
+// A[d][v] of this lane for bar d and the wave's tile (W set up for them).
+// Returns 0 for a bar below the mu threshold and NaN (and counts the wave) on
+// a candidate-list overflow; adds the lane's FP64 re-computations to fallbacks.
+template <bool kExact>
+__device__ double pe_element(const PeLaunch& A, int d, int lane, const WaveSetup& W, WaveLists& L,
+                             uint16_t* redo_cell, uint64_t* redo_mask, unsigned& fallbacks) {
+    const PeGeom& G = A.geom;
+    const Bar<double>& bd = G.bars_d[d];
+    if (!(bd.mut > (double)0.01f)) return 0.0;  // PESysMatGen.cu:1116, :1194
+    int nb, nh;
+    __syncthreads();  // the lists of the previous element are no longer read
+    if (!build_candidates<kExact>(A, d, lane, W, L, nb, nh)) {
+        if (lane == 0) atomicAdd(&A.counters[1], 1ull);
+        return __builtin_nan("");
+    }
+    __syncthreads();
+    using T = typename std::conditional<kExact, double, float>::type;
+    KahanSum<T> acc;
+    const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
+    int cell = 0;
+    while (cell < n_cells) {
+        int n_redo = 0;
+        for (; cell < n_cells && n_redo < kRedoCap; ++cell) {
+            CellCtx cc;
+            tile_cell(A, bd, d, cell, W, lane, cc);
+            int nsb, nsh;
+            __syncthreads();
+            narrow_candidates(A, cc.q, W, lane, L, nb, nh, nsb, nsh);
+            __syncthreads();
+            bool near = false;
+            if (W.valid) {
+                LdsCands C{&L, nsb, nsh, lane};
+                T val = pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near);
+                if (!near) acc.add(val);
+            }
+            if constexpr (!kExact) {
+                const uint64_t m = __ballot(near);
+                if (m) {
+                    if (lane == 0) {
+                        redo_cell[n_redo] = (uint16_t)cell;
+                        redo_mask[n_redo] = m;
+                    }
+                    ++n_redo;
+                }
+            }
+        }
+        if constexpr (!kExact) {
+            for (int i = 0; i < n_redo; ++i) {
+                CellCtx cc;
+                tile_cell(A, bd, d, redo_cell[i], W, lane, cc);
+                int rsb, rsh;
+                __syncthreads();
+                narrow_candidates(A, cc.q, W, lane, L, nb, nh, rsb, rsh);
+                __syncthreads();
+                if ((redo_mask[i] >> lane) & 1ull) {
+                    LdsCands C{&L, rsb, rsh, lane};
+                    bool n2 = false;
+                    acc.add((T)pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2));
+                    ++fallbacks;
+                }
+            }
+        }
+    }
+    // the element as the matrix kernel stores it (FP32), so that projecting
+    // equals multiplying by the stored matrix
+    return (double)float(acc.value());
+}
+
+// Forward projection: one wave per (bar d, voxel tile); the wave's share of
+// y[d] goes to partial[d][tile].
+template <bool kExact>
+__global__ __launch_bounds__(64) void pe_project_kernel(PeLaunch A, const float* x, double* partial) {
+    __shared__ WaveLists L;
+    __shared__ uint16_t redo_cell[kRedoCap];
+    __shared__ uint64_t redo_mask[kRedoCap];
+    const int lane = threadIdx.x;
+    const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    const int d = (int)(blockIdx.x / n_tiles);
+    const int tile = (int)(blockIdx.x % n_tiles);
+    WaveSetup W;
+    wave_setup(A, d, tile, lane, W);
+    unsigned fallbacks = 0;
+    const double a = pe_element<kExact>(A, d, lane, W, L, redo_cell, redo_mask, fallbacks);
+    const double s = wave_sum(W.valid ? a * (double)x[W.v] : 0.0);
+    if (lane == 0) partial[blockIdx.x] = s;
+    if constexpr (!kExact) {
+        unsigned tot = fallbacks;
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+        if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
+    }
+}
+
+// Back-projection: one wave per (group of `group` consecutive bars, voxel
+// tile), looping over the group's bars (those with y[d] = 0 skipped); each
+// lane's share of x[v] goes to partial[group][v].
+template <bool kExact>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_backproject_kernel(
+    PeLaunch A, const float* y, int group, double* partial) {
+    __shared__ WaveLists L;
+    __shared__ uint16_t redo_cell[kRedoCap];
+    __shared__ uint64_t redo_mask[kRedoCap];
+    const GeomDesc& g = A.geom.g;
+    const int lane = threadIdx.x;
+    const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    const int grp = (int)(blockIdx.x / n_tiles);
+    const int tile = (int)(blockIdx.x % n_tiles);
+    const int d0 = grp * group, d1 = min(g.n_det, d0 + group);
+    unsigned fallbacks = 0;
+    double acc = 0.0;
+    for (int d = d0; d < d1; ++d) {
+        const float yd = y[d];
+        if (yd == 0.0f) continue;  // wave-uniform
+        WaveSetup W;
+        wave_setup(A, d, tile, lane, W);
+        const double a = pe_element<kExact>(A, d, lane, W, L, redo_cell, redo_mask, fallbacks);
+        acc += a * (double)yd;
+    }
+    // the lane's voxel (it does not depend on the bar)
+    const int t0 = tile % A.tiles[0], t1 = (tile / A.tiles[0]) % A.tiles[1],
+              t2 = tile / (A.tiles[0] * A.tiles[1]);
+    const int ix = t0 * A.tile[0] + lane % A.tile[0];
+    const int iy = t1 * A.tile[1] + (lane / A.tile[0]) % A.tile[1];
+    const int iz = t2 * A.tile[2] + lane / (A.tile[0] * A.tile[1]);
+    if (ix < g.nx && iy < g.ny && iz < g.nz)
+        partial[(int64_t)grp * g.n_vox + ix + (int64_t)iy * g.nx + (int64_t)iz * g.nx * g.ny] = acc;
+    if constexpr (!kExact) {
+        unsigned tot = fallbacks;
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+        if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
+    }
+}
+
+// out[i] = sum_k partial[k * stride + i * step] for k < n, in k order.
+__global__ __launch_bounds__(256) void sum_rows_kernel(const double* partial, int64_t n_out, int n,
+                                                       int64_t stride, int64_t step, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    KahanSum<double> s;
+    for (int k = 0; k < n; ++k) s.add(partial[(int64_t)k * stride + i * step]);
+    out[i] = float(s.value());
+}
+
 }  // namespace
 
 hipError_t launch_pe_backward(const PeLaunch& A, const PeGradScratch& S, bool exact, double* grad,
@@ -680,6 +830,60 @@ hipError_t launch_pe_points(const PeLaunch& A, bool exact, int bar0, int n_bars,
         hipLaunchKernelGGL((pe_forward_kernel<true, true>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
     else
         hipLaunchKernelGGL((pe_forward_kernel<false, true>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
+    return hipGetLastError();
+}
+
+int backproject_group(const PeLaunch& A) {
+    const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    const int64_t waves = (int64_t)A.geom.g.n_det * n_tiles;
+    // enough waves to fill the GPU (~32768), at most 64 bars per wave
+    return (int)std::min<int64_t>(64, std::max<int64_t>(1, waves / 32768));
+}
+
+size_t project_scratch(const PeLaunch& A, bool back) {
+    const GeomDesc& g = A.geom.g;
+    const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    if (!back) return (size_t)g.n_det * (size_t)n_tiles;
+    const int grp = backproject_group(A);
+    return (size_t)((g.n_det + grp - 1) / grp) * (size_t)g.n_vox;
+}
+
+hipError_t launch_pe_project(const PeLaunch& A, bool exact, const float* x, double* partial, float* y,
+                             hipStream_t stream) {
+    const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    const int64_t blocks = (int64_t)A.geom.g.n_det * n_tiles;
+    if (blocks <= 0) return hipSuccess;
+    if (exact)
+        hipLaunchKernelGGL(pe_project_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A, x, partial);
+    else
+        hipLaunchKernelGGL(pe_project_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A, x, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t n = A.geom.g.n_det;  // y[d] = sum over tiles of partial[d * n_tiles + t]
+    hipLaunchKernelGGL(sum_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, partial,
+                       n, n_tiles, (int64_t)1, (int64_t)n_tiles, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_pe_backproject(const PeLaunch& A, bool exact, const float* y, double* partial, float* x,
+                                 hipStream_t stream) {
+    const GeomDesc& g = A.geom.g;
+    const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    const int grp = backproject_group(A);
+    const int n_grp = (g.n_det + grp - 1) / grp;
+    const int64_t blocks = (int64_t)n_grp * n_tiles;
+    if (blocks <= 0) return hipSuccess;
+    if (exact)
+        hipLaunchKernelGGL(pe_backproject_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A, y, grp,
+                           partial);
+    else
+        hipLaunchKernelGGL(pe_backproject_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A, y, grp,
+                           partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t n = g.n_vox;  // x[v] = sum over groups of partial[k * V + v]
+    hipLaunchKernelGGL(sum_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, partial,
+                       n, n_grp, n, (int64_t)1, x);
     return hipGetLastError();
 }
 

@@ -33,6 +33,7 @@ EXPORTS = (
     "smg_device_count", "smg_create", "smg_destroy", "smg_last_error", "smg_status_string",
     "smg_stream", "smg_synchronize", "smg_set_geometry", "smg_get_dims",
     "smg_set_pe_subsamples", "smg_set_scatter_subsamples", "smg_pe_forward", "smg_pe_backward",
+    "smg_pe_project", "smg_pe_backproject",
     "smg_bitmap_crystal_words", "smg_bitmap_collimator_words", "smg_bitmap_crystal",
     "smg_bitmap_collimator", "smg_load_bitmap_crystal", "smg_load_bitmap_collimator",
     "smg_scatter_crystal", "smg_scatter_collimator", "smg_set_mu_tables",
@@ -95,6 +96,8 @@ def lib():
         "smg_set_mu_tables": (i32, [vp, i32, dp, dp, dp, dp, dp]),
         "smg_pe_forward": (i32, [vp, i32, i32, vp, i32, st]),
         "smg_pe_backward": (i32, [vp, i32, i32, vp, i32, dp, dp, st]),
+        "smg_pe_project": (i32, [vp, i32, i32, vp, i32, vp, i32, st]),
+        "smg_pe_backproject": (i32, [vp, i32, i32, vp, i32, vp, i32, st]),
         "smg_bitmap_crystal_words": (i64, [i32]),
         "smg_bitmap_collimator_words": (i64, [i32, i32]),
         "smg_bitmap_crystal": (i32, [vp, vp, i64, i32]),
@@ -244,6 +247,28 @@ class Context:
                                             grad.ctypes.data_as(C.POINTER(C.c_double)),
                                             C.byref(loss), C.byref(st)))
         return loss.value, grad, st.as_dict()
+
+    def pe_project(self, x: np.ndarray, rotation: int = 0,
+                   precision: str = "fp32") -> tuple[np.ndarray, dict]:
+        """y = A x with the PE matrix of `rotation`, matrix-free (x: V values)."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(self.dims.n_vox)
+        y = np.empty(self.dims.n_det, np.float32)
+        st = Stats()
+        self._check(self._L.smg_pe_project(self._h, rotation, PREC[precision],
+                                           x.ctypes.data_as(C.c_void_p), 0,
+                                           y.ctypes.data_as(C.c_void_p), 0, C.byref(st)))
+        return y, st.as_dict()
+
+    def pe_backproject(self, y: np.ndarray, rotation: int = 0,
+                       precision: str = "fp32") -> tuple[np.ndarray, dict]:
+        """x = A^T y with the PE matrix of `rotation`, matrix-free (y: N values)."""
+        y = np.ascontiguousarray(y, dtype=np.float32).reshape(self.dims.n_det)
+        x = np.empty(self.dims.n_vox, np.float32)
+        st = Stats()
+        self._check(self._L.smg_pe_backproject(self._h, rotation, PREC[precision],
+                                               y.ctypes.data_as(C.c_void_p), 0,
+                                               x.ctypes.data_as(C.c_void_p), 0, C.byref(st)))
+        return x, st.as_dict()
 
     # ---- bitmaps -----------------------------------------------------------------
     def bitmap_crystal(self) -> np.ndarray:

@@ -122,6 +122,19 @@ smg_status smg_pe_forward(smg_ctx* ctx, int rotation, smg_precision prec, float*
 smg_status smg_pe_backward(smg_ctx* ctx, int rotation, smg_precision prec, const float* w,
                            int w_is_device, double* grad, double* loss, smg_stats* stats);
 
+/* Matrix-free projections with the PE matrix of rotation `rotation`: each
+ * element is computed as smg_pe_forward computes it (FP32 value) and used at
+ * once, never stored.  Forward projection y[d] = sum_v A[d][v] x[v] (x: V
+ * floats, y: N floats); back-projection x[v] = sum_d A[d][v] y[d] (bars with
+ * y[d] = 0 are skipped).  FP64 sums in a fixed order (bitwise reproducible).
+ * No counterpart in the reference, which writes the matrix to a file for a
+ * reconstruction program to read (PEGen_RayTracing_CircularHole/
+ * PEGen_CircularHole.cpp:141-147). */
+smg_status smg_pe_project(smg_ctx* ctx, int rotation, smg_precision prec, const float* x,
+                          int x_is_device, float* y, int y_is_device, smg_stats* stats);
+smg_status smg_pe_backproject(smg_ctx* ctx, int rotation, smg_precision prec, const float* y,
+                              int y_is_device, float* x, int x_is_device, smg_stats* stats);
+
 /* ---- geometry bitmaps (the scatter kernels' attenuation culling) ------------
  * Crystal bitmap: N(N+1)/2 * N bits; collimator bitmap: H0 * N * N bits,
  * packed little-endian in 32-bit words, the reference's cache-file format
