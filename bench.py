@@ -13,7 +13,8 @@ all-FP64 time of the same step, the primary-Compton scatter throughputs with
 the reference's scatter points (crystal scatter on 32x32x1 voxels x 2048 bars,
 collimator scatter on 32x32x16 voxels x 64 bars), with scatter points on a
 0.5 mm grid over the collimator tungsten and on a 0.25 mm grid in every bar
-(8 detector rows of 2048 bars), and the forward and adjoint (dL/dmu) times on
+(8 detector rows of 2048 bars), the matrix-free forward and back-projection
+times with the headline matrix, and the forward and adjoint (dL/dmu) times on
 the 32x32x1-voxel, 64-bar geometry.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision fp32|fp64]
@@ -110,6 +111,20 @@ def extras(smg, params, pe_ctx, out, precision):
     ex[f"pe_{other}_kernel_ms"] = round(ms_other, 2)
     ex["pe_fp64_over_fp32_time"] = round((ms_other / ms_this) if other == "fp64" else (ms_this / ms_other), 3)
     log(f"PE {other}: {ms_other:.1f} ms")
+
+    # matrix-free projections with the same matrix (elements formed in-kernel, never stored)
+    rng = np.random.default_rng(4)
+    xv = rng.uniform(0, 1, pe_ctx.dims.n_vox).astype(np.float32)
+    yv = rng.uniform(0, 1, pe_ctx.dims.n_det).astype(np.float32)
+    pe_ctx.pe_project(xv, 0, precision)
+    f_ms = timed(lambda: pe_ctx.pe_project(xv, 0, precision), 1)
+    b_ms = timed(lambda: pe_ctx.pe_backproject(yv, 0, precision), 1)
+    ex["projection"] = {"forward_kernel_ms": round(f_ms, 2), "back_kernel_ms": round(b_ms, 2),
+                        "matrix_kernel_ms": round(ms_this, 2),
+                        "elements_per_s_forward": round(pe_ctx.dims.n_det * pe_ctx.dims.n_vox / f_ms * 1e3, 1),
+                        "workload": "y = A x and x = A^T y with the full-size PE matrix, matrix-free "
+                                    "(no 1 GiB matrix in HBM or on disk)"}
+    log(f"projection: forward {f_ms:.1f} ms, back {b_ms:.1f} ms (matrix {ms_this:.1f} ms)")
 
     # crystal scatter (ScatterGen_Crystal path): 32x32x1 voxels, 2048 bars, 2047 scatter points
     p4 = params.config(4)
