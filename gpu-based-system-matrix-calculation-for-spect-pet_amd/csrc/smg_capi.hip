@@ -760,7 +760,7 @@ smg_status smg_scatter_collimator(smg_ctx* c, int rotation, smg_coll_variant var
     return scatter_entry(c, true, rotation, (int)variant, prec, nullptr, 0, dst, dst_is_device, stats);
 }
 
-smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm,
+smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm, unsigned layer_mask,
                                        smg_precision prec, float* dst, int dst_is_device,
                                        int64_t* n_points, smg_stats* stats) {
     double t_start = now_ms();
@@ -771,10 +771,12 @@ smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm
     const HostGeom& hg = c->hg;
     const GeomDesc& g = hg.g;
     if (g.n_layers < 1) return fail(c, SMG_ERR_BAD_PARAM, "the collimator has no layer");
+    if (layer_mask == 0 || (g.n_layers < 32 && (layer_mask >> g.n_layers) != 0))
+        return fail(c, SMG_ERR_BAD_PARAM, "layer mask 0x%x does not select layers of the %d-layer collimator",
+                    layer_mask, g.n_layers);
     const size_t n_el = (size_t)g.n_det * (size_t)g.n_vox;
     std::vector<ScatterPt<double>> pts;
-    double dv = 0.0;
-    make_coll_grid(hg, pitch_mm, pts, dv);
+    make_coll_grid(hg, pitch_mm, layer_mask, pts);
     if (pts.size() > (size_t)INT32_MAX)
         return fail(c, SMG_ERR_CAPACITY, "%zu grid points", pts.size());
     if (n_points) *n_points = (int64_t)pts.size();
@@ -823,7 +825,6 @@ smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm
                                   c->stream));
         A.tables = c->mu_tab_d.ptr;
     }
-    const double muc0 = hg.layers_d[0].muc;
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
     if (pts.empty()) {
@@ -833,7 +834,7 @@ smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm
         A.pts_d = c->pts_grid.ptr + s0;
         A.n_pts = (int)std::min<int64_t>(chunk, (int64_t)pts.size() - s0);
         A.accumulate = s0 > 0;
-        SMG_HIP(c, launch_first_leg(A, dv, muc0, c->w1.ptr, c->stream));
+        SMG_HIP(c, launch_first_leg(A, c->w1.ptr, c->stream));
         SMG_HIP(c, launch_pair_terms(A, c->stream));
         SMG_HIP(c, scatter_partials(c, A));
         SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));

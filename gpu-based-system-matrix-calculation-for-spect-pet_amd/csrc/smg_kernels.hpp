@@ -102,10 +102,9 @@ struct ScatterLaunch {
                                      // are split over several waves per element
 };
 
-// Grid scatterers: first-leg weights w1[s][v] = dv mu_c0 / (4 pi L^2) exp(-att)
-// of the chunk's scatter points.
-hipError_t launch_first_leg(const ScatterLaunch& a, double dv, double muc0, float* w1,
-                            hipStream_t stream);
+// Grid scatterers: first-leg weights w1[s][v] = w_s / (4 pi L^2) exp(-att) of
+// the chunk's scatter points (w_s = dv mu_c of the point's layer, in its dy).
+hipError_t launch_first_leg(const ScatterLaunch& a, float* w1, hipStream_t stream);
 
 hipError_t launch_pair_terms(const ScatterLaunch& a, hipStream_t stream);
 hipError_t launch_scatter(const ScatterLaunch& a, bool exact, hipStream_t stream);

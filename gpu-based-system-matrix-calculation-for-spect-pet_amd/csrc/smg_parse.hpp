@@ -86,42 +86,45 @@ inline void make_group_boxes(HostGeom& hg) {
     }, hg.hole_groups);
 }
 
-// Scatter points on a grid over the layer-0 plate (the collimator
-// grid-scatterer mode, an extension with no reference counterpart):
-// n_k = max(1, round(extent / pitch)) cells per axis, points at the cell
-// centres, z outermost and x innermost, without the points inside a layer-0
-// hole; dv = the cell volume.
-inline void make_coll_grid(const HostGeom& hg, double pitch, std::vector<ScatterPt<double>>& pts,
-                           double& dv) {
+// Scatter points on a grid over the plates of the layers in layer_mask (bit
+// m: layer m; the collimator grid-scatterer mode, an extension with no
+// reference counterpart): per layer, n_k = max(1, round(extent / pitch))
+// cells per axis, points at the cell centres, z outermost and x innermost,
+// without the points inside a hole of that layer; layers in increasing
+// order.  Each point's dy holds its weight factor, the cell volume times the
+// layer's Compton mu.
+inline void make_coll_grid(const HostGeom& hg, double pitch, unsigned layer_mask,
+                           std::vector<ScatterPt<double>>& pts) {
     pts.clear();
-    dv = 0.0;
-    if (hg.g.n_layers < 1 || !(pitch > 0.0)) return;
-    const Layer<double>& L = hg.layers_d[0];
-    int n[3];
-    double pk[3];
-    // extents as the parameters give them: width, thickness, height
-    const double exts[3] = {L.hi[0] - L.lo[0], L.thick, L.hi[2] - L.lo[2]};
-    for (int k = 0; k < 3; ++k) {
-        const double ext = exts[k];
-        n[k] = (int)std::floor(ext / pitch + 0.5);
-        if (n[k] < 1) n[k] = 1;
-        pk[k] = ext / n[k];
-    }
-    dv = pk[0] * pk[1] * pk[2];
-    const int nh = hg.g.ranges[0].count;
-    for (int kz = 0; kz < n[2]; ++kz)
-        for (int jy = 0; jy < n[1]; ++jy)
-            for (int ix = 0; ix < n[0]; ++ix) {
-                const double x = L.lo[0] + (ix + 0.5) * pk[0], y = L.lo[1] + (jy + 0.5) * pk[1],
-                             z = L.lo[2] + (kz + 0.5) * pk[2];
-                bool inside = false;
-                for (int h = 0; h < nh && !inside; ++h) {
-                    const Hole<double>& o = hg.holes_d[h];
-                    const double dx = x - o.xc, dz = z - o.zc;
-                    inside = dx * dx + dz * dz < o.r * o.r && y >= o.y1 && y <= o.y2;
+    if (!(pitch > 0.0)) return;
+    for (int m = 0; m < hg.g.n_layers; ++m) {
+        if (!((layer_mask >> m) & 1u)) continue;
+        const Layer<double>& L = hg.layers_d[m];
+        int n[3];
+        double pk[3];
+        // extents as the parameters give them: width, thickness, height
+        const double exts[3] = {L.hi[0] - L.lo[0], L.thick, L.hi[2] - L.lo[2]};
+        for (int k = 0; k < 3; ++k) {
+            n[k] = (int)std::floor(exts[k] / pitch + 0.5);
+            if (n[k] < 1) n[k] = 1;
+            pk[k] = exts[k] / n[k];
+        }
+        const double w = pk[0] * pk[1] * pk[2] * L.muc;
+        const int h0 = hg.g.ranges[m].first, h1 = h0 + hg.g.ranges[m].count;
+        for (int kz = 0; kz < n[2]; ++kz)
+            for (int jy = 0; jy < n[1]; ++jy)
+                for (int ix = 0; ix < n[0]; ++ix) {
+                    const double x = L.lo[0] + (ix + 0.5) * pk[0], y = L.lo[1] + (jy + 0.5) * pk[1],
+                                 z = L.lo[2] + (kz + 0.5) * pk[2];
+                    bool inside = false;
+                    for (int h = h0; h < h1 && !inside; ++h) {
+                        const Hole<double>& o = hg.holes_d[h];
+                        const double dx = x - o.xc, dz = z - o.zc;
+                        inside = dx * dx + dz * dz < o.r * o.r && y >= o.y1 && y <= o.y2;
+                    }
+                    if (!inside) pts.push_back(ScatterPt<double>{x, y, z, w});
                 }
-                if (!inside) pts.push_back(ScatterPt<double>{x, y, z, 0.0});
-            }
+    }
 }
 
 // Parse the four parameter arrays the way the reference reads them.

@@ -295,10 +295,10 @@ __global__ __launch_bounds__(256) void scatter_reduce_kernel(ScatterLaunch A, Sl
 
 // Grid scatterers (an extension with no reference counterpart): one thread
 // per (scatter point s, voxel v) of the chunk, the chance of a Compton
-// interaction in the cell of volume dv around s for a photon from v,
-// dv mu_c0 / (4 pi |s - v|^2) exp(-collimator material along v -> s), in FP64.
-__global__ __launch_bounds__(256) void first_leg_kernel(ScatterLaunch A, double dv, double muc0,
-                                                        float* w1) {
+// interaction in the cell around s for a photon from v,
+// w_s / (4 pi |s - v|^2) exp(-collimator material along v -> s), in FP64,
+// with w_s = dv mu_c of the point's layer (held in the point's dy).
+__global__ __launch_bounds__(256) void first_leg_kernel(ScatterLaunch A, float* w1) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const GeomDesc& g = A.geom.g;
     const int64_t total = (int64_t)A.n_pts * g.n_vox;
@@ -315,17 +315,16 @@ __global__ __launch_bounds__(256) void first_leg_kernel(ScatterLaunch A, double 
     coll_material(A.geom, p, q, att_t, s_pe, s_c);
     const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
     const double L2 = dx * dx + dy * dy + dz * dz;
-    w1[tid] = float(dv * muc0 / (4.0 * 3.14159265358979323846 * L2) * exp(-att_t));
+    w1[tid] = float(sp.dy / (4.0 * 3.14159265358979323846 * L2) * exp(-att_t));
 }
 
 }  // namespace
 
-hipError_t launch_first_leg(const ScatterLaunch& A, double dv, double muc0, float* w1,
-                            hipStream_t stream) {
+hipError_t launch_first_leg(const ScatterLaunch& A, float* w1, hipStream_t stream) {
     const int64_t total = (int64_t)A.n_pts * A.geom.g.n_vox;
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(first_leg_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
-                       A, dv, muc0, w1);
+                       A, w1);
     return hipGetLastError();
 }
 

@@ -106,7 +106,7 @@ def lib():
         "smg_load_bitmap_collimator": (i32, [vp, up, i64]),
         "smg_scatter_crystal": (i32, [vp, i32, i32, vp, i32, vp, i32, st]),
         "smg_scatter_collimator": (i32, [vp, i32, i32, i32, vp, i32, st]),
-        "smg_scatter_collimator_grid": (i32, [vp, i32, C.c_double, i32, vp, i32,
+        "smg_scatter_collimator_grid": (i32, [vp, i32, C.c_double, C.c_uint, i32, vp, i32,
                                               C.POINTER(C.c_int64), st]),
         "smg_scatter_crystal_grid": (i32, [vp, i32, i32, i32, i32, vp, i32, i32, vp, i32, st]),
     }
@@ -314,14 +314,15 @@ class Context:
         return out, st.as_dict()
 
     def scatter_collimator_grid(self, pitch_mm: float, rotation: int = 0,
-                                precision: str = "fp32"):
-        """Collimator scatter from grid points over the layer-0 tungsten.
+                                precision: str = "fp32", layers: int = 1):
+        """Collimator scatter from grid points over the tungsten of the layers
+        in the bit mask `layers` (1: the layer-0 plate).
         Returns (matrix [N][V], stats, number of grid points)."""
         out = np.empty((self.dims.n_det, self.dims.n_vox), np.float32)
         st = Stats()
         npts = C.c_int64(0)
         self._check(self._L.smg_scatter_collimator_grid(self._h, rotation, float(pitch_mm),
-                                                        PREC[precision],
+                                                        int(layers), PREC[precision],
                                                         out.ctypes.data_as(C.c_void_p), 0,
                                                         C.byref(npts), C.byref(st)))
         return out, st.as_dict(), npts.value

@@ -162,15 +162,17 @@ smg_status smg_scatter_collimator(smg_ctx* ctx, int rotation, smg_coll_variant v
                                   smg_precision prec, float* dst, int dst_is_device,
                                   smg_stats* stats);
 
-/* Collimator scatter with scatter points on a grid over the layer-0 tungsten
- * (pitch_mm, cells of volume dv; points inside layer-0 holes left out).  No
- * counterpart in the reference, whose collimator scatterers are the hole
- * centres (ScatterGen_RayTracing_CircularHole/scatter.cu:1702-1716).  Each
- * point weighs dv mu_c0 / (4 pi L^2) exp(-collimator material on the way
- * from the voxel); the second leg passes every other bar and the collimator
- * material at E' (mu(E) tables or power laws); the rest is the reference's
- * scatter term.  n_points (optional) receives the number of grid points. */
-smg_status smg_scatter_collimator_grid(smg_ctx* ctx, int rotation, double pitch_mm,
+/* Collimator scatter with scatter points on a grid over the tungsten of the
+ * layers in layer_mask (bit m: layer m; 1 = the layer-0 plate), pitch_mm,
+ * cells of volume dv; points inside a hole of their layer left out.  No
+ * counterpart in the reference, whose collimator scatterers are the layer-0
+ * hole centres (ScatterGen_RayTracing_CircularHole/scatter.cu:1702-1716).
+ * Each point weighs dv mu_c / (4 pi L^2) exp(-collimator material on the way
+ * from the voxel), mu_c of its layer; the second leg passes every other bar
+ * and the collimator material at E' (mu(E) tables or power laws); the rest
+ * is the reference's scatter term.  n_points (optional) receives the number
+ * of grid points. */
+smg_status smg_scatter_collimator_grid(smg_ctx* ctx, int rotation, double pitch_mm, unsigned layer_mask,
                                        smg_precision prec, float* dst, int dst_is_device,
                                        int64_t* n_points, smg_stats* stats);
 

@@ -90,12 +90,11 @@ def lib():
     L.smg_o_seg_cyl.restype = C.c_double
     L.smg_o_seg_cyl.argtypes = [_dp, _dp] + [C.c_double] * 5
     L.smg_o_coll_grid_points.restype = C.c_int64
-    L.smg_o_coll_grid_points.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int64,
-                                         C.POINTER(C.c_double)]
+    L.smg_o_coll_grid_points.argtypes = [C.c_void_p, C.c_double, C.c_uint, C.c_void_p, C.c_int64]
     L.smg_o_coll_grid_weight.restype = C.c_double
     L.smg_o_coll_grid_weight.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
-    L.smg_o_coll_grid_scatter_pairs.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
-                                                C.c_int, _ip, _ip, C.c_int, _dp, C.c_int]
+    L.smg_o_coll_grid_scatter_pairs.argtypes = [C.c_void_p, C.c_double, C.c_uint, C.c_int, C.c_int,
+                                                C.c_int, C.c_int, _ip, _ip, C.c_int, _dp, C.c_int]
     L.smg_o_crystal_grid_weight.restype = C.c_double
     L.smg_o_crystal_grid_weight.argtypes = [C.c_void_p] + [C.c_int] * 7
     L.smg_o_crystal_grid_scatter_pairs.argtypes = [C.c_void_p, _up] + [C.c_int] * 7 + [
@@ -260,22 +259,24 @@ class Oracle:
     def mu_ratio(self, which, e, e0) -> float:
         return lib().smg_o_mu_ratio(self._g, which, e, e0)
 
-    def coll_grid_points(self, pitch):
-        """(points [n][3], cell volume) of the layer-0 tungsten grid."""
-        dv = C.c_double(0.0)
-        n = lib().smg_o_coll_grid_points(self._g, pitch, None, 0, C.byref(dv))
-        pts = np.zeros((max(n, 1), 3))
-        lib().smg_o_coll_grid_points(self._g, pitch, pts.ctypes.data_as(C.c_void_p), n, C.byref(dv))
-        return pts[:n], dv.value
+    def coll_grid_points(self, pitch, layers=1):
+        """(points [n][3], weights [n]) of the grid over the plates of the
+        layers in the bit mask `layers`; weight = cell volume x the layer's
+        Compton mu."""
+        n = lib().smg_o_coll_grid_points(self._g, pitch, layers, None, 0)
+        pts = np.zeros((max(n, 1), 4))
+        lib().smg_o_coll_grid_points(self._g, pitch, layers, pts.ctypes.data_as(C.c_void_p), n)
+        return pts[:n, :3], pts[:n, 3]
 
-    def coll_grid_weight(self, p, s, dv) -> float:
-        return lib().smg_o_coll_grid_weight(self._g, _v3(p), _v3(s), dv)
+    def coll_grid_weight(self, p, s, w) -> float:
+        return lib().smg_o_coll_grid_weight(self._g, _v3(p), _v3(s), w)
 
-    def coll_grid_scatter_pairs(self, pitch, d, v, rot=0, sub=(1, 1, 1), nthreads=0):
+    def coll_grid_scatter_pairs(self, pitch, d, v, rot=0, sub=(1, 1, 1), nthreads=0, layers=1):
         d = np.ascontiguousarray(d, np.int32)
         v = np.ascontiguousarray(v, np.int32)
         out = np.zeros(len(d))
-        lib().smg_o_coll_grid_scatter_pairs(self._g, pitch, rot, *sub, d, v, len(d), out, nthreads)
+        lib().smg_o_coll_grid_scatter_pairs(self._g, pitch, layers, rot, *sub, d, v, len(d), out,
+                                            nthreads)
         return out
 
     def crystal_grid_weight(self, b, v, grid, cell, rot=0) -> float:

@@ -958,57 +958,63 @@ static void coll_material(const smg_o_geom* g, const double a[3], const double b
     }
 }
 
-/* Scatter points on a grid over the layer-0 plate: n_k = max(1, round(extent
- * / pitch)) cells per axis, points at the cell centres, z outermost and x
- * innermost, without the points inside a layer-0 hole.  Writes up to max
- * points (x, y, z) when xyz is given and the cell volume to *dv; returns the
- * number of points. */
-int64_t smg_o_coll_grid_points(const smg_o_geom* g, double pitch, double* xyz, int64_t max,
-                               double* dv) {
-    if (g->n_layers < 1 || !(pitch > 0.0)) return 0;
-    const float* L = g->coll + 10;
-    double lo[3] = {-(double)L[1] / 2.0, -(double)L[2] / 2.0 + g->F + L[4], -(double)L[3] / 2.0};
-    double ext[3] = {L[1], L[2], L[3]};
-    int n[3];
-    double pk[3];
-    for (int k = 0; k < 3; ++k) {
-        n[k] = (int)floor(ext[k] / pitch + 0.5);
-        if (n[k] < 1) n[k] = 1;
-        pk[k] = ext[k] / n[k];
-    }
-    if (dv) *dv = pk[0] * pk[1] * pk[2];
+/* Scatter points on a grid over the plates of the layers in layer_mask (bit
+ * m: layer m): per layer, n_k = max(1, round(extent / pitch)) cells per axis,
+ * points at the cell centres, z outermost and x innermost, without the points
+ * inside a hole of that layer; layers in increasing order.  Writes up to max
+ * points (x, y, z, w) when xyzw is given, w = cell volume x the layer's
+ * Compton mu; returns the number of points. */
+int64_t smg_o_coll_grid_points(const smg_o_geom* g, double pitch, unsigned layer_mask, double* xyzw,
+                               int64_t max) {
+    if (!(pitch > 0.0)) return 0;
     int64_t cnt = 0;
-    for (int kz = 0; kz < n[2]; ++kz)
-        for (int jy = 0; jy < n[1]; ++jy)
-            for (int ix = 0; ix < n[0]; ++ix) {
-                double x = lo[0] + (ix + 0.5) * pk[0], y = lo[1] + (jy + 0.5) * pk[1],
-                       z = lo[2] + (kz + 0.5) * pk[2];
-                int inside = 0;
-                for (int h = 0; h < g->holes_in_layer[0] && !inside; ++h) {
-                    const float* hp = g->coll + 100 + 9 * h;
-                    double dx = x - hp[0], dz = z - hp[3];
-                    inside = dx * dx + dz * dz < (double)hp[4] * hp[4] &&
-                             y >= (double)hp[1] + g->F && y <= (double)hp[2] + g->F;
+    int first = 0;
+    for (int m = 0; m < g->n_layers; first += g->holes_in_layer[m], ++m) {
+        if (!((layer_mask >> m) & 1u)) continue;
+        const float* L = g->coll + (m + 1) * 10;
+        double lo[3] = {-(double)L[1] / 2.0, -(double)L[2] / 2.0 + g->F + L[4], -(double)L[3] / 2.0};
+        double ext[3] = {L[1], L[2], L[3]};
+        int n[3];
+        double pk[3];
+        for (int k = 0; k < 3; ++k) {
+            n[k] = (int)floor(ext[k] / pitch + 0.5);
+            if (n[k] < 1) n[k] = 1;
+            pk[k] = ext[k] / n[k];
+        }
+        const double w = pk[0] * pk[1] * pk[2] * (double)L[7];
+        for (int kz = 0; kz < n[2]; ++kz)
+            for (int jy = 0; jy < n[1]; ++jy)
+                for (int ix = 0; ix < n[0]; ++ix) {
+                    double x = lo[0] + (ix + 0.5) * pk[0], y = lo[1] + (jy + 0.5) * pk[1],
+                           z = lo[2] + (kz + 0.5) * pk[2];
+                    int inside = 0;
+                    for (int h = first; h < first + g->holes_in_layer[m] && !inside; ++h) {
+                        const float* hp = g->coll + 100 + 9 * h;
+                        double dx = x - hp[0], dz = z - hp[3];
+                        inside = dx * dx + dz * dz < (double)hp[4] * hp[4] &&
+                                 y >= (double)hp[1] + g->F && y <= (double)hp[2] + g->F;
+                    }
+                    if (inside) continue;
+                    if (xyzw && cnt < max) {
+                        xyzw[4 * cnt] = x;
+                        xyzw[4 * cnt + 1] = y;
+                        xyzw[4 * cnt + 2] = z;
+                        xyzw[4 * cnt + 3] = w;
+                    }
+                    ++cnt;
                 }
-                if (inside) continue;
-                if (xyz && cnt < max) {
-                    xyz[3 * cnt] = x;
-                    xyz[3 * cnt + 1] = y;
-                    xyz[3 * cnt + 2] = z;
-                }
-                ++cnt;
-            }
+    }
     return cnt;
 }
 
-/* First-leg weight of grid scatter point s for a source at p: the chance of
- * a Compton interaction in the cell of volume dv around s, dv mu_c0 /
- * (4 pi |s - p|^2) exp(-collimator material along p -> s). */
-double smg_o_coll_grid_weight(const smg_o_geom* g, const double p[3], const double s[3], double dv) {
+/* First-leg weight of grid scatter point s (weight w = dv mu_c of its layer)
+ * for a source at p: the chance of a Compton interaction in the cell around
+ * s, w / (4 pi |s - p|^2) exp(-collimator material along p -> s). */
+double smg_o_coll_grid_weight(const smg_o_geom* g, const double p[3], const double s[3], double w) {
     double att_t, s_pe, s_c;
     coll_material(g, p, s, &att_t, &s_pe, &s_c);
     double L = dist3(p, s);
-    return dv * (double)g->coll[17] / (4.0 * M_PI * L * L) * exp(-att_t);
+    return w / (4.0 * M_PI * L * L) * exp(-att_t);
 }
 
 typedef struct {
@@ -1193,12 +1199,12 @@ double smg_o_collimator_scatter_element(const smg_o_geom* g, const uint32_t* bit
 }
 
 /* Element of the collimator scatter matrix with scatter points on a grid
- * over the layer-0 tungsten (an extension with no reference counterpart):
- * first-leg weight smg_o_coll_grid_weight, the second leg through every
- * other bar and through the collimator material at E', the rest as the
- * reference's scatter element (SC:1226-1428). */
-double smg_o_coll_grid_scatter_element(const smg_o_geom* g, const double* pts, int64_t n_pts,
-                                       double dv, int d, int v, int rot, int sx, int sy, int sz) {
+ * over the tungsten (an extension with no reference counterpart): first-leg
+ * weight smg_o_coll_grid_weight, the second leg through every other bar and
+ * through the collimator material at E', the rest as the reference's
+ * scatter element (SC:1226-1428).  pts: (x, y, z, w) per point. */
+double smg_o_coll_grid_scatter_element(const smg_o_geom* g, const double* pts, int64_t n_pts, int d,
+                                       int v, int rot, int sx, int sy, int sz) {
     bar_t bd = get_bar(g, d);
     phys_t ph = phys_for(g, &bd);
     if (ph.compton != 1 || !(bd.mut > (double)0.01f)) return 0.0;
@@ -1206,24 +1212,23 @@ double smg_o_coll_grid_scatter_element(const smg_o_geom* g, const double* pts, i
     voxel_pos(g, v, rot, p);
     double acc = 0.0;
     for (int64_t i = 0; i < n_pts; ++i) {
-        const double* s = pts + 3 * i;
-        double w = smg_o_coll_grid_weight(g, p, s, dv);
+        const double* s = pts + 4 * i;
+        double w = smg_o_coll_grid_weight(g, p, s, s[3]);
         acc += scatter_body(g, &bd, d, p, s, w, NULL, 2, -1, sx, sy, sz, &ph);
     }
     return acc;
 }
 
-void smg_o_coll_grid_scatter_pairs(const smg_o_geom* g, double pitch, int rot, int sx, int sy,
-                                   int sz, const int* d, const int* v, int n, double* out,
-                                   int nthreads) {
-    double dv = 0.0;
-    int64_t np = smg_o_coll_grid_points(g, pitch, NULL, 0, &dv);
-    double* pts = (double*)malloc(sizeof(double) * 3 * (size_t)(np > 0 ? np : 1));
-    smg_o_coll_grid_points(g, pitch, pts, np, &dv);
+void smg_o_coll_grid_scatter_pairs(const smg_o_geom* g, double pitch, unsigned layer_mask, int rot,
+                                   int sx, int sy, int sz, const int* d, const int* v, int n,
+                                   double* out, int nthreads) {
+    int64_t np = smg_o_coll_grid_points(g, pitch, layer_mask, NULL, 0);
+    double* pts = (double*)malloc(sizeof(double) * 4 * (size_t)(np > 0 ? np : 1));
+    smg_o_coll_grid_points(g, pitch, layer_mask, pts, np);
     if (nthreads <= 0) nthreads = smg_o_max_threads();
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads)
     for (int i = 0; i < n; ++i)
-        out[i] = smg_o_coll_grid_scatter_element(g, pts, np, dv, d[i], v[i], rot, sx, sy, sz);
+        out[i] = smg_o_coll_grid_scatter_element(g, pts, np, d[i], v[i], rot, sx, sy, sz);
     free(pts);
 }
 

@@ -113,12 +113,12 @@ void smg_o_collimator_scatter_pairs(const smg_o_geom* g, const uint32_t* bitmap,
 double smg_o_seg_box(const double a[3], const double b[3], const double lo[3], const double hi[3]);
 double smg_o_seg_cyl(const double a[3], const double b[3], double xc, double y1, double y2,
                      double zc, double r);
-int64_t smg_o_coll_grid_points(const smg_o_geom* g, double pitch, double* xyz, int64_t max,
-                               double* dv);
-double smg_o_coll_grid_weight(const smg_o_geom* g, const double p[3], const double s[3], double dv);
-void smg_o_coll_grid_scatter_pairs(const smg_o_geom* g, double pitch, int rot, int sx, int sy,
-                                   int sz, const int* d, const int* v, int n, double* out,
-                                   int nthreads);
+int64_t smg_o_coll_grid_points(const smg_o_geom* g, double pitch, unsigned layer_mask, double* xyzw,
+                               int64_t max);
+double smg_o_coll_grid_weight(const smg_o_geom* g, const double p[3], const double s[3], double w);
+void smg_o_coll_grid_scatter_pairs(const smg_o_geom* g, double pitch, unsigned layer_mask, int rot,
+                                   int sx, int sy, int sz, const int* d, const int* v, int n,
+                                   double* out, int nthreads);
 
 /* ---- crystal scatter from grid points (extension, no counterpart) -------- */
 double smg_o_crystal_grid_weight(const smg_o_geom* g, int b, int v, int rot, int gx, int gy,

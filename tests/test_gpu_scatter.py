@@ -306,3 +306,27 @@ def test_crystal_grid_scatter_2048_bars(smg_mod, oracle_mod, params_mod, prec, t
     assert (ref > 0).all()
     assert_parity(got[ri, v], ref, tol,
                   f"crystal grid 2x8x2, 2048 bars, {prec} (fallbacks {st['fp64_fallbacks']})")
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
+def test_collimator_grid_scatter_three_layers(smg_mod, oracle_mod, params_mod, prec, tol):
+    """Grid scatterers in the tungsten of three plates (each with its own
+    holes and its own Compton mu), rotated bars, ragged tiles: against the
+    oracle, and the three-layer matrix equals the sum of the one-layer ones."""
+    from conftest import small_geometry
+    p = small_geometry(params_mod, beta=0.05, layers=3)
+    for m in range(3):
+        p.coll[(m + 1) * 10 + 7] = 0.35 + 0.1 * m
+    o = oracle_mod.Oracle(p)
+    d, v = _sample(o, 120, 27)
+    with smg_mod.open_context(p) as ctx:
+        got, st, n = ctx.scatter_collimator_grid(1.0, 0, prec, layers=0b111)
+        parts = [ctx.scatter_collimator_grid(1.0, 0, prec, layers=1 << m) for m in range(3)]
+        with pytest.raises(smg_mod.SmgError):
+            ctx.scatter_collimator_grid(1.0, 0, prec, layers=0b1000)
+    assert n == sum(q[2] for q in parts) == len(o.coll_grid_points(1.0, layers=0b111)[0])
+    ref = o.coll_grid_scatter_pairs(1.0, d, v, layers=0b111)
+    assert (ref > 0).mean() > 0.5
+    assert_parity(got[d, v], ref, tol, f"three-layer collimator grid scatter {prec}")
+    total = sum(q[0].astype(np.float64) for q in parts)
+    np.testing.assert_allclose(got, total, rtol=1e-5, atol=1e-7 * np.abs(total).max())

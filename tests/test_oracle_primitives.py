@@ -308,7 +308,9 @@ def test_coll_grid_points_and_weight(oracle_mod, params_mod):
     the tungsten path (here a point behind a hole-free stretch of plate)."""
     p = params_mod.config(1)
     o = oracle_mod.Oracle(p)
-    pts, dv = o.coll_grid_points(1.0)
+    pts, wts = o.coll_grid_points(1.0)
+    dv = wts[0] / float(p.coll[17])
+    assert np.all(wts == wts[0])
     w, t, h = p.coll[11], p.coll[12], p.coll[13]
     holes = sum(np.pi * p.coll[100 + 9 * i + 4] ** 2 * (p.coll[100 + 9 * i + 2] - p.coll[100 + 9 * i + 1])
                 for i in range(int(p.coll[10])))
@@ -324,7 +326,35 @@ def test_coll_grid_points_and_weight(oracle_mod, params_mod):
     depth = s[1] - y_front
     L = np.linalg.norm(s - src)
     ref = dv * p.coll[17] / (4 * np.pi * L * L) * np.exp(-p.coll[15] * depth)
-    assert o.coll_grid_weight(src, s, dv) == pytest.approx(ref, rel=1e-12)
+    assert o.coll_grid_weight(src, s, wts[0]) == pytest.approx(ref, rel=1e-12)
+
+
+def test_coll_grid_points_of_several_layers(oracle_mod, params_mod):
+    """Three plates: each layer's points tile its plate less its own holes,
+    carry that layer's dv mu_c, and a mask of several layers lists the
+    layers' points in increasing layer order."""
+    from conftest import small_geometry
+    p = small_geometry(params_mod, layers=3)
+    for m in range(3):
+        p.coll[(m + 1) * 10 + 7] = 0.35 + 0.1 * m  # a Compton mu of each layer's own
+    o = oracle_mod.Oracle(p)
+    per = [o.coll_grid_points(0.5, layers=1 << m) for m in range(3)]
+    first = 0
+    for m, (pts, wts) in enumerate(per):
+        L = p.coll[(m + 1) * 10:(m + 1) * 10 + 8]
+        nh = int(L[0])
+        holes = sum(np.pi * p.coll[100 + 9 * i + 4] ** 2 * (p.coll[100 + 9 * i + 2] - p.coll[100 + 9 * i + 1])
+                    for i in range(first, first + nh))
+        first += nh
+        n = [max(1, int(np.floor(e / 0.5 + 0.5))) for e in (L[1], L[2], L[3])]
+        dv = (L[1] / n[0]) * (L[2] / n[1]) * (L[3] / n[2])
+        assert len(pts) > 0 and np.allclose(wts, dv * L[7], rtol=1e-12, atol=0)
+        assert len(pts) * dv == pytest.approx(L[1] * L[2] * L[3] - holes, rel=0.05)
+        y0 = -L[2] / 2 + p.img[11] + L[4]
+        assert np.all((pts[:, 1] > y0) & (pts[:, 1] < y0 + L[2]))
+    allp, allw = o.coll_grid_points(0.5, layers=0b111)
+    np.testing.assert_array_equal(allp, np.concatenate([q for q, _ in per]))
+    np.testing.assert_array_equal(allw, np.concatenate([w for _, w in per]))
 
 
 def test_crystal_grid_weight_closed_form(oracle_mod, params_mod):
