@@ -482,13 +482,28 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
         if (g_lt(l, T(0.1f), T(1e-4), near)) att += 100000.0;
     }
 
-    // holes and plate material, FP64 (PESysMatGen.cu:1257-1286)
+    // holes and plate material, FP64 (PESysMatGen.cu:1257-1286).  The FP32
+    // path forms T / cos(alpha) as T |D| / D_y and the hole-ray set-up from
+    // the same two reciprocals, only when the sub-cell has hole candidates.
     CylRay cr;
-    cyl_ray(D, dd, cr);
+    double inv_dd, inv_cos;
+    if constexpr (std::is_same<T, double>::value) {
+        cyl_ray(D, dd, cr);
+        inv_dd = 1.0 / dd;
+        inv_cos = 0.0;
+    } else {
+        inv_dd = 1.0 / dd;
+        inv_cos = dd / D[1];
+        if (C.nholes() > 0) cyl_ray_from(D, inv_dd, inv_cos, cr);
+    }
     for (int m = 0; m < g.n_layers; ++m) {
         const Layer<double>& L = G.layers_d[m];
         int first = g.ranges[m].first, last = first + g.ranges[m].count;
-        double in_coll = L.thick / cosang;
+        double in_coll;
+        if constexpr (std::is_same<T, double>::value)
+            in_coll = L.thick / cosang;
+        else
+            in_coll = L.thick * inv_cos;
         double sum = 0.0;
         for (int j = 0; j < C.nholes(); ++j) {
             int h = C.hole(j);
@@ -505,7 +520,6 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
     }
 
     // other bars on the way (PESysMatGen.cu:1290-1325)
-    const double inv_dd = 1.0 / dd;
     for (int j = 0; j < C.nbars(); ++j) {
         if (!C.bar_on(j)) continue;
         int k = C.bar(j);

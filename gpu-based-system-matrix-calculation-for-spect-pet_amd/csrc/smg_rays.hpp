@@ -415,6 +415,19 @@ SMG_FN void cyl_ray(const double D[3], double len, CylRay& c) {
     c.inv_qa = 1.0 / c.qa;
 }
 
+// cyl_ray from reciprocals the caller already holds: il = 1 / len and
+// inv_ky = len / D_y (= 1 / k_y up to rounding); one division left.
+SMG_FN void cyl_ray_from(const double D[3], double il, double inv_ky, CylRay& c) {
+    c.kx = D[0] * il;
+    c.ky = D[1] * il;
+    c.kz = D[2] * il;
+    c.inv_ky = inv_ky;
+    c.sxk = c.kx * inv_ky;
+    c.szk = c.kz * inv_ky;
+    c.qa = c.kx * c.kx + c.kz * c.kz;
+    c.inv_qa = 1.0 / c.qa;
+}
+
 // Chord through the finite cylinder of radius r, axis parallel to y at
 // (xc, zc), spanning y1..y2; reference length_cylinder_ray
 // (PESysMatGen.cu:688-799).  Evaluated in FP64: rays through a hole run
