@@ -275,7 +275,7 @@ __device__ __forceinline__ float point_weight(const Bar<double>& bd, int n_cells
 }
 
 template <bool kExact, bool kPoints>
-__global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
+__device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
     __shared__ WaveLists L;
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;
@@ -393,6 +393,18 @@ __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
         if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
     }
+}
+
+template <bool kExact>
+__global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
+    pe_forward_body<kExact, false>(A);
+}
+
+// Point weights at 3 waves per SIMD (the register cap costs no spills here;
+// without it the allocation lands one register above the 3-wave limit).
+template <bool kExact>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_points_kernel(PeLaunch A) {
+    pe_forward_body<kExact, true>(A);
 }
 
 // ---- adjoint -------------------------------------------------------------------------
@@ -807,9 +819,9 @@ hipError_t launch_pe_forward(const PeLaunch& A, bool exact, hipStream_t stream) 
     B.bar0 = 0;
     B.cell_slices = 1;
     if (exact)
-        hipLaunchKernelGGL((pe_forward_kernel<true, false>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
+        hipLaunchKernelGGL(pe_forward_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, B);
     else
-        hipLaunchKernelGGL((pe_forward_kernel<false, false>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
+        hipLaunchKernelGGL(pe_forward_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, B);
     return hipGetLastError();
 }
 
@@ -827,9 +839,9 @@ hipError_t launch_pe_points(const PeLaunch& A, bool exact, int bar0, int n_bars,
     B.cell_slices = (int)std::max<int64_t>(slices, 1);
     const int64_t blocks = waves * B.cell_slices;
     if (exact)
-        hipLaunchKernelGGL((pe_forward_kernel<true, true>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
+        hipLaunchKernelGGL(pe_points_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, B);
     else
-        hipLaunchKernelGGL((pe_forward_kernel<false, true>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
+        hipLaunchKernelGGL(pe_points_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, B);
     return hipGetLastError();
 }
 
