@@ -79,6 +79,8 @@ struct smg_ctx {
     DevBuf<PairTerm> pair;
     DevBuf<double> sc_partial;   // per-slice scatter sums
     DevBuf<double> proj;         // projection partial sums
+    DevBuf<double> kn_tab;       // KN prefix table of kn_tab_e0
+    double kn_tab_e0 = -1.0;
     DevBuf<CellCtx> cell_tab;    // PE sub-cell table of every bar
     DevBuf<CellCtx> cell_tab_pts;  // sub-cell table of one chunk of crystal grid points
     int cell_tab_sub[3] = {0, 0, 0};
@@ -273,6 +275,21 @@ smg_status load_bitmap(smg_ctx* c, bool collimator, const uint32_t* words, int64
     return SMG_OK;
 }
 
+// KN prefix table for the FP32 scatter term at the launch's E0 (built on the
+// context's stream when E0 changes).
+hipError_t kn_table(smg_ctx* c, ScatterLaunch& A) {
+    if (c->kn_tab_e0 != A.e0) {
+        c->kn_tab_e0 = -1.0;
+        hipError_t e = c->kn_tab.reserve((size_t)(kKnPhase + 1) * kKnSteps);
+        if (e != hipSuccess) return e;
+        e = launch_kn_table(A.e0, c->kn_tab.ptr, c->stream);
+        if (e != hipSuccess) return e;
+        c->kn_tab_e0 = A.e0;
+    }
+    A.kn_table = c->kn_tab.ptr;
+    return hipSuccess;
+}
+
 // Partial-sum buffer of a scatter launch whose scatter points are split over
 // several waves per element.  It only grows; hipFree waits for the device.
 hipError_t scatter_partials(smg_ctx* c, ScatterLaunch& A) {
@@ -358,6 +375,7 @@ smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
     A.win_hi = ph[6];
     A.e0 = ph[7];
     A.kn_norm = kn_norm(A.e0);
+    SMG_HIP(c, kn_table(c, A));
     A.coll_ratio = (double)hg.coll[17] / (double)hg.coll[15];
     if (collimator && !(hg.coll[15] != 0.0f)) A.coll_ratio = 0.0;
     A.tables = nullptr;
@@ -471,6 +489,7 @@ smg_status smg_destroy(smg_ctx* c) {
     c->pair.release();
     c->sc_partial.release();
     c->proj.release();
+    c->kn_tab.release();
     c->cell_tab.release();
     c->cell_tab_pts.release();
     c->mu_tab_d.release();
@@ -815,6 +834,7 @@ smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm
     A.win_hi = ph[6];
     A.e0 = ph[7];
     A.kn_norm = kn_norm(A.e0);
+    SMG_HIP(c, kn_table(c, A));
     A.coll_ratio = 0.0;
     A.tables = nullptr;
     if (c->mu_tab.n > 0) {
@@ -931,6 +951,7 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
     A.win_hi = ph[6];
     A.e0 = ph[7];
     A.kn_norm = kn_norm(A.e0);
+    SMG_HIP(c, kn_table(c, A));
     A.tables = nullptr;
     if (c->mu_tab.n > 0) {
         const double le0 = std::log(A.e0);

@@ -100,7 +100,11 @@ struct ScatterLaunch {
     int pts_per_bar, pt_bar0;        // kind 3: point s lies in bar pt_bar0 + s / pts_per_bar
     double* partial;                 // [slices][rows][V] per-slice sums when the scatter points
                                      // are split over several waves per element
+    const double* kn_table;          // KN prefix table of e0 (FP32 path), or nullptr
 };
+
+// KN prefix table of e0 ((kKnPhase + 1) x kKnSteps doubles) into tab.
+hipError_t launch_kn_table(double e0, double* tab, hipStream_t stream);
 
 // Grid scatterers: first-leg weights w1[s][v] = w_s / (4 pi L^2) exp(-att) of
 // the chunk's scatter points (w_s = dv mu_c of the point's layer, in its dy).

@@ -194,6 +194,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void sc
     }
     ph.norm = A.kn_norm;
     ph.tab = A.tables;
+    ph.knt = A.kn_table;
     ph.res = bd.res;
     ph.mupe = bd.mupe;
     ph.muc = bd.muc;
@@ -280,6 +281,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void sc
     }
 }
 
+// One thread per row of the KN prefix table.
+__global__ __launch_bounds__(64) void kn_table_kernel(double alpha, double* tab) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > kKnPhase) return;
+    kn_prefix_row(alpha, j, tab + (size_t)j * kKnSteps);
+}
+
 // Sum of the slices' partial sums of one element, in slice order (a fixed
 // order, so the result does not depend on scheduling), compensated in FP64.
 __global__ __launch_bounds__(256) void scatter_reduce_kernel(ScatterLaunch A, SliceShape sh) {
@@ -325,6 +333,11 @@ hipError_t launch_first_leg(const ScatterLaunch& A, float* w1, hipStream_t strea
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(first_leg_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
                        A, w1);
+    return hipGetLastError();
+}
+
+hipError_t launch_kn_table(double e0, double* tab, hipStream_t stream) {
+    hipLaunchKernelGGL(kn_table_kernel, dim3((kKnPhase + 1 + 63) / 64), dim3(64), 0, stream, e0 / 511.0, tab);
     return hipGetLastError();
 }
 

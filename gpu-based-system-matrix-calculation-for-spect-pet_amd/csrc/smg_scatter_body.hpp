@@ -157,6 +157,7 @@ struct ScatterPhys {
     double rcell;          // bounding radius of a detection sub-cell
     double mupe, muc;      // bar d
     const MuTables* tab = nullptr;  // mu(E) tables, or power laws
+    const double* knt = nullptr;    // KN prefix table (FP32 path), or the node loop
 };
 
 // Thresholds with their reference values.
@@ -212,6 +213,56 @@ SMG_FN T kn_sum(T lo, T hi, T alpha, bool& near) {
         }
         return s.value();
     }
+}
+
+// KN rectangle sums from a prefix table.  The reference's sum has nodes
+// lo + i step, i = 0..n; with lo = phi + k0 step (0 <= phi < step) it is
+// P(phi, k0 + n) - P(phi, k0 - 1), P(phi, k) = sum_{i <= k} g(phi + i step),
+// g(t) = KN(cos t) sin t step.  The table holds P in FP64 for kKnPhase + 1
+// phases phi_j = j step / kKnPhase (rows) and k < kKnSteps nodes (columns);
+// the sum is interpolated linearly in phi between rows j and j + 1 (error
+// ~(step / kKnPhase)^2 / 8 times the sum's second derivative, ~1e-9 relative).
+constexpr int kKnPhase = 256;
+constexpr int kKnSteps = 320;  // nodes up to 3.19 rad; polar angles stay below pi
+
+// Row j of the table for alpha = E0 / 511 (FP64, two-sum compensated).
+SMG_FN void kn_prefix_row(double alpha, int j, double* row) {
+    const double step = kKnStep;
+    const double phi = step * (double)j / (double)kKnPhase;
+    KahanSum<double> s;
+    for (int k = 0; k < kKnSteps; ++k) {
+        const double th = phi + (double)k * step;
+        s.add(kn_of_cos<double>(cos(th), alpha) * sin(th) * step);
+        row[k] = s.value();
+    }
+}
+
+// The rectangle sum over [lo, hi] from the table; n as the FP32 node loop
+// forms it (with the same near-integer flag).  A node range outside the
+// table flags the term for FP64.
+SMG_FN double kn_sum_table(const double* tab, float lo, float hi, bool& near) {
+    const float stepf = float(kKnStep);
+    const float x = (hi - lo) / stepf;
+    const int n = (int)x;
+    const float fr = x - float(n);
+    near |= (fr < 2e-4f && n > 0) | (fr > 1.0f - 2e-4f);
+    const double u = (double)lo * (1.0 / kKnStep);
+    const int k0 = (int)floor(u);
+    double ph = (u - (double)k0) * kKnPhase;
+    int j = (int)ph;
+    if (j >= kKnPhase) j = kKnPhase - 1;
+    if (j < 0) j = 0;
+    const double w = ph - (double)j;
+    const int k1 = k0 + n;
+    if (k0 < 0 || k1 >= kKnSteps) {
+        near = true;
+        return 0.0;
+    }
+    const double* r0 = tab + (size_t)j * kKnSteps;
+    const double* r1 = r0 + kKnSteps;
+    const double a0 = r0[k1] - (k0 > 0 ? r0[k0 - 1] : 0.0);
+    const double a1 = r1[k1] - (k0 > 0 ? r1[k0 - 1] : 0.0);
+    return a0 + w * (a1 - a0);
 }
 
 // Angle between u and w (vectors src -> a and a -> det), used for the polar
@@ -425,6 +476,8 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
     T K;
     if constexpr (std::is_same<T, double>::value)
         K = kn_sum<T>(tmin, tmax, alpha_t, near) / T(ph.norm);
+    else if (ph.knt)
+        K = float(kn_sum_table(ph.knt, tmin, tmax, near) / ph.norm);
     else
         K = kn_sum<T>(tmin, tmax, alpha_t, near) * float(1.0 / ph.norm);
 

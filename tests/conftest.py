@@ -87,6 +87,8 @@ def lane_lib():
     L.lane_pe_pairs.argtypes = [fp, fp, fp, fp] + [C.c_int] * 5 + [ip, ip, C.c_int, dp]
     L.lane_narrow_check.restype = C.c_int
     L.lane_narrow_check.argtypes = [fp, fp, fp, fp] + [C.c_int] * 8 + [C.c_uint, dp]
+    L.lane_kn_sum_table.restype = None
+    L.lane_kn_sum_table.argtypes = [C.c_double, fp, fp, C.c_int, dp, ip]
     L.lane_scatter_pairs.restype = C.c_long
     L.lane_scatter_pairs.argtypes = [fp, fp, fp, fp, fp, up] + [C.c_int] * 7 + [ip, ip, C.c_int, dp]
     return L

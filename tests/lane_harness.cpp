@@ -371,6 +371,10 @@ double scatter_element(const Ctx& c, const float* pe, const uint32_t* bitmap, in
     P.mupe = bd.mupe;
     P.muc = bd.muc;
     P.rcell = std::sqrt(bd.w * bd.w / sx / sx + bd.h * bd.h / sz / sz + bd.t * bd.t / sy / sy) / 2.0;
+    // the KN prefix table the GPU's FP32 path reads (built with the same code)
+    std::vector<double> knt((size_t)(kKnPhase + 1) * kKnSteps);
+    for (int j = 0; j <= kKnPhase; ++j) kn_prefix_row(P.e0 / 511.0, j, knt.data() + (size_t)j * kKnSteps);
+    P.knt = knt.data();
     int n_pts = kind == 0 ? g.n_det : hg.n_holes_l0;
     KahanSum<T> acc;
     for (int cell = 0; cell < sx * sy * sz; ++cell) {
@@ -449,6 +453,19 @@ long lane_scatter_pairs(const float* coll, const float* det, const float* img, c
                        : scatter_element<float>(c, pe, bitmap, kind, variant, d[i], v[i], rot, sx,
                                                 sy, sz, &fb);
     return fb;
+}
+
+
+// KN rectangle sums over [lo[i], hi[i]] from the prefix table of e0 (the FP32
+// scatter path's kn_sum_table); near[i] = 1 where the term would go to FP64.
+void lane_kn_sum_table(double e0, const float* lo, const float* hi, int n, double* out, int* near) {
+    std::vector<double> knt((size_t)(kKnPhase + 1) * kKnSteps);
+    for (int j = 0; j <= kKnPhase; ++j) kn_prefix_row(e0 / 511.0, j, knt.data() + (size_t)j * kKnSteps);
+    for (int i = 0; i < n; ++i) {
+        bool nr = false;
+        out[i] = kn_sum_table(knt.data(), lo[i], hi[i], nr);
+        near[i] = nr;
+    }
 }
 
 }  // extern "C"
