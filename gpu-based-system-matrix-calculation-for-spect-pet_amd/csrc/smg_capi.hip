@@ -1033,8 +1033,14 @@ smg_status smg_pe_backward(smg_ctx* c, int rotation, smg_precision prec, const f
     SMG_HIP(c, pe_cell_table(c, A));
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
     PeGradScratch S;
-    // enough waves to fill the GPU: about 8192 (bar, tile group) pairs
-    S.groups = (int)std::max<int64_t>(1, std::min<int64_t>(n_tiles, (8192 + g.n_det - 1) / g.n_det));
+    // enough waves to fill the GPU: about 8192 (bar, group of work items); a
+    // tile's sub-cells are split into ranges (at least 8 sub-cells each) when
+    // bars x tiles alone are fewer
+    const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
+    const int64_t bt = (int64_t)g.n_det * n_tiles;
+    S.cell_slices = (int)std::max<int64_t>(1, std::min<int64_t>(n_cells / 8, (8192 + bt - 1) / bt));
+    S.groups = (int)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)n_tiles * S.cell_slices, (8192 + g.n_det - 1) / g.n_det));
     S.stride = (int64_t)g.n_det + g.n_layers + 2;
     if ((size_t)g.n_det * sizeof(double) > 64 * 1024)
         return fail(c, SMG_ERR_CAPACITY, "adjoint LDS row of %d bars exceeds 64 KiB", g.n_det);

@@ -54,7 +54,8 @@ hipError_t launch_pe_points(const PeLaunch& a, bool exact, int bar0, int n_bars,
 struct PeGradScratch {
     double* rows;
     int64_t stride;
-    int groups;
+    int groups;       // waves per bar
+    int cell_slices;  // sub-cell ranges per tile (work items per bar: tiles x cell_slices)
     const float* w;   // [N][V] weights of L = sum w A
 };
 

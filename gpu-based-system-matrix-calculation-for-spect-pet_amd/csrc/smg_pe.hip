@@ -417,7 +417,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe
 //   mu_total of d:   -w c (l1 - l2) + w Omega e^-att mu_pe (l_a e^(-mu l_a)/mu - (1 - e^(-mu l_a))/mu^2)
 //   mu_pe of d:      w Omega e^-att (1 - e^(-mu l_a)) / mu
 // The chords through other bars are recomputed after the forward term, not
-// stored.  One wave handles detector d and tiles g, g + G, ...; partial sums
+// stored.  One wave handles detector d and work items g, g + G, ... (an item
+// is a tile, or a tile's range of sub-cells when bars x tiles alone would
+// leave the GPU under-filled); partial sums
 // go to a per-(d, g) row of the scratch buffer (bar mu_total partials dense
 // over all bars, then the layer, mu_pe and loss slots), added in a fixed
 // order, and pe_grad_reduce sums the rows in a fixed order: the result does
@@ -513,7 +515,12 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
     if (bd.mut > (double)0.01f) {
         using T = typename std::conditional<kExact, double, float>::type;
         const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
-        for (int tile = grp; tile < n_tiles; tile += S.groups) {
+        // work items: (tile, range of sub-cells); item i is tile i / cs, range i % cs
+        const int cs = S.cell_slices;
+        const int per = (n_cells + cs - 1) / cs;
+        for (int item = grp; item < n_tiles * cs; item += S.groups) {
+            const int tile = item / cs;
+            const int c_begin = (item % cs) * per, c_end = min(n_cells, c_begin + per);
             WaveSetup W;
             wave_setup(A, d, tile, lane, W);
             int nb, nh;
@@ -526,10 +533,10 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
             __syncthreads();
             // sub-cells in the reference's order; FP32 rays near a threshold
             // are queued and redone in FP64 (as in the forward kernel)
-            int cell = 0;
-            while (cell < n_cells) {
+            int cell = c_begin;
+            while (cell < c_end) {
                 int n_redo = 0;
-                for (; cell < n_cells && n_redo < kRedoCap; ++cell) {
+                for (; cell < c_end && n_redo < kRedoCap; ++cell) {
                     CellCtx cc;
                     tile_cell(A, bd, d, cell, W, lane, cc);
                     int nsb, nsh;
@@ -591,27 +598,35 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
 }
 
 // grad[0 .. Lc-1]: layers; grad[Lc + k]: mu_total of bar k; grad[Lc + N + k]:
-// mu_pe of bar k; grad[Lc + 2N]: the loss.  One thread per entry, rows in order.
+// mu_pe of bar k; grad[Lc + 2N]: the loss.  One block per entry, rows in a fixed order.
 __global__ __launch_bounds__(256) void pe_grad_reduce(PeGradScratch S, int n_det, int n_layers,
                                                       double* grad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n_out = n_layers + 2 * n_det + 1;
-    if (i >= n_out) return;
+    // one block per entry: thread t sums rows t, t + 256, ... in order, then a
+    // fixed tree over the threads (the same order on every run)
+    __shared__ double part[256];
+    const int i = blockIdx.x;
+    const int t = threadIdx.x;
+    const int64_t n_rows = (int64_t)n_det * S.groups;
     double s = 0.0;
     if (i < n_layers) {
-        for (int64_t r = 0; r < (int64_t)n_det * S.groups; ++r) s += S.rows[r * S.stride + n_det + i];
+        for (int64_t r = t; r < n_rows; r += 256) s += S.rows[r * S.stride + n_det + i];
     } else if (i < n_layers + n_det) {
         const int k = i - n_layers;
-        for (int64_t r = 0; r < (int64_t)n_det * S.groups; ++r) s += S.rows[r * S.stride + k];
+        for (int64_t r = t; r < n_rows; r += 256) s += S.rows[r * S.stride + k];
     } else if (i < n_layers + 2 * n_det) {
         const int k = i - n_layers - n_det;
-        for (int grp = 0; grp < S.groups; ++grp)
+        for (int grp = t; grp < S.groups; grp += 256)
             s += S.rows[((int64_t)k * S.groups + grp) * S.stride + n_det + n_layers];
     } else {
-        for (int64_t r = 0; r < (int64_t)n_det * S.groups; ++r)
-            s += S.rows[r * S.stride + n_det + n_layers + 1];
+        for (int64_t r = t; r < n_rows; r += 256) s += S.rows[r * S.stride + n_det + n_layers + 1];
     }
-    grad[i] = s;
+    part[t] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) part[t] += part[t + h];
+        __syncthreads();
+    }
+    if (t == 0) grad[i] = part[0];
 }
 
 // ---- matrix-free projections -----------------------------------------------------------
@@ -778,8 +793,8 @@ hipError_t launch_pe_backward(const PeLaunch& A, const PeGradScratch& S, bool ex
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int n_out = A.geom.g.n_layers + 2 * A.geom.g.n_det + 1;
-    hipLaunchKernelGGL(pe_grad_reduce, dim3((n_out + 255) / 256), dim3(256), 0, stream, S,
-                       A.geom.g.n_det, A.geom.g.n_layers, grad);
+    hipLaunchKernelGGL(pe_grad_reduce, dim3(n_out), dim3(256), 0, stream, S, A.geom.g.n_det,
+                       A.geom.g.n_layers, grad);
     return hipGetLastError();
 }
 
