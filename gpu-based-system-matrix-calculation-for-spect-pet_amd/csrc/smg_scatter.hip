@@ -29,10 +29,6 @@ namespace smg {
 
 namespace {
 
-__device__ __forceinline__ bool bit_set(const uint32_t* words, int64_t b) {
-    return (words[b >> 5] >> (b & 31)) & 1u;
-}
-
 __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const PeGeom& G = A.geom;
@@ -68,9 +64,7 @@ __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
     } else if (A.kind == 1) {
         base = ((int64_t)s * N + d) * N;
     }
-    for (int k = 0; k < N; ++k) {
-        if (k == d || ((A.kind == 0 || A.kind == 3) && k == sbar)) continue;
-        if (A.kind != 2 && !bit_set(A.bitmap, base + k)) continue;
+    auto add_bar = [&](int k) {
         const Bar<double>& bk = G.bars_d[k];
         double pa[3], pb[3];
         to_frame(bk, qs, pa);
@@ -82,6 +76,25 @@ __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
         double l = seg_box<double>(sd, pa, lo, hi, near);
         out.s_pe += l * bk.mupe;
         out.s_c += l * bk.muc;
+    };
+    if (A.kind == 2) {
+        for (int k = 0; k < N; ++k)
+            if (k != d) add_bar(k);
+    } else {
+        // the set bits of bits base .. base + N - 1, a word at a time, in
+        // increasing k (the order of a bit-by-bit scan)
+        const int64_t w0 = base >> 5, w1 = (base + N - 1) >> 5;
+        for (int64_t wi = w0; wi <= w1; ++wi) {
+            uint32_t bits = A.bitmap[wi];
+            if (wi == w0) bits &= ~0u << (base & 31);
+            if (wi == w1) bits &= ~0u >> (31 - ((base + N - 1) & 31));
+            while (bits) {
+                const int k = (int)(wi * 32 + __builtin_ctz(bits) - base);
+                bits &= bits - 1;
+                if (k == d || ((A.kind == 0 || A.kind == 3) && k == sbar)) continue;
+                add_bar(k);
+            }
+        }
     }
     if (A.kind == 2) {  // collimator material from the grid point to the sub-cell
         double att_t;
