@@ -268,10 +268,17 @@ __device__ void narrow_candidates(const PeLaunch& A, const double qd[3], const W
 //   dv mu_c,d / (4 pi |q - p|^2) exp(-att),
 // att being the PE ray's attenuation up to the point (the sub-cell's own
 // chord in bar d added back), to pts[((d - bar0) n_cells + cell) V + v].
+// FP64 form (the exact path and FP64 re-computations); the FP32 path forms
+// the same product in FP32 (relative error ~1e-7).
 __device__ __forceinline__ float point_weight(const Bar<double>& bd, int n_cells, const double aux[8]) {
     const double dv = bd.w * bd.t * bd.h / n_cells;
     return float(aux[5] * exp(-aux[4] * bd.mut) * dv * bd.muc /
                  (4.0 * 3.14159265358979323846 * aux[6]));
+}
+
+__device__ __forceinline__ float point_weight_f(const Bar<double>& bd, int n_cells, const double aux[8]) {
+    const float c = float(bd.w * bd.t * bd.h / n_cells * bd.muc / (4.0 * 3.14159265358979323846));
+    return float(aux[5]) * exp_neg<float>(aux[4] * bd.mut) * tdiv(c, float(aux[6]));
 }
 
 template <bool kExact, bool kPoints>
@@ -345,7 +352,9 @@ __device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
                 if constexpr (kPoints) {
                     double aux[8];
                     pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near, nullptr, aux);
-                    if (!near) pts[(int64_t)cell * g.n_vox + W.v] = point_weight(bd, n_cells, aux);
+                    if (!near)
+                        pts[(int64_t)cell * g.n_vox + W.v] =
+                            kExact ? point_weight(bd, n_cells, aux) : point_weight_f(bd, n_cells, aux);
                 } else {
                     T val = pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near);
                     if (!near) acc.add(val);
