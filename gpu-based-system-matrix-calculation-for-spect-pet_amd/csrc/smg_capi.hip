@@ -1042,8 +1042,6 @@ smg_status smg_pe_backward(smg_ctx* c, int rotation, smg_precision prec, const f
     S.groups = (int)std::max<int64_t>(
         1, std::min<int64_t>((int64_t)n_tiles * S.cell_slices, (8192 + g.n_det - 1) / g.n_det));
     S.stride = (int64_t)g.n_det + g.n_layers + 2;
-    if ((size_t)g.n_det * sizeof(double) > 64 * 1024)
-        return fail(c, SMG_ERR_CAPACITY, "adjoint LDS row of %d bars exceeds 64 KiB", g.n_det);
     SMG_HIP(c, c->scratch.reserve((size_t)g.n_det * S.groups * S.stride));
     S.rows = c->scratch.ptr;
     const int n_out = g.n_layers + 2 * g.n_det + 1;

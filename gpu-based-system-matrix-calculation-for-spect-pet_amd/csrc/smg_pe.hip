@@ -506,7 +506,6 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
     __shared__ double lay_acc[kMaxLayers];
     __shared__ uint16_t redo_cell[kRedoCap];
     __shared__ uint64_t redo_mask[kRedoCap];
-    extern __shared__ double row[];  // n_det partials of the bars' mu_total
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;
     const int lane = threadIdx.x;
@@ -515,7 +514,9 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
     const int grp = (int)(blockIdx.x % S.groups);
     const Bar<double>& bd = G.bars_d[d];
     double* out_row = S.rows + (int64_t)blockIdx.x * S.stride;
-    for (int k = lane; k < g.n_det; k += 64) row[k] = 0.0;
+    // the bars' mu_total partials accumulate in this wave's own row of the
+    // scratch buffer (no other wave writes it; LDS would cap the occupancy)
+    for (int k = lane; k < g.n_det; k += 64) out_row[k] = 0.0;
     if (lane < kMaxLayers) lay_acc[lane] = 0.0;
     AdjAcc acc;
     unsigned fallbacks = 0;
@@ -584,14 +585,14 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
             }
             // fold this tile's candidate partials into the dense row (one
             // entry per candidate, each bar at most once per tile)
-            for (int j = lane; j < nb; j += 64) row[L.bar_idx[j]] += cand_acc[j];
+            for (int j = lane; j < nb; j += 64) out_row[L.bar_idx[j]] += cand_acc[j];
             __syncthreads();
         }
     }
     const double own_mut = wave_sum(acc.own_mut);
     const double own_mupe = wave_sum(acc.own_mupe);
     const double loss = wave_sum(acc.loss);
-    for (int k = lane; k < g.n_det; k += 64) out_row[k] = row[k];
+    __syncthreads();  // the lanes' row updates before lane 0 adds to out_row[d]
     if (lane == 0) {
         out_row[d] += own_mut;
         for (int m = 0; m < g.n_layers; ++m) out_row[g.n_det + m] = lay_acc[m];
@@ -794,11 +795,10 @@ hipError_t launch_pe_backward(const PeLaunch& A, const PeGradScratch& S, bool ex
                               hipStream_t stream) {
     const int64_t blocks = (int64_t)A.geom.g.n_det * S.groups;
     if (blocks <= 0) return hipSuccess;
-    const size_t lds = sizeof(double) * (size_t)A.geom.g.n_det;
     if (exact)
-        hipLaunchKernelGGL(pe_backward_kernel<true>, dim3((unsigned)blocks), dim3(64), lds, stream, A, S);
+        hipLaunchKernelGGL(pe_backward_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A, S);
     else
-        hipLaunchKernelGGL(pe_backward_kernel<false>, dim3((unsigned)blocks), dim3(64), lds, stream, A, S);
+        hipLaunchKernelGGL(pe_backward_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A, S);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int n_out = A.geom.g.n_layers + 2 * A.geom.g.n_det + 1;
