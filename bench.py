@@ -204,6 +204,14 @@ def extras(smg, params, pe_ctx, out, precision):
                      "backward_over_forward": round(b_ms / f_ms, 2),
                      "workload": "32x32x1 voxels x 64 bars, 4x4x4 sub-cells, 1 + 128 mu"}
     log(f"adjoint: forward {f_ms:.3f} ms, backward {b_ms:.3f} ms")
+
+    # the same on the headline geometry (2048 bars, 64x64x32 voxels, 1 + 4096 mu)
+    wf = np.random.default_rng(2).uniform(0, 1, (pe_ctx.dims.n_det, pe_ctx.dims.n_vox)).astype(np.float32)
+    bf_ms = timed(lambda: pe_ctx.pe_backward(wf)[1:], 1)
+    ex["adjoint_full_size"] = {"forward_kernel_ms": round(ms_this, 2), "backward_kernel_ms": round(bf_ms, 2),
+                               "backward_over_forward": round(bf_ms / ms_this, 2),
+                               "workload": "64x64x32 voxels x 2048 bars, 3x3x3 sub-cells, 1 + 4096 mu"}
+    log(f"adjoint, full size: backward {bf_ms:.1f} ms (forward {ms_this:.1f} ms)")
     return ex
 
 
