@@ -15,7 +15,7 @@ collimator scatter on 32x32x16 voxels x 64 bars), with scatter points on a
 0.5 mm grid over the collimator tungsten and on a 0.25 mm grid in every bar
 (8 detector rows of 2048 bars), the matrix-free forward and back-projection
 times with the headline matrix, and the forward and adjoint (dL/dmu) times on
-the 32x32x1-voxel, 64-bar geometry.
+the 32x32x1-voxel, 64-bar geometry and on the full-size geometry.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision fp32|fp64]
 """
