@@ -454,7 +454,7 @@ template <class T>
 __device__ bool adjoint_cell(const PeGeom& G, int d, int lane, const WaveSetup& W,
                              const WaveLists& L, const LdsCands& C, const CellCtx& cc,
                              bool active, double wv, AdjAcc& acc, double* lay_acc,
-                             double* cand_acc) {
+                             double* cand_acc, float* keep = nullptr) {
     const GeomDesc& g = G.g;
     const Bar<double>& bd = G.bars_d[d];
     const double mut = bd.mut, mupe = bd.mupe;
@@ -463,7 +463,7 @@ __device__ bool adjoint_cell(const PeGeom& G, int d, int lane, const WaveSetup& 
     for (int m = 0; m < kMaxLayers; ++m) dlay[m] = 0.0;
     bool near = false;
     if (active) {
-        term = (double)pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near, dlay, aux);
+        term = (double)pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near, dlay, aux, keep, 64);
         if (kGuarded<T> && near) active = false;
     }
     const double cw = active ? wv * term : 0.0;
@@ -480,18 +480,27 @@ __device__ bool adjoint_cell(const PeGeom& G, int d, int lane, const WaveSetup& 
             if (lane == 0) lay_acc[m] += x;
         }
     }
-    // chords through the other bars, recomputed
-    double D[3], dd;
+    // chords through the other bars: kept from the ray (the first kKeepChords
+    // candidates, FP32 rays) or recomputed
+    double D[3], dd = 0.0, inv_dd = 0.0;
     SegDir<T> sw;
-    T tol_d;
-    bool nd = false;
-    ray_dir<T>(W.p, cc.q, D, dd, sw, tol_d, nd);
-    const double inv_dd = 1.0 / dd;
+    T tol_d = T(0);
+    if (!keep || C.nbars() > kKeepChords) {  // wave-uniform
+        bool nd = false;
+        ray_dir<T>(W.p, cc.q, D, dd, sw, tol_d, nd);
+        inv_dd = 1.0 / dd;
+    }
     for (int j = 0; j < C.nbars(); ++j) {
         double x = 0.0;
         if (active && cw != 0.0 && C.bar_on(j)) {
-            bool n3 = false;
-            x = -cw * (double)other_bar_chord<T>(G, C.bar(j), W.p, D, dd, inv_dd, cc.q, sw, tol_d, n3);
+            double l;
+            if (keep && j < kKeepChords) {
+                l = (double)keep[j * 64];
+            } else {
+                bool n3 = false;
+                l = (double)other_bar_chord<T>(G, C.bar(j), W.p, D, dd, inv_dd, cc.q, sw, tol_d, n3);
+            }
+            x = -cw * l;
         }
         x = wave_sum(x);
         if (lane == 0) cand_acc[L.bar_sel[j]] += x;
@@ -504,6 +513,7 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
     __shared__ WaveLists L;
     __shared__ double cand_acc[kCapBars];
     __shared__ double lay_acc[kMaxLayers];
+    __shared__ float keep[kKeepChords * 64];  // FP32 rays' chords, [candidate][lane]
     __shared__ uint16_t redo_cell[kRedoCap];
     __shared__ uint64_t redo_mask[kRedoCap];
     const PeGeom& G = A.geom;
@@ -555,7 +565,7 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                     __syncthreads();
                     LdsCands C{&L, nsb, nsh, lane};
                     const bool near = adjoint_cell<T>(G, d, lane, W, L, C, cc, W.valid, wv, acc,
-                                                      lay_acc, cand_acc);
+                                                      lay_acc, cand_acc, kExact ? nullptr : keep + lane);
                     if constexpr (!kExact) {
                         const uint64_t m = __ballot(near);
                         if (m) {

@@ -424,10 +424,13 @@ SMG_FN void ray_dir(const double p[3], const double q[3], double D[3], double& d
     segdir_init<T>(sw, T(D[0]), T(D[1]), T(D[2]), tol_d, near);
 }
 
+// Chords a caller may keep from pe_cell_ray (the adjoint's second use).
+constexpr int kKeepChords = 8;
+
 template <class T, class Cands>
 SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[3],
                      const CellCtx& cc, const Cands& C, bool& near, double* dl_layer = nullptr,
-                     double* aux = nullptr) {
+                     double* aux = nullptr, float* keep = nullptr, int keep_stride = 0) {
     const GeomDesc& g = G.g;
     const Bar<double>& bd = G.bars_d[d];
     double D[3] = {cc.q[0] - p[0], cc.q[1] - p[1], cc.q[2] - p[2]};
@@ -519,11 +522,17 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
         if (dl_layer) store_at<kMaxLayers>(dl_layer, m, counted ? rest : 0.0);
     }
 
-    // other bars on the way (PESysMatGen.cu:1290-1325)
+    // other bars on the way (PESysMatGen.cu:1290-1325); with `keep`, the
+    // chords of the first kKeepChords candidates are also stored (0 where the
+    // candidate is not on this lane's list), keep[j * keep_stride]
     for (int j = 0; j < C.nbars(); ++j) {
-        if (!C.bar_on(j)) continue;
+        if (!C.bar_on(j)) {
+            if (keep && j < kKeepChords) keep[j * keep_stride] = 0.0f;
+            continue;
+        }
         int k = C.bar(j);
         T l = other_bar_chord<T>(G, k, p, D, dd, inv_dd, cc.q, sw, tol_d, near);
+        if (keep && j < kKeepChords) keep[j * keep_stride] = float(l);
         att += double(l) * G.bars_d[k].mut;
     }
 
