@@ -442,9 +442,23 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
         tmin = atan2f(s_lo, c_lo);
         tmax = atan2f(s_hi, c_hi);
     }
-    // the line voxel -> scatter point, extended 1000 mm, through the sub-cell
-    T lthru;
-    {
+    // the line voxel -> scatter point, extended 1000 mm, through the sub-cell.
+    // FP32 path: only its test against 0.001 is used, and a line that passes
+    // the sub-cell's centre farther than its bounding radius + 0.01 mm has
+    // chord 0 in the reference too (its small-component branches move the
+    // line by less than 0.001 mm), so the chord is formed only otherwise.
+    T lthru = T(0);
+    bool lthru_needed = true;
+    if constexpr (std::is_same<T, float>::value) {
+        float w[3];
+        for (int k = 0; k < 3; ++k) w[k] = float(0.5 * (sc.lo[k] + sc.hi[k]) - sc.qss[k]);
+        const float cx = w[1] * us[2] - w[2] * us[1], cy = w[2] * us[0] - w[0] * us[2],
+                    cz = w[0] * us[1] - w[1] * us[0];
+        const float u2 = us[0] * us[0] + us[1] * us[1] + us[2] * us[2];
+        const float R = float(ph.rcell) + 0.01f;
+        lthru_needed = !(cx * cx + cy * cy + cz * cz > R * R * u2);
+    }
+    if (lthru_needed) {
         double dis2 = 0.0;
         double D[3];
         for (int k = 0; k < 3; ++k) D[k] = sc.qss[k] - ps[k];
