@@ -190,6 +190,24 @@ def extras(smg, params, pe_ctx, out, precision):
             f"centres of 10x32x10 cells (0.25 mm) of every bar ({p4.n_det * 3200} points), "
             "1x1x1 cells, power-law mu(E)")
 
+    # geometry bitmaps of the scatter kernels: the full-size detector and 6,000 bars
+    for tag, pb in (("2048_bars", params.config(2)), ("6000_bars", params.large_detector())):
+        with smg.open_context(pb) as c:
+            n = pb.n_det
+            c.build_bitmap(False)
+            ms_c = min(c.build_bitmap(False) for _ in range(2))
+            c.build_bitmap(True)
+            ms_l = min(c.build_bitmap(True) for _ in range(2))
+            bits_c = n * (n + 1) // 2 * n
+            bits_l = c.dims.n_holes_l0 * n * n
+        ex[f"bitmaps_{tag}"] = {
+            "crystal_ms": round(ms_c, 2), "crystal_bits_per_s": round(bits_c / ms_c * 1e3, 1),
+            "crystal_GB": round(bits_c / 8e9, 3),
+            "collimator_ms": round(ms_l, 2), "collimator_bits_per_s": round(bits_l / ms_l * 1e3, 1),
+            "collimator_GB": round(bits_l / 8e9, 3),
+            "workload": f"{n} bars, 400 layer-0 holes; wall time of the call (kernel + sync)"}
+        log(f"bitmaps {tag}: crystal {ms_c:.1f} ms, collimator {ms_l:.1f} ms")
+
     # adjoint: forward + dL/dmu on the 32x32x1-voxel, 64-bar geometry, 4x4x4 sub-cells
     p5 = params.config(5)
     w = np.random.default_rng(2).uniform(0, 1, (p5.n_det, p5.n_vox)).astype(np.float32)

@@ -1,140 +1,79 @@
 // Geometry bitmaps of the scatter kernels, gfx950.
 //
-// Crystal bitmap (ScatterGen_RayTracing_CircularHole/scatter.cu:1440-1540):
-// bit (i<j, k) says bar k lies between bars i and j by an angular
-// bounding-sphere test seen from i or from j; bits of the unordered pair
-// {i, j} start at (i N - i(i+1)/2 + j) N (scatter.cu:1040-1059).
-// Collimator bitmap (scatter.cu:1939-2009): bit (h, j, k) says bar k lies on
-// the ray from layer-0 hole h toward bar j; bit index h N^2 + j N + k.
-//
-// The reference runs one thread per bit and sets bits with atomicOr.  Here one
-// thread owns one 32-bit word: it evaluates its 32 predicates and stores the
-// word once (no atomics, coalesced stores).  Predicates run in FP32 and are
-// re-evaluated in FP64 when within rounding distance of their threshold, so
-// the words match an FP64 evaluation of the reference test.
+// The reference runs one thread per bit and sets bits with atomicOr
+// (scatter.cu:1538).  Here one wave owns 64 consecutive 32-bit words (2,048
+// bits): at step s lane l evaluates bit 64 s + l of the wave's run, so the
+// lanes read consecutive bars k, and a ballot gives words 2s and 2s + 1, which
+// lanes 2s and 2s + 1 keep; after 32 steps each lane stores its word (no
+// atomics, one coalesced 256-byte store per wave).  A lane keeps its row's
+// terms across steps and recomputes them only when its k wraps into the next
+// row.  The bit predicates and the row walk are in smg_bitmap_body.hpp.
+// A grid-stride loop over the runs covers any bitmap size (the reference's
+// limit of N = 6,665 bars gives 4.6e9 words).
 #include <hip/hip_runtime.h>
 
+#include "smg_bitmap_body.hpp"
 #include "smg_kernels.hpp"
 
 namespace smg {
 
 namespace {
 
-template <class T>
-__device__ __forceinline__ T dist3(T ax, T ay, T az, T bx, T by, T bz) {
-    T dx = ax - bx, dy = ay - by, dz = az - bz;
-    return sqrt(dx * dx + dy * dy + dz * dz);
-}
+constexpr int kWordsPerRun = 64;                // one word per lane
+constexpr int64_t kBitsPerRun = 32 * kWordsPerRun;
 
-// |u_{k<-i} - u_{j<-i}| <= R_k / L_ik + R_j / L_ij, the test seen from i.
-template <class T>
-__device__ __forceinline__ bool seen_from(const Sphere<T>& i, const Sphere<T>& j,
-                                          const Sphere<T>& k, bool& near) {
-    T lij = dist3(i.x, i.y, i.z, j.x, j.y, j.z);
-    T lik = dist3(k.x, k.y, k.z, i.x, i.y, i.z);
-    T ux = (k.x - i.x) / lik - (j.x - i.x) / lij;
-    T uy = (k.y - i.y) / lik - (j.y - i.y) / lij;
-    T uz = (k.z - i.z) / lik - (j.z - i.z) / lij;
-    T s = sqrt(ux * ux + uy * uy + uz * uz);
-    T lim = k.r / lik + j.r / lij;
-    return g_le(s, lim, T(64) * kUnit<T> * (s + lim + T(1)), near);
-}
-
-// Crystal predicate: from i, else from j (scatter.cu:1518-1533).
-template <class T>
-__device__ __forceinline__ bool crystal_pred(const Sphere<T>& i, const Sphere<T>& j,
-                                             const Sphere<T>& k, bool& near) {
-    bool n1 = false;
-    bool a = seen_from(i, j, k, n1);
-    near |= n1;
-    if (a && !n1) return true;
-    bool n2 = false;
-    bool b = seen_from(j, i, k, n2);
-    near |= n2;
-    return a || b;
-}
-
-__device__ __forceinline__ bool crystal_bit(const BitmapLaunch& A, int i, int j, int k) {
-    bool near = false;
-    bool r = crystal_pred<float>(A.spheres_f[i], A.spheres_f[j], A.spheres_f[k], near);
-    if (!near) return r;
-    bool n2 = false;
-    return crystal_pred<double>(A.spheres_d[i], A.spheres_d[j], A.spheres_d[k], n2);
-}
-
-__device__ __forceinline__ bool coll_bit(const BitmapLaunch& A, int h, int j, int k) {
-    bool near = false;
-    bool r = seen_from<float>(A.holes_f[h], A.spheres_f[j], A.spheres_f[k], near);
-    if (!near) return r;
-    bool n2 = false;
-    return seen_from<double>(A.holes_d[h], A.spheres_d[j], A.spheres_d[k], n2);
-}
-
-// First pair index of row i (the pair (i, i)): S(i) = i N - i(i-1)/2.
-__device__ __forceinline__ int64_t row_start(int64_t i, int64_t n) { return i * n - i * (i - 1) / 2; }
-
-__global__ __launch_bounds__(256) void bitmap_crystal_kernel(BitmapLaunch A) {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= A.n_words) return;
+template <bool kColl>
+__global__ __launch_bounds__(256) void bitmap_kernel(BitmapLaunch A) {
+    const int lane = threadIdx.x & 63;
     const int64_t n = A.n_det;
-    const int64_t total = n * (n + 1) / 2 * n;
-    int64_t b = w * 32;
-    int64_t pair = b / n;
-    int k = (int)(b % n);
-    // invert pair = S(i) + (j - i)
-    double disc = (2.0 * n + 1) * (2.0 * n + 1) - 8.0 * (double)pair;
-    int64_t i = (int64_t)floor(((2.0 * n + 1) - sqrt(disc > 0 ? disc : 0.0)) / 2.0);
-    if (i < 0) i = 0;
-    if (i > n - 1) i = n - 1;
-    while (i > 0 && row_start(i, n) > pair) --i;
-    while (i + 1 < n && row_start(i + 1, n) <= pair) ++i;
-    int64_t j = pair - row_start(i, n) + i;
-    uint32_t word = 0;
-    for (int bit = 0; bit < 32 && b < total; ++bit, ++b) {
-        if (i < j && k != i && k != j && crystal_bit(A, (int)i, (int)j, k)) word |= 1u << bit;
-        if (++k == n) {
-            k = 0;
-            if (++j == n) {
-                ++i;
-                j = i;
+    const int64_t total = bm_total_bits(kColl, n, A.n_holes);
+    const int64_t n_runs = (A.n_words + kWordsPerRun - 1) / kWordsPerRun;
+    const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    // the run index is the same for the 64 lanes of a wave, so every ballot
+    // below sees the whole wave
+    for (int64_t run = (((int64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6; run < n_runs;
+         run += stride) {
+        int64_t b = run * kBitsPerRun + lane;
+        BitRow r{};
+        int k = 0;
+        if (b < total) {
+            bm_row_of_bit(kColl, b, n, r, k);
+            bm_row_setup(kColl, A.spheres_d, A.holes_d, r);
+        }
+        uint32_t word = 0;
+        for (int s = 0; s < 32; ++s) {
+            const bool bit = b < total && bm_bit(kColl, r, A.spheres_d, A.holes_d, k);
+            const uint64_t m = __ballot(bit);
+            if (lane == 2 * s) word = (uint32_t)m;
+            if (lane == 2 * s + 1) word = (uint32_t)(m >> 32);
+            b += 64;
+            k += 64;
+            if (b < total && k >= n) {
+                do {
+                    k -= (int)n;
+                    bm_next_row(kColl, n, r);
+                } while (k >= n);
+                bm_row_setup(kColl, A.spheres_d, A.holes_d, r);
             }
         }
+        const int64_t w = run * kWordsPerRun + lane;
+        if (w < A.n_words) A.words[w] = word;
     }
-    A.words[w] = word;
-}
-
-__global__ __launch_bounds__(256) void bitmap_coll_kernel(BitmapLaunch A) {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= A.n_words) return;
-    const int64_t n = A.n_det;
-    const int64_t total = (int64_t)A.n_holes * n * n;
-    int64_t b = w * 32;
-    int h = (int)(b / (n * n));
-    int64_t rem = b % (n * n);
-    int j = (int)(rem / n), k = (int)(rem % n);
-    uint32_t word = 0;
-    for (int bit = 0; bit < 32 && b < total; ++bit, ++b) {
-        if (coll_bit(A, h, j, k)) word |= 1u << bit;
-        if (++k == n) {
-            k = 0;
-            if (++j == n) {
-                j = 0;
-                ++h;
-            }
-        }
-    }
-    A.words[w] = word;
 }
 
 }  // namespace
 
 hipError_t launch_bitmap(const BitmapLaunch& A, bool collimator, hipStream_t stream) {
     if (A.n_words <= 0) return hipSuccess;
-    const int64_t blocks = (A.n_words + 255) / 256;
+    const int64_t n_runs = (A.n_words + kWordsPerRun - 1) / kWordsPerRun;
+    // four runs per block of 256 threads; at most 2^16 blocks (262,144 waves),
+    // the rest by the grid-stride loop
+    int64_t blocks = (n_runs + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
     if (collimator)
-        hipLaunchKernelGGL(bitmap_coll_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, A);
+        hipLaunchKernelGGL(bitmap_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, stream, A);
     else
-        hipLaunchKernelGGL(bitmap_crystal_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, A);
+        hipLaunchKernelGGL(bitmap_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, stream, A);
     return hipGetLastError();
 }
 

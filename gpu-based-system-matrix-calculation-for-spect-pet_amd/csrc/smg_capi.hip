@@ -168,33 +168,6 @@ smg_status check_ready(smg_ctx* c, int rotation) {
 }
 
 
-// Records of the bitmaps and scatter kernels.
-void make_scatter_records(const HostGeom& hg, std::vector<Sphere<float>>& sf,
-                          std::vector<Sphere<double>>& sd, std::vector<Sphere<float>>& hf,
-                          std::vector<Sphere<double>>& hd, std::vector<ScatterPt<double>>& pb,
-                          std::vector<ScatterPt<double>>& ph) {
-    const GeomDesc& g = hg.g;
-    const double F = g.fov2coll;
-    for (int k = 0; k < g.n_det; ++k) {
-        const float* b = hg.det.data() + 12 * k;
-        double w = b[4], t = b[5], h = b[6];
-        Sphere<double> S{(double)b[1], (double)b[2], (double)b[3], std::sqrt(w * w + h * h + t * t) / 2.0};
-        sd.push_back(S);
-        sf.push_back(Sphere<float>{(float)S.x, (float)S.y, (float)S.z, (float)S.r});
-        const Bar<double>& B = hg.bars_d[k];
-        pb.push_back(ScatterPt<double>{B.cx, B.cy, B.cz, 0.0});
-    }
-    for (int i = 0; i < hg.n_holes_l0; ++i) {
-        const float* hp = hg.coll.data() + 100 + 9 * i;
-        double y = ((double)hp[1] + (double)hp[2]) / 2.0;
-        Sphere<double> S{(double)hp[0], y, (double)hp[3], 0.0};
-        hd.push_back(S);
-        hf.push_back(Sphere<float>{(float)S.x, (float)S.y, (float)S.z, 0.0f});
-        ph.push_back(ScatterPt<double>{(double)hp[0], y + F, (double)hp[3],
-                                       (double)hp[2] - (double)hp[1]});
-    }
-}
-
 // Sum of KN(theta) sin(theta) * step over [0, pi] with the reference's float
 // constants (ScatterGen_RayTracing_CircularHole/scatter.cu:1169, :980-993).
 double kn_norm(double e0) {

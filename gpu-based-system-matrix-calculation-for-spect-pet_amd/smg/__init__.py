@@ -271,17 +271,34 @@ class Context:
         return x, st.as_dict()
 
     # ---- bitmaps -----------------------------------------------------------------
-    def bitmap_crystal(self) -> np.ndarray:
+    def bitmap_crystal(self, device_ptr: int | None = None) -> np.ndarray | None:
+        """Compute the crystal bitmap; copy it to a new host array, or to the
+        device buffer at device_ptr (then None is returned)."""
         n = self._L.smg_bitmap_crystal_words(self.dims.n_det)
+        if device_ptr is not None:
+            self._check(self._L.smg_bitmap_crystal(self._h, C.c_void_p(device_ptr), n, 1))
+            return None
         words = np.zeros(n, np.uint32)
         self._check(self._L.smg_bitmap_crystal(self._h, words.ctypes.data_as(C.c_void_p), n, 0))
         return words
 
-    def bitmap_collimator(self) -> np.ndarray:
+    def bitmap_collimator(self, device_ptr: int | None = None) -> np.ndarray | None:
         n = self._L.smg_bitmap_collimator_words(self.dims.n_holes_l0, self.dims.n_det)
+        if device_ptr is not None:
+            self._check(self._L.smg_bitmap_collimator(self._h, C.c_void_p(device_ptr), n, 1))
+            return None
         words = np.zeros(n, np.uint32)
         self._check(self._L.smg_bitmap_collimator(self._h, words.ctypes.data_as(C.c_void_p), n, 0))
         return words
+
+    def build_bitmap(self, collimator: bool = False) -> float:
+        """Compute a bitmap into the context only (no copy out); returns the
+        wall time of the call in ms (kernel plus synchronisation)."""
+        import time
+        fn = self._L.smg_bitmap_collimator if collimator else self._L.smg_bitmap_crystal
+        t0 = time.perf_counter()
+        self._check(fn(self._h, None, 0, 0))
+        return (time.perf_counter() - t0) * 1e3
 
     def load_bitmap_crystal(self, words: np.ndarray):
         words = np.ascontiguousarray(words, dtype=np.uint32)

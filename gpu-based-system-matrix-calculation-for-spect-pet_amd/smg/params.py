@@ -262,6 +262,26 @@ def config(idx: int) -> Params:
     return p
 
 
+def large_detector(n_layers: int = 3, nx: int = 50, nz: int = 40) -> Params:
+    """The full-size collimator with a larger detector for the bitmap
+    workloads: n_layers layers of nx x nz bars 2.5x8x2.5 mm at pitch 2.5 mm,
+    y = 18, 26, 34, ... (3 x 50 x 40 = 6,000 bars by default, the size of the
+    crystal bitmap's N^3 pass the reference describes: 13.5 GB of bits)."""
+    p = _cfg2_base()
+    n = n_layers * nx * nz
+    if n > MAX_BARS:
+        raise ValueError(f"{n} bars exceed the parameter layout's {MAX_BARS}")
+    p.det[:] = 0.0
+    p.det[0] = n
+    for layer in range(n_layers):
+        for iz in range(nz):
+            for ix in range(nx):
+                set_bar(p, (layer * nz + iz) * nx + ix, (ix - (nx - 1) / 2) * 2.5, 18.0 + 8.0 * layer,
+                        (iz - (nz - 1) / 2) * 2.5, 2.5, 8.0, 2.5, res=0.10, **BAR_MU)
+    p.meta = dict(name=f"large_detector_{n}bars")
+    return p
+
+
 def grad_param_count(p: Params) -> int:
     """Gradient vector length: one mu per collimator layer, then mu_total and
     mu_pe of every bar."""

@@ -91,6 +91,8 @@ def lane_lib():
     L.lane_kn_sum_table.argtypes = [C.c_double, fp, fp, C.c_int, dp, ip]
     L.lane_scatter_pairs.restype = C.c_long
     L.lane_scatter_pairs.argtypes = [fp, fp, fp, fp, fp, up] + [C.c_int] * 7 + [ip, ip, C.c_int, dp]
+    L.lane_bitmap.restype = C.c_long
+    L.lane_bitmap.argtypes = [fp, fp, fp, fp, C.c_int, up, C.c_long]
     return L
 
 

@@ -12,6 +12,7 @@
 
 #include "smg_parse.hpp"
 #include "smg_pe_body.hpp"
+#include "smg_bitmap_body.hpp"
 #include "smg_scatter_body.hpp"
 
 using namespace smg;
@@ -466,6 +467,54 @@ void lane_kn_sum_table(double e0, const float* lo, const float* hi, int n, doubl
         out[i] = kn_sum_table(knt.data(), lo[i], hi[i], nr);
         near[i] = nr;
     }
+}
+
+// Crystal (collimator = 0) or collimator bitmap words by the GPU kernel's
+// walk: runs of 64 words, lane l of a run walking bits 64 s + l (s = 0..31)
+// with its row terms kept across steps.  Returns the number of words, or -1.
+long lane_bitmap(const float* coll, const float* det, const float* img, const float* phys,
+                 int collimator, uint32_t* words, long n_words) {
+    HostGeom hg;
+    std::string why;
+    if (parse_geometry(coll, det, img, phys, hg, why) != 0) return -1;
+    std::vector<Sphere<float>> sf, hf;
+    std::vector<Sphere<double>> sd, hd;
+    std::vector<ScatterPt<double>> pb, ph;
+    make_scatter_records(hg, sf, sd, hf, hd, pb, ph);
+    const bool cl = collimator != 0;
+    const int64_t n = hg.g.n_det;
+    const int64_t total = bm_total_bits(cl, n, hg.n_holes_l0);
+    const long need = (long)((total + 31) / 32);
+    if (n_words < need) return -1;
+    for (long w = 0; w < need; ++w) words[w] = 0;
+    const int64_t n_runs = (need + 63) / 64;
+    for (int64_t run = 0; run < n_runs; ++run) {
+        for (int lane = 0; lane < 64; ++lane) {
+            int64_t b = run * 2048 + lane;
+            BitRow r{};
+            int k = 0;
+            if (b < total) {
+                bm_row_of_bit(cl, b, n, r, k);
+                bm_row_setup(cl, sd.data(), hd.data(), r);
+            }
+            for (int s = 0; s < 32; ++s) {
+                if (b < total && bm_bit(cl, r, sd.data(), hd.data(), k)) {
+                    const int64_t w = run * 64 + 2 * s + lane / 32;
+                    words[w] |= 1u << (lane % 32);
+                }
+                b += 64;
+                k += 64;
+                if (b < total && k >= n) {
+                    do {
+                        k -= (int)n;
+                        bm_next_row(cl, n, r);
+                    } while (k >= n);
+                    bm_row_setup(cl, sd.data(), hd.data(), r);
+                }
+            }
+        }
+    }
+    return need;
 }
 
 }  // extern "C"

@@ -63,6 +63,80 @@ def test_bitmaps_cfg2_sampled_bits(smg_mod, oracle_mod, params_mod):
         assert got == o.coll_pred(int(a), int(b), int(c)), (a, b, c)
 
 
+def _word_bits(words, bits):
+    """Bits at the given indices of a device bitmap (torch int32 tensor)."""
+    import torch
+    idx = torch.as_tensor(np.asarray(bits, np.int64) // 32, device=words.device)
+    w = words[idx].cpu().numpy().view(np.uint32).astype(np.int64)
+    return (w >> (np.asarray(bits, np.int64) % 32)) & 1
+
+
+def test_bitmaps_6000_bars(smg_mod, oracle_mod, params_mod):
+    """6,000 bars (3 layers of 50x40) and 400 holes: the 13.5 GB crystal
+    bitmap (3.4e9 words, word indices past 2^31) and the 1.8 GB collimator
+    bitmap, built into device memory.  Whole rows of k for random pairs, the last pair row,
+    and seeded triples near each other are checked against the oracle's
+    predicate."""
+    import torch
+    p = params_mod.large_detector()
+    o = oracle_mod.Oracle(p)
+    N, H = o.n_det, o.n_holes_l0
+    assert N == 6000 and H == 400
+    rng = np.random.default_rng(8)
+    with smg_mod.open_context(p) as ctx:
+        n_c = ctx._L.smg_bitmap_crystal_words(N)
+        n_l = ctx._L.smg_bitmap_collimator_words(H, N)
+        assert n_c > 2 ** 32 // 2
+        ms_c = ctx.build_bitmap(False)
+        ms_l = ctx.build_bitmap(True)
+        bc = torch.empty(n_c, dtype=torch.int32, device="cuda")
+        ctx.bitmap_crystal(device_ptr=bc.data_ptr())
+        bl = torch.empty(n_l, dtype=torch.int32, device="cuda")
+        ctx.bitmap_collimator(device_ptr=bl.data_ptr())
+        torch.cuda.synchronize()
+    print(f"\n6000-bar bitmaps: crystal {n_c} words {ms_c:.0f} ms, collimator {n_l} words {ms_l:.0f} ms")
+
+    def pair_base(a, b):
+        return (a * N - a * (a + 1) // 2 + b) * N
+
+    # whole rows: 12 random pairs, neighbours, and the last pair (N-2, N-1)
+    pairs = [tuple(sorted(rng.choice(N, 2, replace=False))) for _ in range(12)]
+    pairs += [(10, 11), (1000, 1050), (N - 2, N - 1)]
+    ks = np.arange(N)
+    n_set = 0
+    for a, b in pairs:
+        got = _word_bits(bc, pair_base(a, b) + ks)
+        want = [0 if c in (a, b) else o.crystal_pred(int(a), int(b), int(c)) for c in ks]
+        bad = np.nonzero(got != np.array(want))[0]
+        assert bad.size == 0, (a, b, bad[:5])
+        n_set += int(got.sum())
+    # the bit after the last pair's row does not exist; the last word's tail is 0
+    total = N * (N + 1) // 2 * N
+    tail = _word_bits(bc, np.arange(total, n_c * 32))
+    assert not tail.any()
+    # triples near each other
+    i = rng.integers(0, N - 1, 3000)
+    j = np.clip(i + rng.integers(1, 300, 3000), 0, N - 1)
+    k = np.clip(i + rng.integers(-150, 450, 3000), 0, N - 1)
+    keep = (i < j) & (k != i) & (k != j)
+    got = _word_bits(bc, pair_base(i[keep], j[keep]) + k[keep])
+    want = np.array([o.crystal_pred(int(a), int(b), int(c)) for a, b, c in zip(i[keep], j[keep], k[keep])])
+    assert np.array_equal(got, want)
+    assert n_set + int(got.sum()) > 50
+    # collimator: whole rows (h, j) and seeded triples
+    for h, b in [(0, 0), (17, 2999), (H - 1, N - 1)] + [tuple(x) for x in zip(rng.integers(0, H, 4), rng.integers(0, N, 4))]:
+        got = _word_bits(bl, (h * N + b) * N + ks)
+        want = np.array([o.coll_pred(int(h), int(b), int(c)) for c in ks])
+        assert np.array_equal(got, want), (h, b)
+    h = rng.integers(0, H, 3000)
+    j = rng.integers(0, N, 3000)
+    k = np.clip(j + rng.integers(-100, 100, 3000), 0, N - 1)
+    got = _word_bits(bl, (h * N + j) * N + k)
+    want = np.array([o.coll_pred(int(a), int(b), int(c)) for a, b, c in zip(h, j, k)])
+    assert np.array_equal(got, want)
+    assert want.sum() > 50
+
+
 def _sample(o, n, seed):
     rng = np.random.default_rng(seed)
     return rng.integers(0, o.n_det, n), rng.integers(0, o.n_vox, n)

@@ -119,3 +119,43 @@ def test_kn_prefix_table_sums(lane_lib, oracle_mod):
         rel = np.abs(out[ok] - ref[ok]) / np.abs(ref[ok])
         print(f"KN table e0={e0}: max rel {rel.max():.2e} over {ok.sum()} ranges", flush=True)
         assert rel.max() < 1e-8
+
+
+def _bitmap_geometry(params_mod, case):
+    if case.startswith("cfg2"):
+        p = params_mod.config(2)
+        p.det[0] = 300
+        p.coll[10] = 40
+        return p
+    p = params_mod.config(1)
+    if case == "ragged":  # words straddle rows
+        p.det[0] = 37
+        p.coll[10] = 5
+    elif case == "tiny":  # rows shorter than one step of 64 bits
+        p.det[0] = 5
+        p.coll[10] = 2
+    elif case == "sizes":  # bars of different sizes: R_i / L_ij != R_j / L_ij
+        n = int(p.det[0])
+        rng = np.random.default_rng(7)
+        for c in (4, 5, 6):
+            p.det[12 * np.arange(n) + c] *= rng.uniform(0.5, 3.0, n).astype(np.float32)
+    elif case == "far":  # coordinates ~400 mm from the origin
+        n = int(p.det[0])
+        p.det[12 * np.arange(n) + 1] += 400.0
+        p.coll[100 + 9 * np.arange(int(p.coll[10]))] += 400.0
+    return p
+
+
+@pytest.mark.parametrize("case", ["cfg1", "ragged", "tiny", "sizes", "far", "cfg2_300"])
+def test_lane_bitmap_walk(lane_lib, oracle_mod, params_mod, case):
+    """The bitmap kernel's walk (64-word runs, lanes on consecutive bits, row
+    terms kept across steps, FP32 test with the FP64 re-check) gives the
+    oracle's bitmaps bit for bit."""
+    p = _bitmap_geometry(params_mod, case)
+    o = oracle_mod.Oracle(p)
+    for coll, want in ((0, o.bitmap_crystal()), (1, o.bitmap_collimator())):
+        words = np.zeros(max(len(want), 1), np.uint32)
+        n = lane_lib.lane_bitmap(p.coll, p.det, p.img, p.phys, coll, words, len(words))
+        assert n == len(want), (case, coll, n, len(want))
+        assert np.array_equal(words[:n], want), (case, coll, int(np.sum(words[:n] != want)))
+        assert want.any()
