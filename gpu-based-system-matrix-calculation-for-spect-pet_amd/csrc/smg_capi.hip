@@ -85,6 +85,7 @@ struct smg_ctx {
     DevBuf<CellCtx> cell_tab_pts;  // sub-cell table of one chunk of crystal grid points
     int cell_tab_sub[3] = {0, 0, 0};
     uint64_t cell_tab_gen = 0, geom_gen = 0;
+    bool elliptic_holes = false;  // smg_set_hole_shapes made a hole elliptical
     MuTables mu_tab{};                 // host copy; n = 0: power laws
     DevBuf<MuTables> mu_tab_d;
     DevBuf<ScatterPt<double>> pts_grid;  // grid scatterers (one chunk)
@@ -526,7 +527,28 @@ smg_status smg_set_geometry(smg_ctx* c, const float* coll, size_t n_coll, const 
     c->hg = std::move(hg);
     c->have_geom = true;
     c->have_bm_crystal = c->have_bm_coll = false;
+    c->elliptic_holes = false;
     ++c->geom_gen;
+    return SMG_OK;
+}
+
+smg_status smg_set_hole_shapes(smg_ctx* c, const float* shapes, int n_holes) {
+    if (!c) return SMG_ERR_BAD_PARAM;
+    if (!c->have_geom) return fail(c, SMG_ERR_STATE, "no geometry set");
+    SMG_HIP(c, hipSetDevice(c->device));
+    HostGeom hg = c->hg;
+    std::string why;
+    int code = apply_hole_shapes(hg, shapes, n_holes, why);
+    if (code != 0) return fail(c, (smg_status)code, "%s", why.c_str());
+    smg_status s;
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    if ((s = upload(c, c->holes_f, hg.holes_f)) != SMG_OK) return s;
+    if ((s = upload(c, c->holes_d, hg.holes_d)) != SMG_OK) return s;
+    if ((s = upload(c, c->hole_groups, hg.hole_groups)) != SMG_OK) return s;
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    c->hg = std::move(hg);
+    c->elliptic_holes = false;
+    for (const Hole<double>& o : c->hg.holes_d) c->elliptic_holes |= o.erx > 0.0;
     return SMG_OK;
 }
 
@@ -763,6 +785,8 @@ smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm
     const HostGeom& hg = c->hg;
     const GeomDesc& g = hg.g;
     if (g.n_layers < 1) return fail(c, SMG_ERR_BAD_PARAM, "the collimator has no layer");
+    if (c->elliptic_holes)
+        return fail(c, SMG_ERR_BAD_PARAM, "collimator grid scatterers need circular holes");
     if (layer_mask == 0 || (g.n_layers < 32 && (layer_mask >> g.n_layers) != 0))
         return fail(c, SMG_ERR_BAD_PARAM, "layer mask 0x%x does not select layers of the %d-layer collimator",
                     layer_mask, g.n_layers);

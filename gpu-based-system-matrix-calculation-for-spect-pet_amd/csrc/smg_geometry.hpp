@@ -3,7 +3,7 @@
 // The reference reads every geometric quantity straight out of the raw float
 // parameter arrays inside each thread (PEGen_RayTracing_CircularHole/
 // PESysMatGen.cu:958-1075).  Here the host parses the arrays once into small
-// fixed-stride records (one 64-byte record per bar, 32 bytes per hole), in a
+// fixed-stride records (one 64-byte record per bar, one per hole), in a
 // float copy and an exact double copy, so a wave can fetch one record with a
 // single scalar load.
 #pragma once
@@ -36,6 +36,11 @@ struct Hole {
     T ymid;            // (y1 + y2) / 2
     T crit;            // sqrt(4 r^2 + (y1 - y2)^2) / 2: culling uses crit / D
     T mupe, muc;       // photoelectric and Compton mu of the filling
+    T erx, erz;        // elliptical hole: semi-axes along x' and z' (erx = 0: circle of radius r;
+                       // otherwise r = max(erx, erz), the bounding radius)
+    T ec, es;          // cos and sin of the turn theta of (x', z') from (x, z); 1, 0 for a circle
+    T ax2, az2;        // squared semi-axes (r^2, r^2 for a circle)
+    T dcut;            // discriminant cut of the chord's quadratic (see hole_chord)
 };
 
 template <class T>

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "smg_geometry.hpp"
+#include "smg_rays.hpp"
 
 namespace smg {
 
@@ -127,6 +128,37 @@ inline void make_coll_grid(const HostGeom& hg, double pitch, unsigned layer_mask
     }
 }
 
+// The culling radius and the FP32 copy of hole h's record.
+inline void hole_derived(HostGeom& hg, int h) {
+    Hole<double>& o = hg.holes_d[h];
+    o.crit = std::sqrt(o.r * o.r * 4 + (o.y1 - o.y2) * (o.y1 - o.y2)) / 2.0;
+    // hole_chord's quadratic in the ellipse's units: for a circle it is the
+    // circle's scaled by r^2, its discriminant by r^4, so the circle's cut
+    // b^2 - a c <= 1e-5 (PESysMatGen.cu:745) becomes 1e-5 r^4
+    const double ax = o.erx > 0.0 ? o.erx : o.r, az = o.erx > 0.0 ? o.erz : o.r;
+    o.ax2 = ax * ax;
+    o.az2 = az * az;
+    o.dcut = o.erx > 0.0 ? kCylDisc : kCylDisc * o.ax2 * o.az2;
+    Hole<float>& of = hg.holes_f[h];
+    of.xc = (float)o.xc;
+    of.y1 = (float)o.y1;
+    of.y2 = (float)o.y2;
+    of.zc = (float)o.zc;
+    of.r = (float)o.r;
+    of.mu = (float)o.mu;
+    of.mupe = (float)o.mupe;
+    of.muc = (float)o.muc;
+    of.ymid = (float)o.ymid;
+    of.crit = (float)o.crit;
+    of.erx = (float)o.erx;
+    of.erz = (float)o.erz;
+    of.ec = (float)o.ec;
+    of.es = (float)o.es;
+    of.ax2 = (float)o.ax2;
+    of.az2 = (float)o.az2;
+    of.dcut = (float)o.dcut;
+}
+
 // Parse the four parameter arrays the way the reference reads them.
 inline int parse_geometry(const float* coll, const float* det, const float* img, const float* phys,
                           HostGeom& hg, std::string& why) {
@@ -213,18 +245,9 @@ inline int parse_geometry(const float* coll, const float* det, const float* img,
         o.mupe = hp[6];
         o.muc = hp[7];
         o.ymid = (o.y1 + o.y2) / 2.0;
-        o.crit = std::sqrt(o.r * o.r * 4 + (o.y1 - o.y2) * (o.y1 - o.y2)) / 2.0;
-        Hole<float>& of = hg.holes_f[h];
-        of.xc = (float)o.xc;
-        of.y1 = (float)o.y1;
-        of.y2 = (float)o.y2;
-        of.zc = (float)o.zc;
-        of.r = (float)o.r;
-        of.mu = (float)o.mu;
-        of.mupe = (float)o.mupe;
-        of.muc = (float)o.muc;
-        of.ymid = (float)o.ymid;
-        of.crit = (float)o.crit;
+        o.erx = o.erz = o.es = 0.0;
+        o.ec = 1.0;
+        hole_derived(hg, h);
     }
 
     hg.bars_d.resize(g.n_det);
@@ -299,6 +322,44 @@ inline void make_scatter_records(const HostGeom& hg, std::vector<Sphere<float>>&
         ph.push_back(ScatterPt<double>{(double)hp[0], y + F, (double)hp[3],
                                        (double)hp[2] - (double)hp[1]});
     }
+}
+
+// Elliptical holes, an extension: the reference's elliptic-cylinder chord
+// (PESysMatGen.cu:801-947) has no caller and the parameter file has no fields
+// for it.  shapes holds x_rho, z_rho, theta (radians) per hole, n = the number
+// of holes; a hole with x_rho <= 0 is the parameter file's circle; NULL makes
+// every hole a circle again.  An elliptical hole's r is its bounding radius
+// max(x_rho, z_rho): the culling predicate (its crit) and the candidate scans
+// use it.  Returns 0, or kBadParam with the reason.
+inline int apply_hole_shapes(HostGeom& hg, const float* shapes, int n, std::string& why) {
+    const int nh = hg.g.n_holes;
+    if (shapes && n != nh)
+        return perr(why, kBadParam, "hole shapes given for %d holes, the geometry has %d", n, nh);
+    for (int h = 0; h < nh; ++h) {
+        const float* s = shapes ? shapes + 3 * h : nullptr;
+        if (s && s[0] > 0.0f && !(s[1] > 0.0f && std::isfinite(s[0]) && std::isfinite(s[1]) &&
+                                   std::isfinite(s[2])))
+            return perr(why, kBadParam, "hole %d: semi-axes must be positive and finite, theta finite",
+                        h);
+    }
+    for (int h = 0; h < nh; ++h) {
+        Hole<double>& o = hg.holes_d[h];
+        const float* s = shapes ? shapes + 3 * h : nullptr;
+        if (s && s[0] > 0.0f) {
+            o.erx = s[0];
+            o.erz = s[1];
+            o.ec = std::cos((double)s[2]);
+            o.es = std::sin((double)s[2]);
+            o.r = std::max(o.erx, o.erz);
+        } else {
+            o.erx = o.erz = o.es = 0.0;
+            o.ec = 1.0;
+            o.r = hg.coll[100 + 9 * h + 4];
+        }
+        hole_derived(hg, h);
+    }
+    make_group_boxes(hg);
+    return 0;
 }
 
 }  // namespace smg

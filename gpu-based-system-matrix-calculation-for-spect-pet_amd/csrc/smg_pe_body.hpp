@@ -512,7 +512,7 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
             int h = C.hole(j);
             if (h < first || h >= last || !C.hole_on(j)) continue;
             const Hole<double>& o = G.holes_d[h];
-            double l = cyl_chord(p, cr, dd, o.xc, o.y1, o.y2, o.zc, o.r);
+            double l = hole_chord(p, cr, dd, o);
             sum += l;
             att += o.mu * l;
         }

@@ -29,6 +29,8 @@ using std::fmin;
 
 #include <type_traits>
 
+#include "smg_geometry.hpp"
+
 namespace smg {
 
 template <class T>
@@ -428,39 +430,49 @@ SMG_FN void cyl_ray_from(const double D[3], double il, double inv_ky, CylRay& c)
     c.inv_qa = 1.0 / c.qa;
 }
 
-// Chord through the finite cylinder of radius r, axis parallel to y at
-// (xc, zc), spanning y1..y2; reference length_cylinder_ray
-// (PESysMatGen.cu:688-799).  Evaluated in FP64: rays through a hole run
-// almost along its axis, where kx^2 + kz^2 -> 0 and the quadratic's roots
-// lose every FP32 digit.
-SMG_FN double cyl_chord(const double a[3], const CylRay& c, double len, double xc, double y1,
-                        double y2, double zc, double r) {
-    if (fabs(y1 - y2) < kCylDy) return 0.0;
-    const double kx = c.kx, kz = c.kz, sxk = c.sxk, szk = c.szk;
-    const double a1 = y1 - a[1], a2 = y2 - a[1];
-    double xl = a[0] + sxk * a1 - xc;
-    double zl = a[2] + szk * a1 - zc;
-    double xr = a[0] + sxk * a2 - xc;
-    double zr = a[2] + szk * a2 - zc;
-    double t_lo = a1 * c.inv_ky;
-    double t_hi = a2 * c.inv_ky;
-    double r2 = r * r;
-    bool in_l = xl * xl + zl * zl <= r2;
-    bool in_r = xr * xr + zr * zr <= r2;
+// Chord through hole o: the finite elliptic cylinder ((x' - xc)/erx)^2 +
+// ((z' - zc)/erz)^2 <= 1, y1 <= y <= y2, where (x', z') is the point turned by
+// theta about the axis (cos, sin = ec, es); reference
+// length_ellipticalcylinder_ray (PESysMatGen.cu:801-947), a routine the
+// reference defines but never calls.  A circular hole is the case erx = erz =
+// r, theta = 0, and gives the circular chord (length_cylinder_ray,
+// PESysMatGen.cu:688-799): the same cases and thresholds,
+// with the quadratic a = kx'^2 erz^2 + kz'^2 erx^2, b = kx' x0' erz^2 +
+// kz' z0' erx^2, c = ... scaled by r^2 against the circle's, so the circle's
+// discriminant cut b^2 - a c <= 1e-5 is o.dcut = 1e-5 r^4 here and the
+// ellipse's own is 1e-5.  The discriminant is formed as erx^2 erz^2 (a -
+// (kx' z0' - kz' x0')^2), the value of b^2 - a c without its cancellation.
+// One code path for both shapes keeps the PE kernels' register budget.
+// Evaluated in FP64: rays through a hole run almost along its axis, where
+// kx^2 + kz^2 -> 0 and the quadratic's roots lose every FP32 digit.
+SMG_FN double hole_chord(const double a[3], const CylRay& c, double len, const Hole<double>& o) {
+    if (fabs(o.y1 - o.y2) < kCylDy) return 0.0;
+    // start point and direction in the ellipse's frame (the turn keeps len and ky)
+    const double dx = a[0] - o.xc, dz = a[2] - o.zc;
+    const double x0 = dx * o.ec + dz * o.es, z0 = -dx * o.es + dz * o.ec;
+    const double kx = c.kx * o.ec + c.kz * o.es, kz = -c.kx * o.es + c.kz * o.ec;
+    const double sxk = kx * c.inv_ky, szk = kz * c.inv_ky;
+    const double a1 = o.y1 - a[1], a2 = o.y2 - a[1];
+    const double xl = x0 + sxk * a1, zl = z0 + szk * a1;
+    const double xr = x0 + sxk * a2, zr = z0 + szk * a2;
+    const double t_lo = a1 * c.inv_ky, t_hi = a2 * c.inv_ky;
+    const double ax2 = o.ax2, az2 = o.az2;
+    const bool in_l = xl * xl * az2 + zl * zl * ax2 <= ax2 * az2;
+    const bool in_r = xr * xr * az2 + zr * zr * ax2 <= ax2 * az2;
     if (in_l && in_r) {
         if (t_lo <= kCylT || t_hi <= kCylT) return 0.0;
         if (t_lo >= len || t_hi >= len) return 0.0;
         return fabs(t_hi - t_lo);
     }
-    double x0 = a[0] - xc, z0 = a[2] - zc;
-    const double qa = c.qa;
-    double cr = kx * z0 - kz * x0;
-    double disc = qa * r2 - cr * cr;
-    if (disc <= kCylDisc) return 0.0;
-    double sq = sqrt(disc);
-    double qb = kx * x0 + kz * z0;
-    double t1 = (-qb - sq) * c.inv_qa;
-    double t2 = (-qb + sq) * c.inv_qa;
+    const double qa = kx * kx * az2 + kz * kz * ax2;
+    const double cr = kx * z0 - kz * x0;
+    const double disc = ax2 * az2 * (qa - cr * cr);
+    if (disc <= o.dcut) return 0.0;
+    const double sq = sqrt(disc);
+    const double qb = kx * x0 * az2 + kz * z0 * ax2;
+    const double iq = 1.0 / qa;
+    const double t1 = (-qb - sq) * iq;
+    const double t2 = (-qb + sq) * iq;
     if (!in_l && !in_r) {
         if (t1 >= t_lo && t2 <= t_hi) {
             if (t2 <= kCylT || t1 <= kCylT) return 0.0;

@@ -37,7 +37,7 @@ EXPORTS = (
     "smg_bitmap_crystal_words", "smg_bitmap_collimator_words", "smg_bitmap_crystal",
     "smg_bitmap_collimator", "smg_load_bitmap_crystal", "smg_load_bitmap_collimator",
     "smg_scatter_crystal", "smg_scatter_collimator", "smg_set_mu_tables",
-    "smg_scatter_collimator_grid", "smg_scatter_crystal_grid",
+    "smg_scatter_collimator_grid", "smg_scatter_crystal_grid", "smg_set_hole_shapes",
 )
 
 
@@ -94,6 +94,7 @@ def lib():
         "smg_set_pe_subsamples": (i32, [vp, i32, i32, i32]),
         "smg_set_scatter_subsamples": (i32, [vp, i32, i32, i32]),
         "smg_set_mu_tables": (i32, [vp, i32, dp, dp, dp, dp, dp]),
+        "smg_set_hole_shapes": (i32, [vp, fp, i32]),
         "smg_pe_forward": (i32, [vp, i32, i32, vp, i32, st]),
         "smg_pe_backward": (i32, [vp, i32, i32, vp, i32, dp, dp, st]),
         "smg_pe_project": (i32, [vp, i32, i32, vp, i32, vp, i32, st]),
@@ -215,6 +216,17 @@ class Context:
         if any(len(a) != n for a in arrs):
             raise ValueError("mu(E) tables must have one value per node")
         self._check(self._L.smg_set_mu_tables(self._h, n, *[_dptr(a) for a in arrs]))
+
+    def set_hole_shapes(self, shapes=None):
+        """Elliptical holes: shapes[h] = (x_rho, z_rho, theta) for every hole of
+        the geometry (x_rho <= 0 keeps hole h circular); None makes every hole
+        a circle.  A new geometry resets the shapes."""
+        if shapes is None:
+            self._check(self._L.smg_set_hole_shapes(self._h, None, 0))
+            return
+        s = np.ascontiguousarray(shapes, np.float32).reshape(-1, 3)
+        self._check(self._L.smg_set_hole_shapes(
+            self._h, s.ctypes.data_as(C.POINTER(C.c_float)), s.shape[0]))
 
     # ---- PE ----------------------------------------------------------------------
     def pe_forward(self, rotation: int = 0, precision: str = "fp32", out=None,

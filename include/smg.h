@@ -107,6 +107,22 @@ smg_status smg_set_mu_tables(smg_ctx* ctx, int n_nodes, const double* energy_kev
                              const double* crystal_pe, const double* crystal_compton,
                              const double* coll_pe, const double* coll_compton);
 
+/* Elliptical holes.  The reference defines the elliptic-cylinder chord
+ * length_ellipticalcylinder_ray (PEGen_RayTracing_CircularHole/PESysMatGen.cu:
+ * 801-947) but never calls it, and its parameter file has no fields for it;
+ * this entry point supplies them.  shapes: 3 floats per hole of the current
+ * geometry (n_holes = all layers' holes, in parameter order): x_rho, z_rho
+ * (semi-axes, mm) and theta (radians, the turn of the ellipse's axes about
+ * the hole axis, as the reference's routine takes it); a hole with x_rho <= 0
+ * keeps the parameter file's circle.  shapes = NULL makes every hole a circle.
+ * The PE matrix, its adjoint, the projections and the crystal grid weights
+ * trace elliptical holes with the reference routine's chord; culling and the
+ * candidate scans use the bounding radius max(x_rho, z_rho).  The collimator
+ * grid scatterers (smg_scatter_collimator_grid) need circular holes
+ * (SMG_ERR_BAD_PARAM otherwise).  smg_set_geometry resets every hole to a
+ * circle. */
+smg_status smg_set_hole_shapes(smg_ctx* ctx, const float* shapes, int n_holes);
+
 /* ---- photoelectric matrix (replaces PESysMatGen) ---------------------------
  * Writes the N x V block of rotation `rotation` to dst. */
 smg_status smg_pe_forward(smg_ctx* ctx, int rotation, smg_precision prec, float* dst,

@@ -47,6 +47,10 @@ def lib():
         getattr(L, name).argtypes = [_dp, _dp, _dp, _dp]
     L.smg_o_cylinder_ray.restype = C.c_double
     L.smg_o_cylinder_ray.argtypes = [_dp, _dp] + [C.c_double] * 5
+    L.smg_o_ellcyl_ray.restype = C.c_double
+    L.smg_o_ellcyl_ray.argtypes = [_dp, _dp] + [C.c_double] * 7
+    L.smg_o_set_hole_shapes.restype = None
+    L.smg_o_set_hole_shapes.argtypes = [C.c_void_p, C.c_void_p]
     L.smg_o_kn.restype = C.c_double
     L.smg_o_kn.argtypes = [C.c_double, C.c_double]
     L.smg_o_kn_sum.restype = C.c_double
@@ -134,6 +138,10 @@ def cylinder_ray(a, b, xc, y1, y2, zc, r) -> float:
     return lib().smg_o_cylinder_ray(_v3(a), _v3(b), xc, y1, y2, zc, r)
 
 
+def ellcyl_ray(a, b, xc, y1, y2, zc, x_rho, z_rho, theta) -> float:
+    return lib().smg_o_ellcyl_ray(_v3(a), _v3(b), xc, y1, y2, zc, x_rho, z_rho, theta)
+
+
 def seg_box(a, b, lo, hi) -> float:
     """Length of segment a -> b inside the box (no reference thresholds)."""
     return lib().smg_o_seg_box(_v3(a), _v3(b), _v3(lo), _v3(hi))
@@ -190,6 +198,16 @@ class Oracle:
         if g:
             lib().smg_o_geom_free(g)
             self._g = None
+
+    def set_hole_shapes(self, shapes=None):
+        """Elliptical holes: (x_rho, z_rho, theta) per hole, x_rho <= 0 keeps a
+        circle; None: all circles."""
+        if shapes is None:
+            self._shapes = None
+            lib().smg_o_set_hole_shapes(self._g, None)
+            return
+        self._shapes = np.ascontiguousarray(shapes, np.float32).reshape(-1)
+        lib().smg_o_set_hole_shapes(self._g, self._shapes.ctypes.data)
 
     def pe_element(self, d, v, rot=0, sub=(8, 16, 8)) -> float:
         return lib().smg_o_pe_element(self._g, d, v, rot, *sub)

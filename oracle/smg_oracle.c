@@ -55,6 +55,8 @@ struct smg_o_geom {
     int n_tab;
     double tab_le[SMG_O_MAX_NODES];
     double tab_lmu[4][SMG_O_MAX_NODES];
+    /* elliptical holes: x_rho, z_rho, theta per hole (NULL: all circles) */
+    const float* hole_shapes;
 };
 
 smg_o_geom* smg_o_geom_new(const float* coll, const float* det, const float* img,
@@ -306,6 +308,72 @@ double smg_o_cylinder_ray(const double a[3], const double b[3], double xc, doubl
     return tmax - t1;
 }
 
+/* Chord of the segment a -> b through the finite elliptic cylinder
+ * ((x'-xc)/x_rho)^2 + ((z'-zc)/z_rho)^2 <= 1, y1 <= y <= y2, with a and b
+ * first turned by theta about the axis (xc, zc): x' = xc + dx cos + dz sin,
+ * z' = zc - dx sin + dz cos (PE:801-947; the reference defines this routine
+ * but never calls it).  After the turn it follows the circular chord's cases
+ * with the ellipse's quadratic a = kx^2 z_rho^2 + kz^2 x_rho^2,
+ * b = kx x0 z_rho^2 + kz z0 x_rho^2, c = z_rho^2 x0^2 + x_rho^2 z0^2 -
+ * x_rho^2 z_rho^2 and the cut b^2 - a c <= 1e-5 (PE:887-893). */
+double smg_o_ellcyl_ray(const double a[3], const double b[3], double xc, double y1, double y2,
+                        double zc, double x_rho, double z_rho, double theta) {
+    double cs = cos(theta), sn = sin(theta);
+    double axr = a[0] - xc, azr = a[2] - zc, bxr = b[0] - xc, bzr = b[2] - zc;
+    double x_in = xc + axr * cs + azr * sn, z_in = zc - axr * sn + azr * cs;
+    double x_out = xc + bxr * cs + bzr * sn, z_out = zc - bxr * sn + bzr * cs;
+    double y_in = a[1], y_out = b[1];
+    if (fabs(y1 - y2) < CYL_DY) return 0.0;
+    double len = sqrt((x_out - x_in) * (x_out - x_in) + (y_out - y_in) * (y_out - y_in) +
+                      (z_out - z_in) * (z_out - z_in));
+    double kx = (x_out - x_in) / len, ky = (y_out - y_in) / len, kz = (z_out - z_in) / len;
+    double xl = x_in + kx / ky * (y1 - y_in), xr = x_in + kx / ky * (y2 - y_in);
+    double zl = z_in + kz / ky * (y1 - y_in), zr = z_in + kz / ky * (y2 - y_in);
+    double tmin = (y1 - y_in) / ky, tmax = (y2 - y_in) / ky;
+    int in_l = (xl - xc) * (xl - xc) / x_rho / x_rho + (zl - zc) * (zl - zc) / z_rho / z_rho <= 1;
+    int in_r = (xr - xc) * (xr - xc) / x_rho / x_rho + (zr - zc) * (zr - zc) / z_rho / z_rho <= 1;
+    if (in_l && in_r) {
+        if (tmin <= CYL_TEPS || tmax <= CYL_TEPS) return 0.0;
+        if (tmin >= len || tmax >= len) return 0.0;
+        return fabs(tmax - tmin);
+    }
+    double x0 = x_in - xc, z0 = z_in - zc;
+    double qa = kx * kx * z_rho * z_rho + kz * kz * x_rho * x_rho;
+    double qb = kx * x0 * z_rho * z_rho + kz * z0 * x_rho * x_rho;
+    double qc = z_rho * z_rho * x0 * x0 + x_rho * x_rho * z0 * z0 - x_rho * x_rho * z_rho * z_rho;
+    double disc = qb * qb - qa * qc;
+    if (disc <= CYL_DEPS) return 0.0;
+    double t1 = (-qb - sqrt(disc)) / qa;
+    double t2 = (-qb + sqrt(disc)) / qa;
+    if (!in_l && !in_r) {
+        if (t1 >= tmin && t2 <= tmax) {
+            if (t2 <= CYL_TEPS || t1 <= CYL_TEPS) return 0.0;
+            if (t2 >= len || t1 >= len) return 0.0;
+            return t2 - t1;
+        }
+        return 0.0;
+    }
+    if (in_l) {
+        if (t2 <= CYL_TEPS || tmin <= CYL_TEPS) return 0.0;
+        if (t2 >= len || tmin >= len) return 0.0;
+        return t2 - tmin;
+    }
+    if (tmax <= CYL_TEPS || t1 <= CYL_TEPS) return 0.0;
+    if (tmax >= len || t1 >= len) return 0.0;
+    return tmax - t1;
+}
+
+void smg_o_set_hole_shapes(smg_o_geom* g, const float* shapes) { g->hole_shapes = shapes; }
+
+/* Elliptical shape of hole h, or 0 for a circle. */
+static int hole_ellipse(const smg_o_geom* g, int h, double* xr, double* zr, double* th) {
+    if (!g->hole_shapes || !(g->hole_shapes[3 * h] > 0.0f)) return 0;
+    *xr = g->hole_shapes[3 * h];
+    *zr = g->hole_shapes[3 * h + 1];
+    *th = g->hole_shapes[3 * h + 2];
+    return 1;
+}
+
 /* ======================================================================== */
 /* Compton physics helpers (SC:961-1038)                                    */
 /* ======================================================================== */
@@ -482,6 +550,8 @@ static void pe_cull(const smg_o_geom* g, int d, const double p[3], cull_t* cl) {
             const float* hp = g->coll + 100 + 9 * h;
             double xc = hp[0], y1 = (double)hp[1] + g->F, y2 = (double)hp[2] + g->F, zc = hp[3];
             double R = hp[4];
+            double exr, ezr, eth;  /* an elliptical hole culls with its bounding radius */
+            if (hole_ellipse(g, h, &exr, &ezr, &eth)) R = exr > ezr ? exr : ezr;
             double yc = (y1 + y2) / 2.0;
             double crit = sqrt(R * R * 4 + (y1 - y2) * (y1 - y2)) / 2.0;
             double dh = sqrt((p[0] - xc) * (p[0] - xc) + (p[1] - yc) * (p[1] - yc) +
@@ -544,8 +614,12 @@ static double pe_ray_att(const smg_o_geom* g, const bar_t* bdp, const cull_t* cl
         for (int i = 0; i < cl->n_holes[m]; ++i) {
             const float* hp = g->coll + 100 + 9 * cl->holes[m][i];
             OPS(7);
-            double l = smg_o_cylinder_ray(p, q, hp[0], (double)hp[1] + g->F,
-                                          (double)hp[2] + g->F, hp[3], hp[4]);
+            double exr, ezr, eth;
+            double l = hole_ellipse(g, cl->holes[m][i], &exr, &ezr, &eth)
+                           ? smg_o_ellcyl_ray(p, q, hp[0], (double)hp[1] + g->F,
+                                              (double)hp[2] + g->F, hp[3], exr, ezr, eth)
+                           : smg_o_cylinder_ray(p, q, hp[0], (double)hp[1] + g->F,
+                                                (double)hp[2] + g->F, hp[3], hp[4]);
             sum += l;
             att += (double)hp[5] * l;
         }

@@ -43,6 +43,15 @@ double smg_o_box_ray_inside(const double a[3], const double b[3], const double l
                             const double hi[3]);
 double smg_o_cylinder_ray(const double a[3], const double b[3], double xc, double y1,
                           double y2, double zc, double radius);
+/* Elliptic-cylinder chord (PE:801-947, defined but never called by the
+ * reference): a and b turned by theta about the axis, then the ellipse
+ * ((x-xc)/x_rho)^2 + ((z-zc)/z_rho)^2 <= 1 between y1 and y2. */
+double smg_o_ellcyl_ray(const double a[3], const double b[3], double xc, double y1, double y2,
+                        double zc, double x_rho, double z_rho, double theta);
+/* Elliptical holes for the PE element: x_rho, z_rho, theta per hole (the
+ * array is not copied; NULL = all circles).  Culling uses the bounding radius
+ * max(x_rho, z_rho). */
+void smg_o_set_hole_shapes(smg_o_geom* g, const float* shapes);
 
 /* ---- Compton physics helpers ------------------------------------------- */
 double smg_o_kn(double theta, double e0);

@@ -64,6 +64,19 @@ def params_mod():
     return params
 
 
+def elliptic_hole_shapes(n_holes, seed=0, circles_every=4):
+    """(x_rho, z_rho, theta) per hole: ellipses of semi-axes 0.35-1.0 mm at
+    random turns, every circles_every-th hole left circular (x_rho = 0)."""
+    rng = np.random.default_rng(seed)
+    s = np.zeros((n_holes, 3), np.float32)
+    s[:, 0] = rng.uniform(0.5, 1.0, n_holes)
+    s[:, 1] = rng.uniform(0.35, 0.7, n_holes)
+    s[:, 2] = rng.uniform(-np.pi, np.pi, n_holes)
+    if circles_every:
+        s[::circles_every, 0] = 0.0
+    return s
+
+
 @pytest.fixture(scope="session")
 def lane_lib():
     """Host build of the GPU lane arithmetic (tests/lane_harness.cpp)."""
@@ -92,6 +105,8 @@ def lane_lib():
     L.lane_scatter_pairs.restype = C.c_long
     L.lane_scatter_pairs.argtypes = [fp, fp, fp, fp, fp, up] + [C.c_int] * 7 + [ip, ip, C.c_int, dp]
     L.lane_bitmap.restype = C.c_long
+    L.lane_set_hole_shapes.restype = None
+    L.lane_set_hole_shapes.argtypes = [fp, C.c_int]
     L.lane_bitmap.argtypes = [fp, fp, fp, fp, C.c_int, up, C.c_long]
     return L
 

@@ -34,9 +34,15 @@ struct Ctx {
     PeGeom G;
 };
 
+// Hole shapes applied to every geometry the harness builds (empty: circles).
+std::vector<float> g_hole_shapes;
+
 bool make_ctx(const float* coll, const float* det, const float* img, const float* phys, Ctx& c) {
     std::string why;
     if (parse_geometry(coll, det, img, phys, c.hg, why) != 0) return false;
+    if (!g_hole_shapes.empty() &&
+        apply_hole_shapes(c.hg, g_hole_shapes.data(), (int)g_hole_shapes.size() / 3, why) != 0)
+        return false;
     c.G.g = c.hg.g;
     c.G.bars_f = c.hg.bars_f.data();
     c.G.bars_d = c.hg.bars_d.data();
@@ -467,6 +473,12 @@ void lane_kn_sum_table(double e0, const float* lo, const float* hi, int n, doubl
         out[i] = kn_sum_table(knt.data(), lo[i], hi[i], nr);
         near[i] = nr;
     }
+}
+
+// Elliptical holes (x_rho, z_rho, theta per hole) for the following calls;
+// n = 0 restores circles.
+void lane_set_hole_shapes(const float* shapes, int n) {
+    g_hole_shapes.assign(shapes, shapes + 3 * (size_t)n);
 }
 
 // Crystal (collimator = 0) or collimator bitmap words by the GPU kernel's

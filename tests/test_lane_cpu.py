@@ -4,7 +4,8 @@ must stay within 1e-5, the FP64 path within float output rounding."""
 import numpy as np
 import pytest
 
-from conftest import TOL_FP32, TOL_FP64, assert_parity, narrow_plate_geometry, small_geometry
+from conftest import (TOL_FP32, TOL_FP64, assert_parity, elliptic_hole_shapes, narrow_plate_geometry,
+                      small_geometry)
 
 
 def lane_pe(L, p, d, v, sub, exact, rot=0):
@@ -159,3 +160,31 @@ def test_lane_bitmap_walk(lane_lib, oracle_mod, params_mod, case):
         assert n == len(want), (case, coll, n, len(want))
         assert np.array_equal(words[:n], want), (case, coll, int(np.sum(words[:n] != want)))
         assert want.any()
+
+
+@pytest.mark.parametrize("exact", [0, 1])
+def test_lane_pe_elliptic_holes(lane_lib, oracle_mod, params_mod, exact):
+    """The full-size geometry (400 holes on a 4 mm grid) with 300 of its holes
+    turned into ellipses (semi-axes 0.35-1.0 mm, random turns): the lane PE
+    elements (hole_chord, culling with the bounding radius) against the
+    oracle's elliptic-cylinder chord (PESysMatGen.cu:801-947)."""
+    p = params_mod.config(2)
+    o = oracle_mod.Oracle(p)
+    shapes = elliptic_hole_shapes(400, seed=3)
+    o.set_hole_shapes(shapes)
+    rng = np.random.default_rng(44)
+    d = rng.integers(0, o.n_det, 800)
+    v = rng.integers(0, o.n_vox, 800)
+    sub = p.meta["subsamples"]
+    lane_lib.lane_set_hole_shapes(shapes.reshape(-1), 400)
+    try:
+        got, fb = lane_pe(lane_lib, p, d, v, sub, exact)
+    finally:
+        lane_lib.lane_set_hole_shapes(np.zeros(3, np.float32), 0)
+    want = o.pe_pairs(d, v, sub=sub)
+    o.set_hole_shapes(None)
+    circ = o.pe_pairs(d, v, sub=sub)
+    # the ellipses change the matrix: the test is not passing on circles
+    assert np.max(np.abs(want - circ) / np.maximum(np.abs(circ), 1e-30)) > 0.05
+    assert_parity(got, want, TOL_FP64 if exact else TOL_FP32,
+                  f"lane PE elliptic holes exact={exact} fallbacks={fb}")

@@ -6,6 +6,7 @@ References restated by the oracle:
   length_box_ray          PEGen_RayTracing_CircularHole/PESysMatGen.cu:20-312
   length_box_ray_inside   PESysMatGen.cu:314-686
   length_cylinder_ray     PESysMatGen.cu:688-799
+  length_ellipticalcylinder_ray   PESysMatGen.cu:801-947 (never called by the reference)
   KN / KN sum / cone angle / E' / window   ScatterGen_RayTracing_CircularHole/scatter.cu:961-1038
 """
 import math
@@ -169,6 +170,120 @@ def test_cylinder_wall_crossings_brute_force(oracle_mod):
         assert got == pytest.approx(ref, abs=3 * step + 1e-9)
         n_wall += 0.0 < got < 6.0 * np.linalg.norm(b - a) / 40.0 - 1e-6
     assert n_wall > 5
+
+
+def ell_chord_brute(a, b, xc, y1, y2, zc, xr, zr, theta, n=200001):
+    """Length of the segment inside the turned elliptic cylinder, by sampling."""
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    s = np.linspace(0.0, 1.0, n)
+    pts = a[None, :] + s[:, None] * (b - a)[None, :]
+    dx, dz = pts[:, 0] - xc, pts[:, 2] - zc
+    xp = dx * math.cos(theta) + dz * math.sin(theta)
+    zp = -dx * math.sin(theta) + dz * math.cos(theta)
+    inside = ((xp / xr) ** 2 + (zp / zr) ** 2 <= 1.0) & (pts[:, 1] >= y1) & (pts[:, 1] <= y2)
+    return inside.mean() * np.linalg.norm(b - a), np.linalg.norm(b - a) / (n - 1)
+
+
+def test_ellcyl_brute_force(oracle_mod):
+    """Elliptic-cylinder chord (PESysMatGen.cu:801-947) of segments whose ends
+    lie beyond both caps, against dense sampling of the segment: wall
+    crossings, cap crossings and misses, turned ellipses of aspect up to 3.
+    Semi-axes and obliquity are such that the discriminant cut (next test)
+    only acts next to tangency."""
+    rng = np.random.default_rng(11)
+    kinds = {"wall": 0, "caps": 0, "miss": 0}
+    for _ in range(200):
+        xc, zc = rng.uniform(-1, 1), rng.uniform(-1, 1)
+        xr, zr = rng.uniform(0.8, 2.5), rng.uniform(0.8, 2.5)
+        th = rng.uniform(-math.pi, math.pi)
+        a = np.array([xc + rng.uniform(-5, 5), -10.0, zc + rng.uniform(-5, 5)])
+        b = np.array([xc + rng.uniform(-5, 5), 30.0, zc + rng.uniform(-5, 5)])
+        got = oracle_mod.ellcyl_ray(a, b, xc, 8.0, 14.0, zc, xr, zr, th)
+        ref, step = ell_chord_brute(a, b, xc, 8.0, 14.0, zc, xr, zr, th)
+        assert got == pytest.approx(ref, abs=3 * step + 1e-9), (xc, zc, xr, zr, th)
+        full = 6.0 * np.linalg.norm(b - a) / 40.0
+        kinds["miss" if got == 0.0 else ("caps" if abs(got - full) < 1e-9 else "wall")] += 1
+    assert min(kinds.values()) > 5, kinds
+
+
+def test_ellcyl_discriminant_cut_in_ellipse_units(oracle_mod):
+    """The routine cuts b^2 - a c <= 1e-5 with its quadratic a = kx^2 z_rho^2 +
+    kz^2 x_rho^2, ... (PESysMatGen.cu:887-893), which is x_rho^2 z_rho^2 times
+    the unit-circle form: a ray that crosses the wall of a small ellipse almost
+    along its axis gets 0.  The decision follows the discriminant formed
+    independently here (in the turned frame, Lagrange form)."""
+    rng = np.random.default_rng(13)
+    n_cut = n_kept = 0
+    for i in range(600):
+        xc, zc = rng.uniform(-1, 1), rng.uniform(-1, 1)
+        xr, zr, th = rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6), rng.uniform(-3, 3)
+        w = 1.0 if i % 2 else 4.0  # near-axis and oblique rays
+        a = np.array([xc + rng.uniform(-w, w), -10.0, zc + rng.uniform(-w, w)])
+        b = np.array([xc + rng.uniform(-w, w), 30.0, zc + rng.uniform(-w, w)])
+        got = oracle_mod.ellcyl_ray(a, b, xc, 8.0, 14.0, zc, xr, zr, th)
+        ref, step = ell_chord_brute(a, b, xc, 8.0, 14.0, zc, xr, zr, th)
+        full = 6.0 * np.linalg.norm(b - a) / 40.0
+        if ref == 0.0 or abs(ref - full) < 3 * step:
+            continue  # misses and cap-to-cap rays do not reach the quadratic
+        c, s = math.cos(th), math.sin(th)
+        d = (b - a) / np.linalg.norm(b - a)
+        kx, kz = d[0] * c + d[2] * s, -d[0] * s + d[2] * c
+        x0 = (a[0] - xc) * c + (a[2] - zc) * s
+        z0 = -(a[0] - xc) * s + (a[2] - zc) * c
+        disc = xr * xr * zr * zr * (kx * kx * zr * zr + kz * kz * xr * xr - (kx * z0 - kz * x0) ** 2)
+        if abs(disc - 1e-5) < 1e-9:
+            continue
+        if disc <= 1e-5:
+            assert got == 0.0
+            n_cut += 1
+        else:
+            assert got == pytest.approx(ref, abs=3 * step + 1e-9)
+            n_kept += 1
+    assert n_cut > 5 and n_kept > 5, (n_cut, n_kept)
+
+
+def test_ellcyl_circle_and_axis_swap(oracle_mod):
+    """x_rho = z_rho = 1, theta = 0 is the circular chord, bit for bit up to
+    rounding, including rays next to tangency (the discriminant cut of both is
+    1e-5 when r = 1); any theta turns a circle onto itself; (x_rho, z_rho,
+    theta) and (z_rho, x_rho, theta + pi/2) are the same ellipse."""
+    rng = np.random.default_rng(12)
+    n_tangent = 0
+    for i in range(400):
+        xc, zc = rng.uniform(-1, 1), rng.uniform(-1, 1)
+        a = np.array([xc + rng.uniform(-2, 2), -10.0, zc + rng.uniform(-2, 2)])
+        b = np.array([xc + rng.uniform(-2, 2), 30.0, zc + rng.uniform(-2, 2)])
+        if i % 4 == 0:  # near tangency: the line passes at distance 1 - tiny
+            d = np.array([b[0] - a[0], 0.0, b[2] - a[2]])
+            nrm = np.array([-d[2], 0.0, d[0]]) / np.hypot(d[0], d[2])
+            off = (1.0 - 10.0 ** rng.uniform(-9, -4)) * nrm
+            mid = np.array([xc, 11.0, zc]) + off
+            a, b = mid - 0.5 * (b - a), mid + 0.5 * (b - a)
+            n_tangent += 1
+        c = oracle_mod.cylinder_ray(a, b, xc, 8.0, 14.0, zc, 1.0)
+        e = oracle_mod.ellcyl_ray(a, b, xc, 8.0, 14.0, zc, 1.0, 1.0, 0.0)
+        assert e == pytest.approx(c, rel=1e-12, abs=1e-12)
+        if c > 1e-3:
+            th = rng.uniform(-3, 3)
+            assert oracle_mod.ellcyl_ray(a, b, xc, 8.0, 14.0, zc, 1.0, 1.0, th) == pytest.approx(c, rel=1e-9)
+        xr, zr, th = rng.uniform(0.3, 1.2), rng.uniform(0.3, 1.2), rng.uniform(-3, 3)
+        e1 = oracle_mod.ellcyl_ray(a, b, xc, 8.0, 14.0, zc, xr, zr, th)
+        e2 = oracle_mod.ellcyl_ray(a, b, xc, 8.0, 14.0, zc, zr, xr, th + math.pi / 2)
+        if min(e1, e2) > 1e-3 or max(e1, e2) == 0.0:
+            assert e1 == pytest.approx(e2, rel=1e-9)
+    assert n_tangent == 100
+
+
+def test_ellcyl_through_both_caps(oracle_mod):
+    """A ray through both end faces of a turned ellipse: (y2 - y1)/|k_y|."""
+    a, b = np.array([0.1, -20.0, -0.2]), np.array([0.5, 40.0, 0.3])
+    k = (b - a) / np.linalg.norm(b - a)
+    got = oracle_mod.ellcyl_ray(a, b, 0.3, 10.0, 15.0, 0.05, 0.8, 0.3, 0.4)
+    assert got == pytest.approx(5.0 / k[1], rel=1e-13)
+    # the same line misses the ellipse turned by pi/2 (minor axis across it)
+    a2, b2 = np.array([0.85, -20.0, 0.0]), np.array([0.85, 40.0, 0.0])
+    assert oracle_mod.ellcyl_ray(a2, b2, 0.3, 10.0, 15.0, 0.0, 0.8, 0.3, 0.0) > 0.0
+    assert oracle_mod.ellcyl_ray(a2, b2, 0.3, 10.0, 15.0, 0.0, 0.8, 0.3, math.pi / 2) == 0.0
 
 
 def test_kn_textbook(oracle_mod):
