@@ -98,11 +98,42 @@ def timed(fn, reps):
     return ks[len(ks) // 2]
 
 
-def extras(smg, params, pe_ctx, out, precision):
+WORKLOADS = ("crystal_scatter", "collimator_scatter", "collimator_grid", "crystal_grid",
+             "bitmaps", "adjoint")
+
+
+def extras(smg, params, pe_ctx, out, precision, sections=None, precs=("fp32", "fp64")):
     """Secondary measurements: FP64 vs FP32 on the headline step, scatter
-    throughputs, adjoint cost."""
+    throughputs, bitmaps, adjoint cost.  sections: the WORKLOADS to run (all,
+    plus the ones on the headline context, when None); precs: the precisions
+    of the scatter workloads."""
     import numpy as np
 
+    def want(name):
+        return sections is None or name in sections
+
+    ex = {}
+    if pe_ctx is not None:
+        ex.update(_headline_extras(np, pe_ctx, out, precision))
+    if want("crystal_scatter"):
+        _crystal_scatter(np, smg, params, ex, precs)
+    if want("collimator_scatter"):
+        _collimator_scatter(np, smg, params, ex, precs)
+    if want("collimator_grid"):
+        _collimator_grid(np, smg, params, ex, precs)
+    if want("crystal_grid"):
+        _crystal_grid(np, smg, params, ex, precs)
+    if want("bitmaps"):
+        _bitmaps(np, smg, params, ex)
+    if want("adjoint"):
+        _adjoint(np, smg, params, ex)
+    if pe_ctx is not None:
+        ms_this = ex["projection"]["matrix_kernel_ms"]
+        _adjoint_full_size(np, pe_ctx, ms_this, ex)
+    return ex
+
+
+def _headline_extras(np, pe_ctx, out, precision):
     ex = {}
     other = "fp64" if precision == "fp32" else "fp32"
     pe_ctx.pe_forward(0, other, device_ptr=out.data_ptr())
@@ -125,7 +156,10 @@ def extras(smg, params, pe_ctx, out, precision):
                         "workload": "y = A x and x = A^T y with the full-size PE matrix, matrix-free "
                                     "(no 1 GiB matrix in HBM or on disk)"}
     log(f"projection: forward {f_ms:.1f} ms, back {b_ms:.1f} ms (matrix {ms_this:.1f} ms)")
+    return ex
 
+
+def _crystal_scatter(np, smg, params, ex, precs):
     # crystal scatter (ScatterGen_Crystal path): 32x32x1 voxels, 2048 bars, 2047 scatter points
     p4 = params.config(4)
     with smg.open_context(p4) as c:
@@ -133,7 +167,7 @@ def extras(smg, params, pe_ctx, out, precision):
         pe, _ = c.pe_forward(0, "fp32")
         c.set_scatter_subsamples(*p4.meta["subsamples"])
         c.scatter_crystal(pe, 0, "fp32")  # builds the bitmap, warms up
-        for prec in ("fp32", "fp64"):
+        for prec in precs:
             _, st = c.scatter_crystal(pe, 0, prec)
             ms = timed(lambda: c.scatter_crystal(pe, 0, prec), 3 if prec == "fp32" else 1)
             ex[f"crystal_scatter_{prec}"] = {
@@ -144,12 +178,14 @@ def extras(smg, params, pe_ctx, out, precision):
         ex["crystal_scatter_workload"] = ("32x32x1 voxels x 2048 bars, scatter points at the "
                                           "2047 other bar centres (reference semantics), 1x1x1 cells")
 
+
+def _collimator_scatter(np, smg, params, ex, precs):
     # collimator scatter (combined variant): 32x32x16 voxels, 64 bars, 16 hole centres
     p3 = params.config(3)
     with smg.open_context(p3) as c:
         c.set_scatter_subsamples(*p3.meta["subsamples"])
         c.scatter_collimator(0, "combined", "fp32")
-        for prec in ("fp32", "fp64"):
+        for prec in precs:
             _, st = c.scatter_collimator(0, "combined", prec)
             ms = timed(lambda: c.scatter_collimator(0, "combined", prec), 3)
             ex[f"collimator_scatter_{prec}"] = {
@@ -160,10 +196,13 @@ def extras(smg, params, pe_ctx, out, precision):
         ex["collimator_scatter_workload"] = ("32x32x16 voxels x 64 bars, scatter points at the 16 "
                                              "hole centres (reference semantics), 1x1x1 cells")
 
+
+def _collimator_grid(np, smg, params, ex, precs):
     # collimator scatter from a 0.5 mm grid over the tungsten (first- and second-leg
     # collimator attenuation), 32x32x16 voxels, 64 bars
+    p3 = params.config(3)
     with smg.open_context(p3) as c:
-        for prec in ("fp32", "fp64"):
+        for prec in precs:
             _, st, n = c.scatter_collimator_grid(0.5, 0, prec)
             ex[f"collimator_grid_scatter_{prec}"] = {
                 "elements_per_s": round(st["elements"] / st["kernel_ms"] * 1e3, 1),
@@ -174,11 +213,14 @@ def extras(smg, params, pe_ctx, out, precision):
                                                   "a 0.5 mm grid over the 60x5x60 mm tungsten, "
                                                   "1x1x1 cells, power-law mu(E)")
 
+
+def _crystal_grid(np, smg, params, ex, precs):
     # crystal scatter from a 0.25 mm grid in every bar (10x32x10 cells of the 2.5x8x2.5 mm
     # bars), 32x32x1 voxels, 2048 bars, 8 detector rows (every 256th bar)
+    p4 = params.config(4)
     rows = np.arange(0, p4.n_det, 256, dtype=np.int32)
     with smg.open_context(p4) as c:
-        for prec in ("fp32", "fp64"):
+        for prec in precs:
             _, st = c.scatter_crystal_grid((10, 32, 10), rows, 0, prec)
             ex[f"crystal_grid_scatter_{prec}"] = {
                 "elements_per_s": round(st["elements"] / st["kernel_ms"] * 1e3, 1),
@@ -190,6 +232,8 @@ def extras(smg, params, pe_ctx, out, precision):
             f"centres of 10x32x10 cells (0.25 mm) of every bar ({p4.n_det * 3200} points), "
             "1x1x1 cells, power-law mu(E)")
 
+
+def _bitmaps(np, smg, params, ex):
     # geometry bitmaps of the scatter kernels: the full-size detector and 6,000 bars
     for tag, pb in (("2048_bars", params.config(2)), ("6000_bars", params.large_detector())):
         with smg.open_context(pb) as c:
@@ -208,6 +252,8 @@ def extras(smg, params, pe_ctx, out, precision):
             "workload": f"{n} bars, 400 layer-0 holes; wall time of the call (kernel + sync)"}
         log(f"bitmaps {tag}: crystal {ms_c:.1f} ms, collimator {ms_l:.1f} ms")
 
+
+def _adjoint(np, smg, params, ex):
     # adjoint: forward + dL/dmu on the 32x32x1-voxel, 64-bar geometry, 4x4x4 sub-cells
     p5 = params.config(5)
     w = np.random.default_rng(2).uniform(0, 1, (p5.n_det, p5.n_vox)).astype(np.float32)
@@ -223,14 +269,15 @@ def extras(smg, params, pe_ctx, out, precision):
                      "workload": "32x32x1 voxels x 64 bars, 4x4x4 sub-cells, 1 + 128 mu"}
     log(f"adjoint: forward {f_ms:.3f} ms, backward {b_ms:.3f} ms")
 
-    # the same on the headline geometry (2048 bars, 64x64x32 voxels, 1 + 4096 mu)
+
+def _adjoint_full_size(np, pe_ctx, ms_this, ex):
+    # the adjoint on the headline geometry (2048 bars, 64x64x32 voxels, 1 + 4096 mu)
     wf = np.random.default_rng(2).uniform(0, 1, (pe_ctx.dims.n_det, pe_ctx.dims.n_vox)).astype(np.float32)
     bf_ms = timed(lambda: pe_ctx.pe_backward(wf)[1:], 1)
     ex["adjoint_full_size"] = {"forward_kernel_ms": round(ms_this, 2), "backward_kernel_ms": round(bf_ms, 2),
                                "backward_over_forward": round(bf_ms / ms_this, 2),
                                "workload": "64x64x32 voxels x 2048 bars, 3x3x3 sub-cells, 1 + 4096 mu"}
     log(f"adjoint, full size: backward {bf_ms:.1f} ms (forward {ms_this:.1f} ms)")
-    return ex
 
 
 def main():
@@ -243,6 +290,11 @@ def main():
                     help="host time budget of the CPU-baseline sample (0 to skip)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the FP64, scatter and adjoint measurements")
+    ap.add_argument("--workloads", default=None,
+                    help="only run these secondary workloads (comma list of "
+                         + ", ".join(WORKLOADS) + ") and print their timings as one JSON "
+                         "line; with --precision, the scatter workloads run in that "
+                         "precision only (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -266,6 +318,16 @@ def main():
 
     import smg
     from smg import params
+
+    if args.workloads:
+        names = [w for w in args.workloads.split(",") if w]
+        unknown = [w for w in names if w not in WORKLOADS]
+        if unknown:
+            print(f"bench.py: unknown workloads {unknown}; choose from {WORKLOADS}", file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps(extras(smg, params, None, None, args.precision, sections=names,
+                                precs=(args.precision,))))
+        return
 
     p = params.config(2)
     sub = p.meta["subsamples"]

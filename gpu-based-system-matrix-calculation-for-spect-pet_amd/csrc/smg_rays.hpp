@@ -447,11 +447,20 @@ SMG_FN void cyl_ray_from(const double D[3], double il, double inv_ky, CylRay& c)
 // kx^2 + kz^2 -> 0 and the quadratic's roots lose every FP32 digit.
 SMG_FN double hole_chord(const double a[3], const CylRay& c, double len, const Hole<double>& o) {
     if (fabs(o.y1 - o.y2) < kCylDy) return 0.0;
-    // start point and direction in the ellipse's frame (the turn keeps len and ky)
-    const double dx = a[0] - o.xc, dz = a[2] - o.zc;
-    const double x0 = dx * o.ec + dz * o.es, z0 = -dx * o.es + dz * o.ec;
-    const double kx = c.kx * o.ec + c.kz * o.es, kz = -c.kx * o.es + c.kz * o.ec;
-    const double sxk = kx * c.inv_ky, szk = kz * c.inv_ky;
+    // start point and direction in the ellipse's frame (the turn keeps len and
+    // ky); no turn for circles (the record is the same for the whole wave, so
+    // this is a branch, not a select)
+    double x0 = a[0] - o.xc, z0 = a[2] - o.zc;
+    double kx = c.kx, kz = c.kz, sxk = c.sxk, szk = c.szk;
+    if (o.es != 0.0) {
+        const double dx = x0, dz = z0;
+        x0 = dx * o.ec + dz * o.es;
+        z0 = -dx * o.es + dz * o.ec;
+        kx = c.kx * o.ec + c.kz * o.es;
+        kz = -c.kx * o.es + c.kz * o.ec;
+        sxk = kx * c.inv_ky;
+        szk = kz * c.inv_ky;
+    }
     const double a1 = o.y1 - a[1], a2 = o.y2 - a[1];
     const double xl = x0 + sxk * a1, zl = z0 + szk * a1;
     const double xr = x0 + sxk * a2, zr = z0 + szk * a2;
