@@ -160,6 +160,13 @@ double now_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// The PE-family kernels run in FP64 when asked, and whenever a hole is
+// elliptical: only the all-FP64 kernels carry the elliptic-cylinder chord,
+// which keeps its code and registers out of the FP32 kernels.
+bool pe_exact(const smg_ctx* c, smg_precision prec) {
+    return prec == SMG_PREC_FP64 || c->elliptic_holes;
+}
+
 smg_status check_ready(smg_ctx* c, int rotation) {
     if (!c) return SMG_ERR_BAD_PARAM;
     if (!c->have_geom) return fail(c, SMG_ERR_STATE, "no geometry set");
@@ -637,7 +644,7 @@ smg_status smg_pe_forward(smg_ctx* c, int rotation, smg_precision prec, float* d
     SMG_HIP(c, pe_cell_table(c, A));
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
-    SMG_HIP(c, launch_pe_forward(A, prec == SMG_PREC_FP64, c->stream));
+    SMG_HIP(c, launch_pe_forward(A, pe_exact(c, prec), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long cnt[4] = {0, 0, 0, 0};
     SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
@@ -696,9 +703,9 @@ static smg_status project_entry(smg_ctx* c, int rotation, smg_precision prec, bo
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
     if (back)
-        SMG_HIP(c, launch_pe_backproject(A, prec == SMG_PREC_FP64, in_d, c->proj.ptr, out_d, c->stream));
+        SMG_HIP(c, launch_pe_backproject(A, pe_exact(c, prec), in_d, c->proj.ptr, out_d, c->stream));
     else
-        SMG_HIP(c, launch_pe_project(A, prec == SMG_PREC_FP64, in_d, c->proj.ptr, out_d, c->stream));
+        SMG_HIP(c, launch_pe_project(A, pe_exact(c, prec), in_d, c->proj.ptr, out_d, c->stream));
     SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long cnt[4] = {0, 0, 0, 0};
     SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
@@ -983,7 +990,7 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
             SMG_HIP(c, launch_cell_table(P.geom, b0, nb, P.sub, c->cell_tab_pts.ptr, c->stream));
             P.cells = c->cell_tab_pts.ptr;
         }
-        SMG_HIP(c, launch_pe_points(P, prec == SMG_PREC_FP64, b0, nb, c->stream));
+        SMG_HIP(c, launch_pe_points(P, pe_exact(c, prec), b0, nb, c->stream));
         A.n_pts = nb * npb;
         A.pt_bar0 = b0;
         A.accumulate = b0 > 0;
@@ -1052,7 +1059,7 @@ smg_status smg_pe_backward(smg_ctx* c, int rotation, smg_precision prec, const f
     }
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
-    SMG_HIP(c, launch_pe_backward(A, S, prec == SMG_PREC_FP64, c->grad.ptr, c->stream));
+    SMG_HIP(c, launch_pe_backward(A, S, pe_exact(c, prec), c->grad.ptr, c->stream));
     SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
     std::vector<double> out((size_t)n_out);
     unsigned long long cnt[4] = {0, 0, 0, 0};

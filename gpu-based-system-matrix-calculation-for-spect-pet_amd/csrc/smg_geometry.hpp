@@ -38,9 +38,7 @@ struct Hole {
     T mupe, muc;       // photoelectric and Compton mu of the filling
     T erx, erz;        // elliptical hole: semi-axes along x' and z' (erx = 0: circle of radius r;
                        // otherwise r = max(erx, erz), the bounding radius)
-    T ec, es;          // cos and sin of the turn theta of (x', z') from (x, z); 1, 0 for a circle
-    T ax2, az2;        // squared semi-axes (r^2, r^2 for a circle)
-    T dcut;            // discriminant cut of the chord's quadratic (see hole_chord)
+    T ec, es;          // cos and sin of the turn theta of (x', z') from (x, z)
 };
 
 template <class T>

@@ -16,7 +16,6 @@
 #include <vector>
 
 #include "smg_geometry.hpp"
-#include "smg_rays.hpp"
 
 namespace smg {
 
@@ -132,13 +131,6 @@ inline void make_coll_grid(const HostGeom& hg, double pitch, unsigned layer_mask
 inline void hole_derived(HostGeom& hg, int h) {
     Hole<double>& o = hg.holes_d[h];
     o.crit = std::sqrt(o.r * o.r * 4 + (o.y1 - o.y2) * (o.y1 - o.y2)) / 2.0;
-    // hole_chord's quadratic in the ellipse's units: for a circle it is the
-    // circle's scaled by r^2, its discriminant by r^4, so the circle's cut
-    // b^2 - a c <= 1e-5 (PESysMatGen.cu:745) becomes 1e-5 r^4
-    const double ax = o.erx > 0.0 ? o.erx : o.r, az = o.erx > 0.0 ? o.erz : o.r;
-    o.ax2 = ax * ax;
-    o.az2 = az * az;
-    o.dcut = o.erx > 0.0 ? kCylDisc : kCylDisc * o.ax2 * o.az2;
     Hole<float>& of = hg.holes_f[h];
     of.xc = (float)o.xc;
     of.y1 = (float)o.y1;
@@ -154,9 +146,6 @@ inline void hole_derived(HostGeom& hg, int h) {
     of.erz = (float)o.erz;
     of.ec = (float)o.ec;
     of.es = (float)o.es;
-    of.ax2 = (float)o.ax2;
-    of.az2 = (float)o.az2;
-    of.dcut = (float)o.dcut;
 }
 
 // Parse the four parameter arrays the way the reference reads them.

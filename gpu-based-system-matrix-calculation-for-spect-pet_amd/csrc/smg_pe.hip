@@ -351,12 +351,12 @@ __device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
                 LdsCands C{&L, nsb, nsh, lane};
                 if constexpr (kPoints) {
                     double aux[8];
-                    pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near, nullptr, aux);
+                    pe_cell_ray<T, kExact>(G, d, W.p, W.ps, cc, C, near, nullptr, aux);
                     if (!near)
                         pts[(int64_t)cell * g.n_vox + W.v] =
                             kExact ? point_weight(bd, n_cells, aux) : point_weight_f(bd, n_cells, aux);
                 } else {
-                    T val = pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near);
+                    T val = pe_cell_ray<T, kExact>(G, d, W.p, W.ps, cc, C, near);
                     if (!near) acc.add(val);
                 }
             }
@@ -384,10 +384,10 @@ __device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
                     bool n2 = false;
                     if constexpr (kPoints) {
                         double aux[8];
-                        pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2, nullptr, aux);
+                        pe_cell_ray<double, kExact>(G, d, W.p, W.ps, cc, C, n2, nullptr, aux);
                         pts[(int64_t)redo_cell[i] * g.n_vox + W.v] = point_weight(bd, n_cells, aux);
                     } else {
-                        acc.add((T)pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2));
+                        acc.add((T)pe_cell_ray<double, kExact>(G, d, W.p, W.ps, cc, C, n2));
                     }
                     ++fallbacks;
                 }
@@ -450,7 +450,7 @@ struct AdjAcc {
 // FP32 a lane whose ray came within tolerance of a threshold contributes
 // nothing and is reported through the return value (the caller redoes it in
 // FP64).  Every lane of the wave takes part in the wave sums.
-template <class T>
+template <class T, bool kEll>
 __device__ bool adjoint_cell(const PeGeom& G, int d, int lane, const WaveSetup& W,
                              const WaveLists& L, const LdsCands& C, const CellCtx& cc,
                              bool active, double wv, AdjAcc& acc, double* lay_acc,
@@ -463,7 +463,7 @@ __device__ bool adjoint_cell(const PeGeom& G, int d, int lane, const WaveSetup& 
     for (int m = 0; m < kMaxLayers; ++m) dlay[m] = 0.0;
     bool near = false;
     if (active) {
-        term = (double)pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near, dlay, aux, keep, 64);
+        term = (double)pe_cell_ray<T, kEll>(G, d, W.p, W.ps, cc, C, near, dlay, aux, keep, 64);
         if (kGuarded<T> && near) active = false;
     }
     const double cw = active ? wv * term : 0.0;
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                     narrow_candidates(A, cc.q, W, lane, L, nb, nh, nsb, nsh);
                     __syncthreads();
                     LdsCands C{&L, nsb, nsh, lane};
-                    const bool near = adjoint_cell<T>(G, d, lane, W, L, C, cc, W.valid, wv, acc,
+                    const bool near = adjoint_cell<T, kExact>(G, d, lane, W, L, C, cc, W.valid, wv, acc,
                                                       lay_acc, cand_acc, kExact ? nullptr : keep + lane);
                     if constexpr (!kExact) {
                         const uint64_t m = __ballot(near);
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                         __syncthreads();
                         LdsCands C{&L, rsb, rsh, lane};
                         const bool mine = (redo_mask[i] >> lane) & 1ull;
-                        adjoint_cell<double>(G, d, lane, W, L, C, cc, mine, wv, acc, lay_acc, cand_acc);
+                        adjoint_cell<double, kExact>(G, d, lane, W, L, C, cc, mine, wv, acc, lay_acc, cand_acc);
                         fallbacks += mine;
                     }
                 }
@@ -689,7 +689,7 @@ __device__ double pe_element(const PeLaunch& A, int d, int lane, const WaveSetup
             bool near = false;
             if (W.valid) {
                 LdsCands C{&L, nsb, nsh, lane};
-                T val = pe_cell_ray<T>(G, d, W.p, W.ps, cc, C, near);
+                T val = pe_cell_ray<T, kExact>(G, d, W.p, W.ps, cc, C, near);
                 if (!near) acc.add(val);
             }
             if constexpr (!kExact) {
@@ -714,7 +714,7 @@ __device__ double pe_element(const PeLaunch& A, int d, int lane, const WaveSetup
                 if ((redo_mask[i] >> lane) & 1ull) {
                     LdsCands C{&L, rsb, rsh, lane};
                     bool n2 = false;
-                    acc.add((T)pe_cell_ray<double>(G, d, W.p, W.ps, cc, C, n2));
+                    acc.add((T)pe_cell_ray<double, kExact>(G, d, W.p, W.ps, cc, C, n2));
                     ++fallbacks;
                 }
             }

@@ -427,7 +427,11 @@ SMG_FN void ray_dir(const double p[3], const double q[3], double D[3], double& d
 // Chords a caller may keep from pe_cell_ray (the adjoint's second use).
 constexpr int kKeepChords = 8;
 
-template <class T, class Cands>
+// kEll: the elliptical-hole chord is compiled in (the all-FP64 kernels, which
+// the library runs whenever a hole is elliptical); otherwise every hole is
+// traced as the parameter file's circle, which keeps the ellipse's code, and
+// its registers, out of the FP32 kernels.
+template <class T, bool kEll, class Cands>
 SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[3],
                      const CellCtx& cc, const Cands& C, bool& near, double* dl_layer = nullptr,
                      double* aux = nullptr, float* keep = nullptr, int keep_stride = 0) {
@@ -512,7 +516,12 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
             int h = C.hole(j);
             if (h < first || h >= last || !C.hole_on(j)) continue;
             const Hole<double>& o = G.holes_d[h];
-            double l = hole_chord(p, cr, dd, o);
+            double l;
+            if constexpr (kEll)
+                l = o.erx > 0.0 ? ell_chord(o, p[0], p[1], p[2], cr.kx, cr.kz, cr.inv_ky, dd)
+                                : cyl_chord(p, cr, dd, o.xc, o.y1, o.y2, o.zc, o.r);
+            else
+                l = cyl_chord(p, cr, dd, o.xc, o.y1, o.y2, o.zc, o.r);
             sum += l;
             att += o.mu * l;
         }

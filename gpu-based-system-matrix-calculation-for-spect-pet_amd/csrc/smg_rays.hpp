@@ -430,44 +430,83 @@ SMG_FN void cyl_ray_from(const double D[3], double il, double inv_ky, CylRay& c)
     c.inv_qa = 1.0 / c.qa;
 }
 
-// Chord through hole o: the finite elliptic cylinder ((x' - xc)/erx)^2 +
-// ((z' - zc)/erz)^2 <= 1, y1 <= y <= y2, where (x', z') is the point turned by
-// theta about the axis (cos, sin = ec, es); reference
-// length_ellipticalcylinder_ray (PESysMatGen.cu:801-947), a routine the
-// reference defines but never calls.  A circular hole is the case erx = erz =
-// r, theta = 0, and gives the circular chord (length_cylinder_ray,
-// PESysMatGen.cu:688-799): the same cases and thresholds,
-// with the quadratic a = kx'^2 erz^2 + kz'^2 erx^2, b = kx' x0' erz^2 +
-// kz' z0' erx^2, c = ... scaled by r^2 against the circle's, so the circle's
-// discriminant cut b^2 - a c <= 1e-5 is o.dcut = 1e-5 r^4 here and the
-// ellipse's own is 1e-5.  The discriminant is formed as erx^2 erz^2 (a -
-// (kx' z0' - kz' x0')^2), the value of b^2 - a c without its cancellation.
-// One code path for both shapes keeps the PE kernels' register budget.
-// Evaluated in FP64: rays through a hole run almost along its axis, where
-// kx^2 + kz^2 -> 0 and the quadratic's roots lose every FP32 digit.
-SMG_FN double hole_chord(const double a[3], const CylRay& c, double len, const Hole<double>& o) {
-    if (fabs(o.y1 - o.y2) < kCylDy) return 0.0;
-    // start point and direction in the ellipse's frame (the turn keeps len and
-    // ky); no turn for circles (the record is the same for the whole wave, so
-    // this is a branch, not a select)
-    double x0 = a[0] - o.xc, z0 = a[2] - o.zc;
-    double kx = c.kx, kz = c.kz, sxk = c.sxk, szk = c.szk;
-    if (o.es != 0.0) {
-        const double dx = x0, dz = z0;
-        x0 = dx * o.ec + dz * o.es;
-        z0 = -dx * o.es + dz * o.ec;
-        kx = c.kx * o.ec + c.kz * o.es;
-        kz = -c.kx * o.es + c.kz * o.ec;
-        sxk = kx * c.inv_ky;
-        szk = kz * c.inv_ky;
+// Chord through the finite cylinder of radius r, axis parallel to y at
+// (xc, zc), spanning y1..y2; reference length_cylinder_ray
+// (PESysMatGen.cu:688-799).  Evaluated in FP64: rays through a hole run
+// almost along its axis, where kx^2 + kz^2 -> 0 and the quadratic's roots
+// lose every FP32 digit.
+SMG_FN double cyl_chord(const double a[3], const CylRay& c, double len, double xc, double y1,
+                        double y2, double zc, double r) {
+    if (fabs(y1 - y2) < kCylDy) return 0.0;
+    const double kx = c.kx, kz = c.kz, sxk = c.sxk, szk = c.szk;
+    const double a1 = y1 - a[1], a2 = y2 - a[1];
+    double xl = a[0] + sxk * a1 - xc;
+    double zl = a[2] + szk * a1 - zc;
+    double xr = a[0] + sxk * a2 - xc;
+    double zr = a[2] + szk * a2 - zc;
+    double t_lo = a1 * c.inv_ky;
+    double t_hi = a2 * c.inv_ky;
+    double r2 = r * r;
+    bool in_l = xl * xl + zl * zl <= r2;
+    bool in_r = xr * xr + zr * zr <= r2;
+    if (in_l && in_r) {
+        if (t_lo <= kCylT || t_hi <= kCylT) return 0.0;
+        if (t_lo >= len || t_hi >= len) return 0.0;
+        return fabs(t_hi - t_lo);
     }
-    const double a1 = o.y1 - a[1], a2 = o.y2 - a[1];
+    double x0 = a[0] - xc, z0 = a[2] - zc;
+    const double qa = c.qa;
+    double cr = kx * z0 - kz * x0;
+    double disc = qa * r2 - cr * cr;
+    if (disc <= kCylDisc) return 0.0;
+    double sq = sqrt(disc);
+    double qb = kx * x0 + kz * z0;
+    double t1 = (-qb - sq) * c.inv_qa;
+    double t2 = (-qb + sq) * c.inv_qa;
+    if (!in_l && !in_r) {
+        if (t1 >= t_lo && t2 <= t_hi) {
+            if (t2 <= kCylT || t1 <= kCylT) return 0.0;
+            if (t2 >= len || t1 >= len) return 0.0;
+            return t2 - t1;
+        }
+        return 0.0;
+    }
+    if (in_l) {
+        if (t2 <= kCylT || t_lo <= kCylT) return 0.0;
+        if (t2 >= len || t_lo >= len) return 0.0;
+        return t2 - t_lo;
+    }
+    if (t_hi <= kCylT || t1 <= kCylT) return 0.0;
+    if (t_hi >= len || t1 >= len) return 0.0;
+    return t_hi - t1;
+}
+
+// Chord through an elliptical hole o: the finite elliptic cylinder
+// ((x' - xc)/erx)^2 + ((z' - zc)/erz)^2 <= 1, y1 <= y <= y2, where (x', z') is
+// the point turned by theta about the axis (cos, sin = ec, es); reference
+// length_ellipticalcylinder_ray (PESysMatGen.cu:801-947), a routine the
+// reference defines but never calls.  The ray is the start point a, the unit
+// direction (kx, ky, kz) with inv_ky = 1/ky, and len.  After the turn the
+// cases and thresholds are the circular chord's; the quadratic is the
+// ellipse's, a = kx'^2 erz^2 + kz'^2 erx^2, b = kx' x0' erz^2 + kz' z0' erx^2,
+// with the cut b^2 - a c <= 1e-5 (PESysMatGen.cu:887-893), formed as
+// erx^2 erz^2 (a - (kx' z0' - kz' x0')^2), the value of b^2 - a c without its
+// cancellation.  FP64 only: the FP32 ray loop hands rays that meet an
+// elliptical hole to the FP64 re-computation, so this code stays out of it.
+SMG_FN double ell_chord(const Hole<double>& o, double ax, double ay, double az, double kx_,
+                        double kz_, double inv_ky, double len) {
+    if (fabs(o.y1 - o.y2) < kCylDy) return 0.0;
+    const double dx = ax - o.xc, dz = az - o.zc;
+    const double x0 = dx * o.ec + dz * o.es, z0 = -dx * o.es + dz * o.ec;
+    const double kx = kx_ * o.ec + kz_ * o.es, kz = -kx_ * o.es + kz_ * o.ec;
+    const double sxk = kx * inv_ky, szk = kz * inv_ky;
+    const double a1 = o.y1 - ay, a2 = o.y2 - ay;
     const double xl = x0 + sxk * a1, zl = z0 + szk * a1;
     const double xr = x0 + sxk * a2, zr = z0 + szk * a2;
-    const double t_lo = a1 * c.inv_ky, t_hi = a2 * c.inv_ky;
-    const double ax2 = o.ax2, az2 = o.az2;
-    const bool in_l = xl * xl * az2 + zl * zl * ax2 <= ax2 * az2;
-    const bool in_r = xr * xr * az2 + zr * zr * ax2 <= ax2 * az2;
+    const double t_lo = a1 * inv_ky, t_hi = a2 * inv_ky;
+    const double ax2 = o.erx * o.erx, az2 = o.erz * o.erz;
+    const bool in_l = xl * xl / ax2 + zl * zl / az2 <= 1.0;
+    const bool in_r = xr * xr / ax2 + zr * zr / az2 <= 1.0;
     if (in_l && in_r) {
         if (t_lo <= kCylT || t_hi <= kCylT) return 0.0;
         if (t_lo >= len || t_hi >= len) return 0.0;
@@ -476,12 +515,11 @@ SMG_FN double hole_chord(const double a[3], const CylRay& c, double len, const H
     const double qa = kx * kx * az2 + kz * kz * ax2;
     const double cr = kx * z0 - kz * x0;
     const double disc = ax2 * az2 * (qa - cr * cr);
-    if (disc <= o.dcut) return 0.0;
+    if (disc <= kCylDisc) return 0.0;
     const double sq = sqrt(disc);
     const double qb = kx * x0 * az2 + kz * z0 * ax2;
-    const double iq = 1.0 / qa;
-    const double t1 = (-qb - sq) * iq;
-    const double t2 = (-qb + sq) * iq;
+    const double t1 = (-qb - sq) / qa;
+    const double t2 = (-qb + sq) / qa;
     if (!in_l && !in_r) {
         if (t1 >= t_lo && t2 <= t_hi) {
             if (t2 <= kCylT || t1 <= kCylT) return 0.0;

@@ -85,10 +85,10 @@ double element(const Ctx& c, int d, int v, int rot, int sx, int sy, int sz, long
                 CellCtx cc;
                 make_cell_ctx(bd, a, b, cz, sx, sy, sz, cc);
                 bool near = false;
-                T val = pe_cell_ray<T>(G, d, p, ps, cc, C, near);
+                T val = pe_cell_ray<T, kExact>(G, d, p, ps, cc, C, near);
                 if (!kExact && near) {
                     bool n2 = false;
-                    val = (T)pe_cell_ray<double>(G, d, p, ps, cc, C, n2);
+                    val = (T)pe_cell_ray<double, kExact>(G, d, p, ps, cc, C, n2);
                     ++*fallbacks;
                 }
                 acc.add(val);
@@ -251,8 +251,8 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
                         if (!valid[l]) continue;
                         full.lane = nar.lane = l;
                         bool n1 = false, n2 = false;
-                        double v1 = pe_cell_ray<double>(G, d, P[l], PS[l], cc, full, n1);
-                        double v2 = pe_cell_ray<double>(G, d, P[l], PS[l], cg, nar, n2);
+                        double v1 = pe_cell_ray<double, true>(G, d, P[l], PS[l], cc, full, n1);
+                        double v2 = pe_cell_ray<double, true>(G, d, P[l], PS[l], cg, nar, n2);
                         checked += 1;
                         if (!(v1 == v2)) bad += 1;
                     }
@@ -276,6 +276,8 @@ long lane_pe_pairs(const float* coll, const float* det, const float* img, const 
                    double* out) {
     Ctx c;
     if (!make_ctx(coll, det, img, phys, c)) return -1;
+    // as the library does: elliptical holes run the all-FP64 path
+    for (const Hole<double>& o : c.hg.holes_d) exact |= o.erx > 0.0;
     long fb = 0;
     for (int i = 0; i < n; ++i)
         out[i] = exact ? element<true>(c, d[i], v[i], rot, sx, sy, sz, &fb)
