@@ -23,8 +23,8 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP kernels through the C ABI)")
 
 
-# Relative-error bar of the FP32 path against the FP64 oracle (BASELINE
-# metric: <= 1e-5 max relative error), and of the FP64 path (output rounded
+# Relative-error bar of the FP32 path against the FP64 oracle (the benchmark
+# metric's <= 1e-5 max relative error), and of the FP64 path (output rounded
 # to float32: half an ulp plus the oracle's own rounding).
 TOL_FP32 = 1e-5
 TOL_FP64 = 2e-7
