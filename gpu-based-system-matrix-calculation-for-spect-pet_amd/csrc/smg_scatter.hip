@@ -215,6 +215,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void sc
         const double sx = A.sub[0], sy = A.sub[1], sz = A.sub[2];
         ph.rcell = sqrt(bd.w * bd.w / sx / sx + bd.h * bd.h / sz / sz + bd.t * bd.t / sy / sy) / 2.0;
     }
+    ph.c_rej = window_reject_cos(ph, kExact ? 1e-9 : 1e-5);
     const int ncell = A.sub[0] * A.sub[1] * A.sub[2];
     const T wscale = T(bd.muc / bd.mupe);  // crystal weight uses bar d's mu (scatter.cu:1218-1219)
     // scatter points inside bar d itself do not count: its centre (kind 0),

@@ -158,12 +158,38 @@ struct ScatterPhys {
     double mupe, muc;      // bar d
     const MuTables* tab = nullptr;  // mu(E) tables, or power laws
     const double* knt = nullptr;    // KN prefix table (FP32 path), or the node loop
+    double c_rej = -2.0;            // cos(theta) below this is outside the window for
+                                    // sure (window_reject_cos with the path's margin)
 };
 
 // Thresholds with their reference values.
 constexpr double kFwhm = (double)2.35482f;  // scatter.cu:1026, :1263
 constexpr double kKnStep = (double)0.01f;   // scatter.cu:1338 (0.01 passed as float)
 constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
+
+// The lower 2-sigma window test of a term (scatter.cu:1263-1269) rejects it
+// when e0 g (1 + res sqrt(g) 2/FWHM) <= lo, g = E'/E0 = 1 / (1 + alpha (1 -
+// cos theta)).  The left side increases with g and g with cos theta, so the
+// test is cos theta <= c_lo for one c_lo per (E0, window, bar resolution),
+// found here by bisection in FP64.  Returns c_lo less `margin`, which covers
+// the rounding of the cosine the caller compares (FP32: below 1e-6 for the
+// unit vectors of a term, margin 1e-5; FP64: the reference's acos/cos round
+// trip, margin 1e-9), so a term below it is rejected by the full test too;
+// -2 when nothing is rejected.
+SMG_FN double window_reject_cos(const ScatterPhys& ph, double margin) {
+    const double alpha = ph.e0 / 511.0, r = ph.res * (2.0 / kFwhm);
+    if (!(ph.lo > 0.0) || !(alpha > 0.0)) return -2.0;
+    if (ph.e0 * (1.0 + r) <= ph.lo) return 2.0;  // every term is below the window
+    double g0 = 0.0, g1 = 1.0;  // e0 g (1 + r sqrt g) <= lo at g0, > lo at g1
+    for (int it = 0; it < 80; ++it) {
+        const double gm = 0.5 * (g0 + g1);
+        if (ph.e0 * gm * (1.0 + r * sqrt(gm)) <= ph.lo) g0 = gm;
+        else g1 = gm;
+    }
+    const double c_lo = 1.0 - (1.0 / g0 - 1.0) / alpha;
+    if (c_lo < -1.5) return -2.0;
+    return c_lo - margin;
+}
 
 // Klein-Nishina in the reference's factorised form (scatter.cu:962-977), from
 // cos(theta).
@@ -315,6 +341,16 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
         a[k] = sc.qs[k] - p[k];
         b[k] = sc.q[k] - sc.qs[k];
     }
+    if constexpr (std::is_same<T, float>::value) {
+        // FP32 pre-test of the lower window cut (most terms of a scatter
+        // matrix fall below the window): cos theta < c_rej  <=>
+        // a.b < c_rej |a| |b|; a term near the cut goes on to the FP64 test
+        const float ax = float(a[0]), ay = float(a[1]), az = float(a[2]);
+        const float bx = float(b[0]), by = float(b[1]), bz = float(b[2]);
+        const float ab = ax * bx + ay * by + az * bz;
+        const float nn = (ax * ax + ay * ay + az * az) * (bx * bx + by * by + bz * bz);
+        if (ab < float(ph.c_rej) * tsqrt(nn)) return T(0);
+    }
     const double alpha = ph.e0 / 511.0;
     double omc, st;  // 1 - cos(theta), sin(theta)
     {
@@ -323,6 +359,7 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
         double nb2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
         if constexpr (std::is_same<T, double>::value) {
             double c = dot / (sqrt(na2) * sqrt(nb2));
+            if (c < ph.c_rej) return T(0);  // below the window (the test further down)
             if (c > 1.0) c = 1.0;
             if (c < -1.0) c = -1.0;
             double theta = acos(c);

@@ -105,6 +105,8 @@ def lane_lib():
     L.lane_scatter_pairs.restype = C.c_long
     L.lane_scatter_pairs.argtypes = [fp, fp, fp, fp, fp, up] + [C.c_int] * 7 + [ip, ip, C.c_int, dp]
     L.lane_bitmap.restype = C.c_long
+    L.lane_window_reject_cos.restype = C.c_double
+    L.lane_window_reject_cos.argtypes = [C.c_double] * 4
     L.lane_set_hole_shapes.restype = None
     L.lane_set_hole_shapes.argtypes = [fp, C.c_int]
     L.lane_bitmap.argtypes = [fp, fp, fp, fp, C.c_int, up, C.c_long]

@@ -380,6 +380,7 @@ double scatter_element(const Ctx& c, const float* pe, const uint32_t* bitmap, in
     P.mupe = bd.mupe;
     P.muc = bd.muc;
     P.rcell = std::sqrt(bd.w * bd.w / sx / sx + bd.h * bd.h / sz / sz + bd.t * bd.t / sy / sy) / 2.0;
+    P.c_rej = window_reject_cos(P, std::is_same<T, double>::value ? 1e-9 : 1e-5);
     // the KN prefix table the GPU's FP32 path reads (built with the same code)
     std::vector<double> knt((size_t)(kKnPhase + 1) * kKnSteps);
     for (int j = 0; j <= kKnPhase; ++j) kn_prefix_row(P.e0 / 511.0, j, knt.data() + (size_t)j * kKnSteps);
@@ -475,6 +476,16 @@ void lane_kn_sum_table(double e0, const float* lo, const float* hi, int n, doubl
         out[i] = kn_sum_table(knt.data(), lo[i], hi[i], nr);
         near[i] = nr;
     }
+}
+
+// The FP32 scatter term's lower-window threshold on cos(theta).
+double lane_window_reject_cos(double e0, double lo, double res, double margin) {
+    ScatterPhys P;
+    P.e0 = e0;
+    P.lo = lo;
+    P.hi = 2.0 * e0;
+    P.res = res;
+    return window_reject_cos(P, margin);
 }
 
 // Elliptical holes (x_rho, z_rho, theta per hole) for the following calls;

@@ -188,3 +188,35 @@ def test_lane_pe_elliptic_holes(lane_lib, oracle_mod, params_mod, exact):
     assert np.max(np.abs(want - circ) / np.maximum(np.abs(circ), 1e-30)) > 0.05
     assert_parity(got, want, TOL_FP64 if exact else TOL_FP32,
                   f"lane PE elliptic holes exact={exact} fallbacks={fb}")
+
+
+def test_window_reject_cos(lane_lib):
+    """The FP32 scatter term rejects a term without the FP64 window test when
+    cos(theta) < c_rej.  Every cosine below c_rej must fail the reference's
+    lower 2-sigma test e0 g (1 + res sqrt(g) 2/FWHM) <= lo, g = 1 / (1 +
+    alpha (1 - cos)) (scatter.cu:1263-1269), evaluated here independently, and
+    c_rej must lie within 2e-5 of the exact cut (FP32 margin 1e-5); with the
+    FP64 path's margin 1e-9, within 2e-9."""
+    fwhm = float(np.float32(2.35482))
+    rng = np.random.default_rng(9)
+    for _ in range(300):
+        e0 = rng.uniform(60.0, 400.0)
+        res = rng.uniform(0.03, 0.3)
+        lo = e0 * rng.uniform(0.5, 0.98)
+        c_rej = lane_lib.lane_window_reject_cos(e0, lo, res, 1e-5)
+        c_rej64 = lane_lib.lane_window_reject_cos(e0, lo, res, 1e-9)
+        alpha = e0 / 511.0
+
+        def rejected(c):
+            g = 1.0 / (1.0 + alpha * (1.0 - c))
+            return e0 * g * (1.0 + res * np.sqrt(g) * 2.0 / fwhm) <= lo
+
+        cs = np.linspace(-1.0, 1.0, 4001)
+        rej = np.array([rejected(c) for c in cs])
+        assert np.all(rej[cs < c_rej]), (e0, lo, res, c_rej)
+        if -1.0 < c_rej < 1.0:
+            assert not rejected(c_rej + 2e-5) and rejected(c_rej)
+            assert rejected(c_rej64) and not rejected(c_rej64 + 2e-9)
+    # nothing below the window: never rejects; window above e0 (1 + r): all
+    assert lane_lib.lane_window_reject_cos(140.5, 0.0, 0.1, 1e-5) == -2.0
+    assert lane_lib.lane_window_reject_cos(140.5, 1.2 * 140.5, 0.1, 1e-5) == 2.0
