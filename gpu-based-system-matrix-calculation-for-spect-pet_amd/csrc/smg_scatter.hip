@@ -29,7 +29,7 @@ namespace smg {
 
 namespace {
 
-__global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void pair_terms_kernel(ScatterLaunch A) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const PeGeom& G = A.geom;
     const int N = G.g.n_det;
@@ -64,16 +64,28 @@ __global__ __launch_bounds__(256) void pair_terms_kernel(ScatterLaunch A) {
     } else if (A.kind == 1) {
         base = ((int64_t)s * N + d) * N;
     }
+    // In the frame of an unturned bar (beta = 0) the segment's direction is
+    // q - qs, the same for every such bar: its direction data (length, the
+    // eps decisions, len / D_k) are formed once per record.  A turned bar
+    // gets its own.
+    SegDir<double> sdw;
+    segdir_init<double>(sdw, cc.q[0] - qs[0], cc.q[1] - qs[1], cc.q[2] - qs[2], 0.0, near);
     auto add_bar = [&](int k) {
         const Bar<double>& bk = G.bars_d[k];
-        double pa[3], pb[3];
-        to_frame(bk, qs, pa);
-        to_frame(bk, cc.q, pb);
         double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
         double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
-        SegDir<double> sd;
-        segdir_init<double>(sd, pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2], 0.0, near);
-        double l = seg_box<double>(sd, pa, lo, hi, near);
+        double l;
+        if (bk.sb == 0.0 && bk.cb == 1.0) {
+            const double pa[3] = {qs[0] - bk.cx, qs[1] - bk.cy, qs[2] - bk.cz};
+            l = seg_box<double>(sdw, pa, lo, hi, near);
+        } else {
+            double pa[3], pb[3];
+            to_frame(bk, qs, pa);
+            to_frame(bk, cc.q, pb);
+            SegDir<double> sd;
+            segdir_init<double>(sd, pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2], 0.0, near);
+            l = seg_box<double>(sd, pa, lo, hi, near);
+        }
         out.s_pe += l * bk.mupe;
         out.s_c += l * bk.muc;
     };
