@@ -323,6 +323,32 @@ struct ScatterCell {
     PairTerm pt;
 };
 
+// True when the term of voxel p, scatter point sc.qs and sub-cell centre sc.q
+// lies below the energy window by the per-row cosine threshold ph.c_rej
+// (window_reject_cos): FP32 a.b < c_rej |a| |b|, or the FP64 cosine below
+// it.  Such a term is 0 (scatter_term rejects it too); callers use this to
+// skip the term, and loading its weight, early.
+template <class T>
+SMG_FN bool below_window(const ScatterPhys& ph, const double p[3], const ScatterCell& sc) {
+    double a[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+        a[k] = sc.qs[k] - p[k];
+        b[k] = sc.q[k] - sc.qs[k];
+    }
+    if constexpr (std::is_same<T, float>::value) {
+        const float ax = float(a[0]), ay = float(a[1]), az = float(a[2]);
+        const float bx = float(b[0]), by = float(b[1]), bz = float(b[2]);
+        const float ab = ax * bx + ay * by + az * bz;
+        const float nn = (ax * ax + ay * ay + az * az) * (bx * bx + by * by + bz * bz);
+        return ab < float(ph.c_rej) * tsqrt(nn);
+    } else {
+        const double dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+        const double na2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+        const double nb2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+        return dot / (sqrt(na2) * sqrt(nb2)) < ph.c_rej;
+    }
+}
+
 // One (voxel, scatter point, sub-cell) term.  wgt is the weight of the
 // scatter point for this voxel.  Returns 0 for terms outside the 2-sigma
 // energy window (scatter.cu:1263-1269).
