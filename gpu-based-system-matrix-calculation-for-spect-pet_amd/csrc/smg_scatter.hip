@@ -29,6 +29,52 @@ namespace smg {
 
 namespace {
 
+// The energy-window cut of bar d's row (window_reject_cos with no margin).
+__device__ double row_window_cos(const ScatterLaunch& A, const Bar<double>& bd) {
+    ScatterPhys ph;
+    ph.e0 = A.e0;
+    if (A.same_window) {
+        ph.lo = A.win_lo;
+        ph.hi = A.win_hi;
+    } else {  // per-bar window E0 (1 -+ res/2) (scatter.cu:1154-1163)
+        ph.lo = (1 - bd.res / 2.0) * A.e0;
+        ph.hi = (1 + bd.res / 2.0) * A.e0;
+    }
+    ph.res = bd.res;
+    return window_reject_cos(ph, 0.0);
+}
+
+// True when the terms of scatter point qs and sub-cell centre q lie below the
+// energy window for every voxel of the image, so scatter_kernel's per-lane
+// test (below_window, cosine threshold c_lo - 1e-5 in FP32, c_lo - 1e-9 in
+// FP64) rejects each of them before it reads the pair term.  The voxel
+// centres lie in a ball (C, R); seen from qs they are within asin(R / |qs -
+// C|) of the direction qs - C, so no voxel's cosine exceeds cos(angle(qs - C,
+// q - qs) - asin(R / |qs - C|)); the test asks that bound to be 1e-4 below
+// the FP32 threshold.
+__device__ bool below_window_all(const ScatterLaunch& A, const Bar<double>& bd, const double qs[3],
+                                 const double q[3]) {
+    const double c_lo = row_window_cos(A, bd);
+    if (c_lo <= -1.5) return false;  // nothing is rejected
+    const GeomDesc& g = A.geom.g;
+    const int whole[3] = {g.nx, g.ny, g.nz};
+    double C[3], R;
+    tile_ball(g, 0, 0, 0, whole, A.cphi, A.sphi, C, R);
+    double u[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+        u[k] = qs[k] - C[k];
+        b[k] = q[k] - qs[k];
+    }
+    const double L = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double B = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    if (!(L > R * (1.0 + 1e-9) + 1e-9) || !(B > 0.0)) return false;
+    double cu = (u[0] * b[0] + u[1] * b[1] + u[2] * b[2]) / (L * B);
+    cu = cu > 1.0 ? 1.0 : (cu < -1.0 ? -1.0 : cu);
+    const double th = acos(cu) - asin(R / L);
+    if (!(th > 0.0)) return false;
+    return cos(th) < c_lo - 1.1e-4;
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void pair_terms_kernel(ScatterLaunch A) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const PeGeom& G = A.geom;
@@ -54,6 +100,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
     make_cell_ctx(bd, a, b, c, A.sub[0], A.sub[1], A.sub[2], cc);
     const ScatterPt<double>& sp = A.pts_d[s];
     double qs[3] = {sp.x, sp.y, sp.z};
+    if (below_window_all(A, bd, qs, cc.q)) {  // never read: every voxel's term is below
+        A.pair[tid] = out;                    // the window (scatter_kernel tests first)
+        return;
+    }
     bool near = false;
     // bars flagged between the scatter point and bar d (scatter.cu:1343-1401);
     // grid scatterers: every other bar
