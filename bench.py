@@ -303,6 +303,11 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(a multi-GPU run would give each GPU its own detector rows)", file=sys.stderr)
         sys.exit(2)
+    names = [w for w in (args.workloads or "").split(",") if w]
+    unknown = [w for w in names if w not in WORKLOADS]
+    if unknown:
+        print(f"bench.py: unknown workloads {unknown}; choose from {WORKLOADS}", file=sys.stderr)
+        sys.exit(2)
 
     import torch
 
@@ -319,12 +324,7 @@ def main():
     import smg
     from smg import params
 
-    if args.workloads:
-        names = [w for w in args.workloads.split(",") if w]
-        unknown = [w for w in names if w not in WORKLOADS]
-        if unknown:
-            print(f"bench.py: unknown workloads {unknown}; choose from {WORKLOADS}", file=sys.stderr)
-            sys.exit(2)
+    if names:
         print(json.dumps(extras(smg, params, None, None, args.precision, sections=names,
                                 precs=(args.precision,))))
         return

@@ -95,3 +95,14 @@ def test_scatter_driver_short_pe_file(exe, tmp_path, params_mod):
     assert r.returncode == 1 and "short.sysmat holds 65541 floats, 131072 expected" in r.stderr
     assert sorted(x.name for x in tmp_path.iterdir() if x.suffix in (".sysmat", ".part")) == \
         ["short.sysmat"]
+
+
+def test_bench_refuses_more_gpus_and_unknown_workloads():
+    """bench.py runs on one GPU and names its secondary workloads; both
+    refusals happen before any GPU work."""
+    bench = os.path.join(ROOT, "bench.py")
+    r = subprocess.run([sys.executable, bench, "--gpus", "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+    r = subprocess.run([sys.executable, bench, "--workloads", "crystal_scatter,bogus"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "bogus" in r.stderr
