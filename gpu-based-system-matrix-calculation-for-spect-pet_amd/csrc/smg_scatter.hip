@@ -308,8 +308,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void sc
             sc.qs[0] = sp.x;
             sc.qs[1] = sp.y;
             sc.qs[2] = sp.z;
-            to_frame(bd, sc.qs, sc.qss);
             if (!valid || below_window<T>(ph, p, sc)) continue;
+            to_frame(bd, sc.qs, sc.qss);
             sc.pt = A.pair[((int64_t)r * A.n_pts + s) * ncell + cell];
             // weight of the scatter point for this voxel
             T wgt;
