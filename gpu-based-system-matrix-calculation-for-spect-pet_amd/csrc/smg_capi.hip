@@ -91,6 +91,7 @@ struct smg_ctx {
     DevBuf<ScatterPt<double>> pts_grid;  // grid scatterers (one chunk)
     DevBuf<float> w1;                    // their first-leg weights
     DevBuf<int> rows;                    // detector rows of a row-subset call
+    DevBuf<double> use_rows;             // crystal grid: the rows' window data (point_used)
 };
 
 namespace {
@@ -477,6 +478,7 @@ smg_status smg_destroy(smg_ctx* c) {
     c->pts_grid.release();
     c->w1.release();
     c->rows.release();
+    c->use_rows.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -970,6 +972,34 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
     A.pts_per_bar = npb;
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    // rows that will read the point weights: bar centre, bounding radius and
+    // window cosine cut each, so the points kernel can skip weights no row
+    // uses (one lane per row: up to 64 rows)
+    P.use_rows = nullptr;
+    P.n_use_rows = 0;
+    {
+        const int nr = rows ? n_rows : g.n_det;
+        if (nr <= 64) {
+            std::vector<double> ur((size_t)nr * 5);
+            for (int i = 0; i < nr; ++i) {
+                const int d = rows ? rows[i] : i;
+                const Bar<double>& bd = hg.bars_d[d];
+                ScatterPhys sp;
+                sp.e0 = A.e0;
+                sp.lo = A.same_window ? A.win_lo : (1 - bd.res / 2.0) * A.e0;
+                sp.hi = A.same_window ? A.win_hi : (1 + bd.res / 2.0) * A.e0;
+                sp.res = bd.res;
+                ur[5 * i + 0] = bd.cx;
+                ur[5 * i + 1] = bd.cy;
+                ur[5 * i + 2] = bd.cz;
+                ur[5 * i + 3] = 0.5 * std::sqrt(bd.w * bd.w + bd.t * bd.t + bd.h * bd.h) * (1.0 + 1e-9);
+                ur[5 * i + 4] = window_reject_cos(sp, 0.0);
+            }
+            if ((s = upload(c, c->use_rows, ur)) != SMG_OK) return s;
+            P.use_rows = c->use_rows.ptr;
+            P.n_use_rows = nr;
+        }
+    }
     std::vector<ScatterPt<double>> pts((size_t)nbc * npb);
     const int sub[3] = {gx, gy, gz};
     for (int b0 = 0; b0 < g.n_det; b0 += (int)nbc) {

@@ -28,6 +28,12 @@ struct PeLaunch {
     int cell_slices;              // point weights: waves per (bar, tile), each a range of cells
     const CellCtx* cells;         // sub-cell table [bar - cells_bar0][cell] (cell_table), or
     int cells_bar0;               // nullptr: the kernel forms each sub-cell itself
+    // point weights: the detector rows that will use them, 5 doubles each (bar
+    // centre, bounding radius, energy-window cosine cut); a point whose terms
+    // are below the window for every row and voxel of the tile gets weight 0
+    // without its ray.  n_use_rows = 0: every weight is formed.
+    const double* use_rows = nullptr;
+    int n_use_rows = 0;
 };
 
 // Matrix-free projections with the PE matrix of A's rotation: y = A x

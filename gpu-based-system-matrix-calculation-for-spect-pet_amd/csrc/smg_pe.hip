@@ -281,6 +281,37 @@ __device__ __forceinline__ float point_weight_f(const Bar<double>& bd, int n_cel
     return float(aux[5]) * exp_neg<float>(aux[4] * bd.mut) * tdiv(c, float(aux[6]));
 }
 
+// Point weights: true when some row in A.use_rows may use the weight of grid
+// point qs for a voxel of the wave's tile.  Lane l tests row l: with the
+// tile's voxel centres in the ball (tc, rho) and the row's sub-cell centres in
+// its bar's bounding ball (Q, rb), every scattering angle at qs is at least
+// angle(qs - tc, Q - qs) - asin(rho / |qs - tc|) - asin(rb / |Q - qs|); when
+// its cosine is 1.1e-4 below the row's window cut, the scatter kernel's
+// per-lane window test rejects every such term before it reads the weight.
+__device__ bool point_used(const PeLaunch& A, const WaveSetup& W, const double qs[3], int lane) {
+    bool use = false;
+    if (lane < A.n_use_rows) {
+        const double* R = A.use_rows + 5 * lane;
+        const double c_lo = R[4];
+        double u[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            u[k] = qs[k] - W.tc[k];
+            b[k] = R[k] - qs[k];
+        }
+        const double L = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        const double B = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        if (c_lo <= -1.5 || !(L > W.rho * (1.0 + 1e-9) + 1e-9) || !(B > R[3] * (1.0 + 1e-9) + 1e-9)) {
+            use = true;
+        } else {
+            double cu = (u[0] * b[0] + u[1] * b[1] + u[2] * b[2]) / (L * B);
+            cu = cu > 1.0 ? 1.0 : (cu < -1.0 ? -1.0 : cu);
+            const double th = acos(cu) - asin(W.rho / L) - asin(R[3] / B);
+            use = !(th > 0.0) || !(cos(th) < c_lo - 1.1e-4);
+        }
+    }
+    return __ballot(use) != 0;
+}
+
 template <bool kExact, bool kPoints>
 __device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
     __shared__ WaveLists L;
@@ -342,6 +373,12 @@ __device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
         for (; cell < c_end && n_redo < kRedoCap; ++cell) {
             CellCtx cc;
             tile_cell(A, bd, d, cell, W, lane, cc);
+            if constexpr (kPoints) {
+                if (A.n_use_rows > 0 && !point_used(A, W, cc.q, lane)) {  // wave-uniform
+                    if (W.valid) pts[(int64_t)cell * g.n_vox + W.v] = 0.0f;
+                    continue;
+                }
+            }
             int nsb, nsh;
             __syncthreads();
             narrow_candidates(A, cc.q, W, lane, L, nb, nh, nsb, nsh);
