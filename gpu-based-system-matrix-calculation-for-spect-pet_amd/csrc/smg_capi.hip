@@ -92,6 +92,8 @@ struct smg_ctx {
     DevBuf<float> w1;                    // their first-leg weights
     DevBuf<int> rows;                    // detector rows of a row-subset call
     DevBuf<double> use_rows;             // crystal grid: the rows' window data (point_used)
+    DevBuf<int> spill_queue;             // PE-family work items that overflowed LDS
+    DevBuf<unsigned char> spill_slots;   // their candidate lists in global memory
 };
 
 namespace {
@@ -166,6 +168,26 @@ double now_ms() {
 // which keeps its code and registers out of the FP32 kernels.
 bool pe_exact(const smg_ctx* c, smg_precision prec) {
     return prec == SMG_PREC_FP64 || c->elliptic_holes;
+}
+
+// Spill area of the PE-family launches (PeSpill): one queue entry per block
+// of the largest launch on this geometry (n_det x tiles blocks for the matrix
+// and the projections, at most 32,768 more for point weights and 8,192 more
+// for the adjoint) and kSpillWaves list slots sized for every bar and hole.
+smg_status ensure_spill(smg_ctx* c, PeLaunch& A) {
+    const GeomDesc& g = c->hg.g;
+    const int64_t n_tiles = (int64_t)A.tiles[0] * A.tiles[1] * A.tiles[2];
+    PeSpill& S = A.spill;
+    S.queue_cap = (int64_t)g.n_det * n_tiles + 65536;
+    S.cap_bars = g.n_det;
+    S.cap_holes = g.n_holes;
+    S.slot_bytes = pe_spill_slot_bytes(g.n_det, g.n_holes);
+    S.n_slots = kSpillWaves;
+    SMG_HIP(c, c->spill_queue.reserve((size_t)S.queue_cap));
+    SMG_HIP(c, c->spill_slots.reserve((size_t)(S.slot_bytes * S.n_slots)));
+    S.queue = c->spill_queue.ptr;
+    S.slots = c->spill_slots.ptr;
+    return SMG_OK;
 }
 
 smg_status check_ready(smg_ctx* c, int rotation) {
@@ -478,6 +500,8 @@ smg_status smg_destroy(smg_ctx* c) {
     c->pts_grid.release();
     c->w1.release();
     c->rows.release();
+    c->spill_queue.release();
+    c->spill_slots.release();
     c->use_rows.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -513,6 +537,11 @@ smg_status smg_set_geometry(smg_ctx* c, const float* coll, size_t n_coll, const 
     if (code != 0) return fail(c, (smg_status)code, "%s", why.c_str());
     smg_status s;
     SMG_HIP(c, hipStreamSynchronize(c->stream));
+    // the device buffers are replaced one by one: until all of them hold the
+    // new geometry the context has none, so a failed upload leaves it refusing
+    // launches instead of mixing two geometries
+    c->have_geom = false;
+    c->have_bm_crystal = c->have_bm_coll = false;
     if ((s = upload(c, c->bars_f, hg.bars_f)) != SMG_OK) return s;
     if ((s = upload(c, c->bars_d, hg.bars_d)) != SMG_OK) return s;
     if ((s = upload(c, c->holes_f, hg.holes_f)) != SMG_OK) return s;
@@ -551,11 +580,13 @@ smg_status smg_set_hole_shapes(smg_ctx* c, const float* shapes, int n_holes) {
     if (code != 0) return fail(c, (smg_status)code, "%s", why.c_str());
     smg_status s;
     SMG_HIP(c, hipStreamSynchronize(c->stream));
+    c->have_geom = false;  // until every hole buffer holds the new shapes
     if ((s = upload(c, c->holes_f, hg.holes_f)) != SMG_OK) return s;
     if ((s = upload(c, c->holes_d, hg.holes_d)) != SMG_OK) return s;
     if ((s = upload(c, c->hole_groups, hg.hole_groups)) != SMG_OK) return s;
     SMG_HIP(c, hipStreamSynchronize(c->stream));
     c->hg = std::move(hg);
+    c->have_geom = true;
     c->elliptic_holes = false;
     for (const Hole<double>& o : c->hg.holes_d) c->elliptic_holes |= o.erx > 0.0;
     return SMG_OK;
@@ -643,6 +674,7 @@ smg_status smg_pe_forward(smg_ctx* c, int rotation, smg_precision prec, float* d
     A.sphi = std::sin(phi);
     for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
     pick_tile(g, A.tile, A.tiles);
+    if ((s = ensure_spill(c, A)) != SMG_OK) return s;
     SMG_HIP(c, pe_cell_table(c, A));
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
@@ -663,10 +695,6 @@ smg_status smg_pe_forward(smg_ctx* c, int rotation, smg_precision prec, float* d
         stats->fp64_fallbacks = cnt[0];
         stats->overflow_waves = cnt[1];
     }
-    if (cnt[1])
-        return fail(c, SMG_ERR_CAPACITY,
-                    "%llu waves exceeded the candidate capacity (%d bars, %d holes)", cnt[1],
-                    kCapBars, kCapHoles);
     return SMG_OK;
 }
 
@@ -689,6 +717,7 @@ static smg_status project_entry(smg_ctx* c, int rotation, smg_precision prec, bo
     for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
     A.cell_slices = 1;
     pick_tile(g, A.tile, A.tiles);
+    if ((s = ensure_spill(c, A)) != SMG_OK) return s;
     SMG_HIP(c, pe_cell_table(c, A));
     const float* in_d = in;
     if (!in_is_device) {
@@ -725,10 +754,6 @@ static smg_status project_entry(smg_ctx* c, int rotation, smg_precision prec, bo
         stats->fp64_fallbacks = cnt[0];
         stats->overflow_waves = cnt[1];
     }
-    if (cnt[1])
-        return fail(c, SMG_ERR_CAPACITY,
-                    "%llu waves exceeded the candidate capacity (%d bars, %d holes)", cnt[1],
-                    kCapBars, kCapHoles);
     return SMG_OK;
 }
 
@@ -934,6 +959,7 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
     P.sub[1] = gy;
     P.sub[2] = gz;
     pick_tile(g, P.tile, P.tiles);
+    if ((s = ensure_spill(c, P)) != SMG_OK) return s;
     P.pts = c->w1.ptr;
     // sub-cell table of each chunk's bars (grid cells), up to 1 GiB
     const bool cell_table = (int64_t)nbc * npb * (int64_t)sizeof(CellCtx) <= ((int64_t)1 << 30);
@@ -1064,6 +1090,7 @@ smg_status smg_pe_backward(smg_ctx* c, int rotation, smg_precision prec, const f
     A.sphi = std::sin(phi);
     for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
     pick_tile(g, A.tile, A.tiles);
+    if ((s = ensure_spill(c, A)) != SMG_OK) return s;
     SMG_HIP(c, pe_cell_table(c, A));
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
     PeGradScratch S;
@@ -1098,8 +1125,6 @@ smg_status smg_pe_backward(smg_ctx* c, int rotation, smg_precision prec, const f
     SMG_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0.0f;
     SMG_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    if (cnt[1])
-        return fail(c, SMG_ERR_CAPACITY, "%llu waves exceeded the candidate capacity", cnt[1]);
     std::memcpy(grad, out.data(), (size_t)(n_out - 1) * sizeof(double));
     if (loss) *loss = out[(size_t)n_out - 1];
     if (stats) {
@@ -1108,7 +1133,7 @@ smg_status smg_pe_backward(smg_ctx* c, int rotation, smg_precision prec, const f
         stats->elements = n_el;
         stats->rays = n_el * (uint64_t)(c->pe_sub[0] * c->pe_sub[1] * c->pe_sub[2]);
         stats->fp64_fallbacks = cnt[0];
-        stats->overflow_waves = 0;
+        stats->overflow_waves = cnt[1];
     }
     return SMG_OK;
 }

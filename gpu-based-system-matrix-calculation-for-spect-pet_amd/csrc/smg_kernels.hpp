@@ -11,14 +11,43 @@
 namespace smg {
 
 // LDS capacity of one wave's candidate lists (bars, holes).  A wave whose
-// conservative candidate count exceeds these is reported, not truncated.
+// candidates exceed these is not truncated: it queues its work item, and a
+// second kernel redoes the queued items with lists in global memory (PeSpill).
 constexpr int kCapBars = 512;
 constexpr int kCapHoles = 256;
+
+// Waves of a spill kernel (one list slot each).
+constexpr int kSpillWaves = 1024;
+
+// Work items whose candidates overflowed the LDS lists.  Each PE-family
+// launch clears counters[2], its kernel appends the block index of every
+// overflowing wave to queue (room for one entry per block of the launch),
+// and the launch's spill kernel (kSpillWaves waves or fewer) redoes the
+// queued items, wave b with the lists in slot b of slots; cap_bars and
+// cap_holes are the geometry's bar and hole counts, so those lists cannot
+// overflow.  The spilled items are added to counters[1].
+struct PeSpill {
+    int* queue = nullptr;
+    int64_t queue_cap = 0;
+    unsigned char* slots = nullptr;
+    int64_t slot_bytes = 0;
+    int n_slots = 0;
+    int cap_bars = 0, cap_holes = 0;
+};
+
+// Bytes of one spill slot: masks, adjoint sums and indices of every bar and
+// hole, in 256-byte units.
+inline int64_t pe_spill_slot_bytes(int n_det, int n_holes) {
+    const int64_t b = n_det > 0 ? n_det : 1, h = n_holes > 0 ? n_holes : 1;
+    const int64_t bytes = b * (8 + 8 + 2 + 2) + h * (8 + 2 + 2);
+    return (bytes + 255) / 256 * 256;
+}
 
 struct PeLaunch {
     PeGeom geom;
     float* dst;                   // [N][V] block of this rotation
-    unsigned long long* counters; // [0] FP64 fallback rays, [1] overflowed waves
+    unsigned long long* counters; // [0] FP64 fallback rays, [1] work items redone with
+                                  // spill lists, [2] spill queue length of the launch
     double cphi, sphi;            // rotation of the voxel grid about z
     int sub[3];                   // sub-cells per bar (x, y, z)
     int tile[3];                  // voxel tile of one wave, product 64
@@ -34,6 +63,7 @@ struct PeLaunch {
     // without its ray.  n_use_rows = 0: every weight is formed.
     const double* use_rows = nullptr;
     int n_use_rows = 0;
+    PeSpill spill;                // lists of the waves that overflow LDS
 };
 
 // Matrix-free projections with the PE matrix of A's rotation: y = A x

@@ -35,6 +35,7 @@ constexpr int kRedoCap = 64;
 
 // Candidate lists of one wave, in LDS.
 struct WaveLists {
+    static constexpr int cap_bars = kCapBars, cap_holes = kCapHoles;
     uint64_t bar_mask[kCapBars];
     uint64_t hole_mask[kCapHoles];
     uint16_t bar_idx[kCapBars];
@@ -43,9 +44,55 @@ struct WaveLists {
     uint16_t hole_sel[kCapHoles];
 };
 
+// Candidate lists of a wave whose tile overflowed the LDS lists, in its own
+// slot of global memory with room for every bar and hole of the geometry (so
+// they cannot overflow), plus the adjoint's per-candidate sums.
+struct SpillLists {
+    int cap_bars, cap_holes;
+    uint64_t* bar_mask;
+    uint64_t* hole_mask;
+    double* cand_acc;
+    uint16_t* bar_idx;
+    uint16_t* hole_idx;
+    uint16_t* bar_sel;
+    uint16_t* hole_sel;
+};
+
+__device__ __forceinline__ SpillLists spill_lists(const PeSpill& S, int slot) {
+    unsigned char* p = S.slots + (int64_t)slot * S.slot_bytes;
+    SpillLists L;
+    L.cap_bars = S.cap_bars;
+    L.cap_holes = S.cap_holes;
+    L.bar_mask = (uint64_t*)p;
+    p += sizeof(uint64_t) * S.cap_bars;
+    L.hole_mask = (uint64_t*)p;
+    p += sizeof(uint64_t) * S.cap_holes;
+    L.cand_acc = (double*)p;
+    p += sizeof(double) * S.cap_bars;
+    L.bar_idx = (uint16_t*)p;
+    p += sizeof(uint16_t) * S.cap_bars;
+    L.hole_idx = (uint16_t*)p;
+    p += sizeof(uint16_t) * S.cap_holes;
+    L.bar_sel = (uint16_t*)p;
+    p += sizeof(uint16_t) * S.cap_bars;
+    L.hole_sel = (uint16_t*)p;
+    return L;
+}
+
+// A wave whose candidates overflowed its LDS lists queues its work item
+// (its block index) for the spill kernel of the same launch; without a
+// queue (a launch with no spill area) it is only counted.
+__device__ __forceinline__ void spill_push(const PeLaunch& A, int lane) {
+    if (lane == 0) {
+        const unsigned long long i = atomicAdd(&A.counters[2], 1ull);
+        if (A.spill.queue) A.spill.queue[i] = (int)blockIdx.x;
+    }
+}
+
 // View of the narrowed lists for one lane.
-struct LdsCands {
-    const WaveLists* L;
+template <class Lists>
+struct ListCands {
+    const Lists* L;
     int nb, nh, lane;
     __device__ int nbars() const { return nb; }
     // the index is wave-uniform: readfirstlane lets the record loads go
@@ -113,9 +160,9 @@ __device__ __forceinline__ void wave_setup(const PeLaunch& A, int d, int tile, i
 // groups it keeps; any other has chord 0 for every ray of the tile, so its
 // culling decision does not matter), each with the ballot of the lanes whose
 // reference culling test keeps it.  Returns false on overflow.
-template <bool kExact>
+template <bool kExact, class Lists>
 __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveSetup& W,
-                                 WaveLists& L, int& nb, int& nh) {
+                                 Lists& L, int& nb, int& nh) {
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;
     const Bar<float>& bdf = G.bars_f[d];
@@ -150,7 +197,7 @@ __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveS
                 bool keep = W.valid && cull_bar<kExact>(G, kk, W.p, W.pf, W.sc, W.udf, W.critf);
                 uint64_t m = __ballot(keep);
                 if (m) {
-                    if (nb < kCapBars && lane == 0) {
+                    if (nb < L.cap_bars && lane == 0) {
                         L.bar_idx[nb] = (uint16_t)kk;
                         L.bar_mask[nb] = m;
                     }
@@ -186,7 +233,7 @@ __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveS
                 bool keep = W.valid && cull_hole<kExact>(G, hh, W.p, W.pf, W.sc, W.udf, W.critf);
                 uint64_t m = __ballot(keep);
                 if (m) {
-                    if (nh < kCapHoles && lane == 0) {
+                    if (nh < L.cap_holes && lane == 0) {
                         L.hole_idx[nh] = (uint16_t)hh;
                         L.hole_mask[nh] = m;
                     }
@@ -195,7 +242,7 @@ __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveS
             }
         }
     }
-    return nb <= kCapBars && nh <= kCapHoles;
+    return nb <= L.cap_bars && nh <= L.cap_holes;
 }
 
 // Sub-cell `cell` of bar d for this wave's tile (from the sub-cell table when
@@ -222,8 +269,9 @@ __device__ __forceinline__ void tile_cell(const PeLaunch& A, const Bar<double>& 
 // Step 2: candidates whose box a segment from the sub-cell centre qd to the
 // tile can meet (box_meets_frustum with a point source), bars and holes in
 // one pass; writes their list indices to bar_sel / hole_sel.
+template <class Lists>
 __device__ void narrow_candidates(const PeLaunch& A, const double qd[3], const WaveSetup& W,
-                                  int lane, WaveLists& L, int nb, int nh, int& nsb, int& nsh) {
+                                  int lane, Lists& L, int nb, int nh, int& nsb, int& nsh) {
     const PeGeom& G = A.geom;
     const float q[3] = {float(qd[0]), float(qd[1]), float(qd[2])};
     const float rq[3] = {0.0f, 0.0f, 0.0f};
@@ -312,22 +360,24 @@ __device__ bool point_used(const PeLaunch& A, const WaveSetup& W, const double q
     return __ballot(use) != 0;
 }
 
-template <bool kExact, bool kPoints>
-__device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
-    __shared__ WaveLists L;
+// Work item `item` of the launch (a block index: bar, tile and, for point
+// weights, slice of cells) with candidate lists L.  Returns false, having
+// written nothing, when the candidates overflow L.
+template <bool kExact, bool kPoints, class Lists>
+__device__ __forceinline__ bool pe_forward_body(const PeLaunch& A, int item, Lists& L) {
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;
     const int lane = threadIdx.x;
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
     // point weights: cells [c_begin, c_end) of the wave's slice; PE: all cells
     const int slices = kPoints ? A.cell_slices : 1;
-    const int wave = (int)(blockIdx.x / slices);
+    const int wave = item / slices;
     const int d = A.bar0 + wave / n_tiles;
     const int tile = wave % n_tiles;
     float* out = A.dst + (int64_t)d * g.n_vox;
     const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
     const int per = (n_cells + slices - 1) / slices;
-    const int c_begin = kPoints ? (int)(blockIdx.x % slices) * per : 0;
+    const int c_begin = kPoints ? (item % slices) * per : 0;
     const int c_end = kPoints ? min(n_cells, c_begin + per) : n_cells;
     float* pts = kPoints ? A.pts + (int64_t)(d - A.bar0) * n_cells * g.n_vox : nullptr;
     const Bar<double>& bd = G.bars_d[d];
@@ -340,20 +390,10 @@ __device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
         } else {
             if (W.valid) out[W.v] = 0.0f;
         }
-        return;
+        return true;
     }
     int nb, nh;
-    if (!build_candidates<kExact>(A, d, lane, W, L, nb, nh)) {
-        // too many candidates for the LDS lists: report, write NaN
-        if (lane == 0) atomicAdd(&A.counters[1], 1ull);
-        if constexpr (kPoints) {
-            if (W.valid)
-                for (int c = c_begin; c < c_end; ++c) pts[(int64_t)c * g.n_vox + W.v] = __builtin_nanf("");
-        } else {
-            if (W.valid) out[W.v] = __builtin_nanf("");
-        }
-        return;
-    }
+    if (!build_candidates<kExact>(A, d, lane, W, L, nb, nh)) return false;
     __syncthreads();
 
     using T = typename std::conditional<kExact, double, float>::type;
@@ -385,7 +425,7 @@ __device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
             __syncthreads();
             bool near = false;
             if (W.valid) {
-                LdsCands C{&L, nsb, nsh, lane};
+                ListCands<Lists> C{&L, nsb, nsh, lane};
                 if constexpr (kPoints) {
                     double aux[8];
                     pe_cell_ray<T, kExact>(G, d, W.p, W.ps, cc, C, near, nullptr, aux);
@@ -417,7 +457,7 @@ __device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
                 narrow_candidates(A, cc.q, W, lane, L, nb, nh, rsb, rsh);
                 __syncthreads();
                 if ((redo_mask[i] >> lane) & 1ull) {
-                    LdsCands C{&L, rsb, rsh, lane};
+                    ListCands<Lists> C{&L, rsb, rsh, lane};
                     bool n2 = false;
                     if constexpr (kPoints) {
                         double aux[8];
@@ -439,18 +479,35 @@ __device__ __forceinline__ void pe_forward_body(const PeLaunch& A) {
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
         if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
     }
+    return true;
 }
 
 template <bool kExact>
 __global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
-    pe_forward_body<kExact, false>(A);
+    __shared__ WaveLists L;
+    if (!pe_forward_body<kExact, false>(A, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
 }
 
 // Point weights at 3 waves per SIMD (the register cap costs no spills here;
 // without it the allocation lands one register above the 3-wave limit).
 template <bool kExact>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_points_kernel(PeLaunch A) {
-    pe_forward_body<kExact, true>(A);
+    __shared__ WaveLists L;
+    if (!pe_forward_body<kExact, true>(A, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
+}
+
+// The work items the kernel above queued (counters[2] of them), redone with
+// candidate lists in global memory: wave b takes queue entries b, b + grid, ...
+// and its own list slot.  The entry count is wave-uniform, so every wave ends.
+template <bool kExact, bool kPoints>
+__global__ __launch_bounds__(64) void pe_forward_spill_kernel(PeLaunch A) {
+    SpillLists L = spill_lists(A.spill, (int)blockIdx.x);
+    const unsigned long long n = A.counters[2];
+    for (unsigned long long i = blockIdx.x; i < n; i += gridDim.x) {
+        __syncthreads();  // the previous item's list reads are done
+        pe_forward_body<kExact, kPoints>(A, A.spill.queue[i], L);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&A.counters[1], n);
 }
 
 // ---- adjoint -------------------------------------------------------------------------
@@ -487,9 +544,9 @@ struct AdjAcc {
 // FP32 a lane whose ray came within tolerance of a threshold contributes
 // nothing and is reported through the return value (the caller redoes it in
 // FP64).  Every lane of the wave takes part in the wave sums.
-template <class T, bool kEll>
+template <class T, bool kEll, class Lists>
 __device__ bool adjoint_cell(const PeGeom& G, int d, int lane, const WaveSetup& W,
-                             const WaveLists& L, const LdsCands& C, const CellCtx& cc,
+                             const Lists& L, const ListCands<Lists>& C, const CellCtx& cc,
                              bool active, double wv, AdjAcc& acc, double* lay_acc,
                              double* cand_acc, float* keep = nullptr) {
     const GeomDesc& g = G.g;
@@ -545,10 +602,13 @@ __device__ bool adjoint_cell(const PeGeom& G, int d, int lane, const WaveSetup& 
     return near;
 }
 
-template <bool kExact>
-__global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScratch S) {
-    __shared__ WaveLists L;
-    __shared__ double cand_acc[kCapBars];
+// Adjoint of work item `item` (a block index: bar d, group of tiles) with
+// candidate lists L and per-candidate sums cand_acc.  Returns false when the
+// candidates overflow L (its scratch row is then incomplete; the spill kernel
+// rewrites all of it).
+template <bool kExact, class Lists>
+__device__ __forceinline__ bool pe_backward_body(const PeLaunch& A, const PeGradScratch& S, int item,
+                                                 Lists& L, double* cand_acc) {
     __shared__ double lay_acc[kMaxLayers];
     __shared__ float keep[kKeepChords * 64];  // FP32 rays' chords, [candidate][lane]
     __shared__ uint16_t redo_cell[kRedoCap];
@@ -557,10 +617,10 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
     const GeomDesc& g = G.g;
     const int lane = threadIdx.x;
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
-    const int d = (int)(blockIdx.x / S.groups);
-    const int grp = (int)(blockIdx.x % S.groups);
+    const int d = item / S.groups;
+    const int grp = item % S.groups;
     const Bar<double>& bd = G.bars_d[d];
-    double* out_row = S.rows + (int64_t)blockIdx.x * S.stride;
+    double* out_row = S.rows + (int64_t)item * S.stride;
     // the bars' mu_total partials accumulate in this wave's own row of the
     // scratch buffer (no other wave writes it; LDS would cap the occupancy)
     for (int k = lane; k < g.n_det; k += 64) out_row[k] = 0.0;
@@ -600,7 +660,7 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                     __syncthreads();
                     narrow_candidates(A, cc.q, W, lane, L, nb, nh, nsb, nsh);
                     __syncthreads();
-                    LdsCands C{&L, nsb, nsh, lane};
+                    ListCands<Lists> C{&L, nsb, nsh, lane};
                     const bool near = adjoint_cell<T, kExact>(G, d, lane, W, L, C, cc, W.valid, wv, acc,
                                                       lay_acc, cand_acc, kExact ? nullptr : keep + lane);
                     if constexpr (!kExact) {
@@ -623,7 +683,7 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
                         __syncthreads();
                         narrow_candidates(A, cc.q, W, lane, L, nb, nh, rsb, rsh);
                         __syncthreads();
-                        LdsCands C{&L, rsb, rsh, lane};
+                        ListCands<Lists> C{&L, rsb, rsh, lane};
                         const bool mine = (redo_mask[i] >> lane) & 1ull;
                         adjoint_cell<double, kExact>(G, d, lane, W, L, C, cc, mine, wv, acc, lay_acc, cand_acc);
                         fallbacks += mine;
@@ -645,13 +705,34 @@ __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScrat
         for (int m = 0; m < g.n_layers; ++m) out_row[g.n_det + m] = lay_acc[m];
         out_row[g.n_det + g.n_layers] = own_mupe;
         out_row[g.n_det + g.n_layers + 1] = loss;
-        if (overflow) atomicAdd(&A.counters[1], 1ull);
     }
+    if (overflow) return false;
     if constexpr (!kExact) {
         unsigned tot = fallbacks;
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
         if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
     }
+    return true;
+}
+
+template <bool kExact>
+__global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScratch S) {
+    __shared__ WaveLists L;
+    __shared__ double cand_acc[kCapBars];
+    if (!pe_backward_body<kExact>(A, S, (int)blockIdx.x, L, cand_acc)) spill_push(A, threadIdx.x);
+}
+
+// Queued work items of pe_backward_kernel redone with lists in global memory
+// (as pe_forward_spill_kernel); each rewrites its whole scratch row.
+template <bool kExact>
+__global__ __launch_bounds__(64) void pe_backward_spill_kernel(PeLaunch A, PeGradScratch S) {
+    SpillLists L = spill_lists(A.spill, (int)blockIdx.x);
+    const unsigned long long n = A.counters[2];
+    for (unsigned long long i = blockIdx.x; i < n; i += gridDim.x) {
+        __syncthreads();
+        pe_backward_body<kExact>(A, S, A.spill.queue[i], L, L.cand_acc);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&A.counters[1], n);
 }
 
 // grad[0 .. Lc-1]: layers; grad[Lc + k]: mu_total of bar k; grad[Lc + N + k]:
@@ -694,21 +775,19 @@ __global__ __launch_bounds__(256) void pe_grad_reduce(PeGradScratch S, int n_det
 // forward projection y[d] = sum_v A[d][v] x[v], back-projection
 // x[v] = sum_d A[d][v] y[d].  Sums are FP64 in a fixed order.
 
-// A[d][v] of this lane for bar d and the wave's tile (W set up for them).
-// Returns 0 for a bar below the mu threshold and NaN (and counts the wave) on
-// a candidate-list overflow; adds the lane's FP64 re-computations to fallbacks.
-template <bool kExact>
-__device__ double pe_element(const PeLaunch& A, int d, int lane, const WaveSetup& W, WaveLists& L,
-                             uint16_t* redo_cell, uint64_t* redo_mask, unsigned& fallbacks) {
+// A[d][v] of this lane for bar d and the wave's tile (W set up for them) into
+// a: 0 for a bar below the mu threshold.  Returns false (wave-uniform) when
+// the candidates overflow L; adds the lane's FP64 re-computations to fallbacks.
+template <bool kExact, class Lists>
+__device__ bool pe_element(const PeLaunch& A, int d, int lane, const WaveSetup& W, Lists& L,
+                           uint16_t* redo_cell, uint64_t* redo_mask, unsigned& fallbacks, double& a) {
     const PeGeom& G = A.geom;
     const Bar<double>& bd = G.bars_d[d];
-    if (!(bd.mut > (double)0.01f)) return 0.0;  // PESysMatGen.cu:1116, :1194
+    a = 0.0;
+    if (!(bd.mut > (double)0.01f)) return true;  // PESysMatGen.cu:1116, :1194
     int nb, nh;
     __syncthreads();  // the lists of the previous element are no longer read
-    if (!build_candidates<kExact>(A, d, lane, W, L, nb, nh)) {
-        if (lane == 0) atomicAdd(&A.counters[1], 1ull);
-        return __builtin_nan("");
-    }
+    if (!build_candidates<kExact>(A, d, lane, W, L, nb, nh)) return false;
     __syncthreads();
     using T = typename std::conditional<kExact, double, float>::type;
     KahanSum<T> acc;
@@ -725,7 +804,7 @@ __device__ double pe_element(const PeLaunch& A, int d, int lane, const WaveSetup
             __syncthreads();
             bool near = false;
             if (W.valid) {
-                LdsCands C{&L, nsb, nsh, lane};
+                ListCands<Lists> C{&L, nsb, nsh, lane};
                 T val = pe_cell_ray<T, kExact>(G, d, W.p, W.ps, cc, C, near);
                 if (!near) acc.add(val);
             }
@@ -749,7 +828,7 @@ __device__ double pe_element(const PeLaunch& A, int d, int lane, const WaveSetup
                 narrow_candidates(A, cc.q, W, lane, L, nb, nh, rsb, rsh);
                 __syncthreads();
                 if ((redo_mask[i] >> lane) & 1ull) {
-                    LdsCands C{&L, rsb, rsh, lane};
+                    ListCands<Lists> C{&L, rsb, rsh, lane};
                     bool n2 = false;
                     acc.add((T)pe_cell_ray<double, kExact>(G, d, W.p, W.ps, cc, C, n2));
                     ++fallbacks;
@@ -759,47 +838,59 @@ __device__ double pe_element(const PeLaunch& A, int d, int lane, const WaveSetup
     }
     // the element as the matrix kernel stores it (FP32), so that projecting
     // equals multiplying by the stored matrix
-    return (double)float(acc.value());
+    a = (double)float(acc.value());
+    return true;
 }
 
-// Forward projection: one wave per (bar d, voxel tile); the wave's share of
-// y[d] goes to partial[d][tile].
-template <bool kExact>
-__global__ __launch_bounds__(64) void pe_project_kernel(PeLaunch A, const float* x, double* partial) {
-    __shared__ WaveLists L;
+__device__ __forceinline__ void add_fallbacks(const PeLaunch& A, int lane, unsigned fallbacks) {
+    unsigned tot = fallbacks;
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
+}
+
+// Forward projection of work item `item` (bar d, voxel tile): the wave's share
+// of y[d] goes to partial[item].  Returns false, having written nothing, when
+// the candidates overflow L.
+template <bool kExact, class Lists>
+__device__ __forceinline__ bool pe_project_body(const PeLaunch& A, const float* x, double* partial, int item,
+                                                Lists& L) {
     __shared__ uint16_t redo_cell[kRedoCap];
     __shared__ uint64_t redo_mask[kRedoCap];
     const int lane = threadIdx.x;
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
-    const int d = (int)(blockIdx.x / n_tiles);
-    const int tile = (int)(blockIdx.x % n_tiles);
+    const int d = item / n_tiles;
+    const int tile = item % n_tiles;
     WaveSetup W;
     wave_setup(A, d, tile, lane, W);
     unsigned fallbacks = 0;
-    const double a = pe_element<kExact>(A, d, lane, W, L, redo_cell, redo_mask, fallbacks);
+    double a;
+    if (!pe_element<kExact>(A, d, lane, W, L, redo_cell, redo_mask, fallbacks, a)) return false;
     const double s = wave_sum(W.valid ? a * (double)x[W.v] : 0.0);
-    if (lane == 0) partial[blockIdx.x] = s;
-    if constexpr (!kExact) {
-        unsigned tot = fallbacks;
-        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-        if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
-    }
+    if (lane == 0) partial[item] = s;
+    if constexpr (!kExact) add_fallbacks(A, lane, fallbacks);
+    return true;
 }
 
-// Back-projection: one wave per (group of `group` consecutive bars, voxel
-// tile), looping over the group's bars (those with y[d] = 0 skipped); each
-// lane's share of x[v] goes to partial[group][v].
 template <bool kExact>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_backproject_kernel(
-    PeLaunch A, const float* y, int group, double* partial) {
+__global__ __launch_bounds__(64) void pe_project_kernel(PeLaunch A, const float* x, double* partial) {
     __shared__ WaveLists L;
+    if (!pe_project_body<kExact>(A, x, partial, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
+}
+
+// Back-projection of work item `item` (group of `group` consecutive bars,
+// voxel tile), looping over the group's bars (those with y[d] = 0 skipped);
+// each lane's share of x[v] goes to partial[group][v].  Returns false, having
+// written nothing, when the candidates of one of the bars overflow L.
+template <bool kExact, class Lists>
+__device__ __forceinline__ bool pe_backproject_body(const PeLaunch& A, const float* y, int group,
+                                                    double* partial, int item, Lists& L) {
     __shared__ uint16_t redo_cell[kRedoCap];
     __shared__ uint64_t redo_mask[kRedoCap];
     const GeomDesc& g = A.geom.g;
     const int lane = threadIdx.x;
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
-    const int grp = (int)(blockIdx.x / n_tiles);
-    const int tile = (int)(blockIdx.x % n_tiles);
+    const int grp = item / n_tiles;
+    const int tile = item % n_tiles;
     const int d0 = grp * group, d1 = min(g.n_det, d0 + group);
     unsigned fallbacks = 0;
     double acc = 0.0;
@@ -808,7 +899,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe
         if (yd == 0.0f) continue;  // wave-uniform
         WaveSetup W;
         wave_setup(A, d, tile, lane, W);
-        const double a = pe_element<kExact>(A, d, lane, W, L, redo_cell, redo_mask, fallbacks);
+        double a;
+        if (!pe_element<kExact>(A, d, lane, W, L, redo_cell, redo_mask, fallbacks, a)) return false;
         acc += a * (double)yd;
     }
     // the lane's voxel (it does not depend on the bar)
@@ -819,11 +911,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe
     const int iz = t2 * A.tile[2] + lane / (A.tile[0] * A.tile[1]);
     if (ix < g.nx && iy < g.ny && iz < g.nz)
         partial[(int64_t)grp * g.n_vox + ix + (int64_t)iy * g.nx + (int64_t)iz * g.nx * g.ny] = acc;
-    if constexpr (!kExact) {
-        unsigned tot = fallbacks;
-        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-        if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
+    if constexpr (!kExact) add_fallbacks(A, lane, fallbacks);
+    return true;
+}
+
+template <bool kExact>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_backproject_kernel(
+    PeLaunch A, const float* y, int group, double* partial) {
+    __shared__ WaveLists L;
+    if (!pe_backproject_body<kExact>(A, y, group, partial, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
+}
+
+// Queued work items of the projection kernels redone with lists in global
+// memory (as pe_forward_spill_kernel).
+template <bool kExact, bool kBack>
+__global__ __launch_bounds__(64) void pe_project_spill_kernel(PeLaunch A, const float* in, int group,
+                                                              double* partial) {
+    SpillLists L = spill_lists(A.spill, (int)blockIdx.x);
+    const unsigned long long n = A.counters[2];
+    for (unsigned long long i = blockIdx.x; i < n; i += gridDim.x) {
+        __syncthreads();
+        if constexpr (kBack)
+            pe_backproject_body<kExact>(A, in, group, partial, A.spill.queue[i], L);
+        else
+            pe_project_body<kExact>(A, in, partial, A.spill.queue[i], L);
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&A.counters[1], n);
 }
 
 // out[i] = sum_k partial[k * stride + i * step] for k < n, in k order.
@@ -838,15 +951,40 @@ __global__ __launch_bounds__(256) void sum_rows_kernel(const double* partial, in
 
 }  // namespace
 
+namespace {
+
+// Checks the spill area against a launch of `blocks` blocks, clears the
+// launch's queue length, and returns the spill kernel's grid.  A launch with
+// no spill area (queue == nullptr; the caller has cleared counters[2]) gets
+// grid 0: its overflowing work items are only counted in counters[2].
+hipError_t spill_begin(const PeLaunch& A, int64_t blocks, hipStream_t stream, unsigned& spill_grid) {
+    const PeSpill& S = A.spill;
+    spill_grid = 0;
+    if (!S.queue) return hipSuccess;
+    if (!S.slots || S.n_slots < 1 || blocks > S.queue_cap || S.cap_bars < A.geom.g.n_det ||
+        S.cap_holes < A.geom.g.n_holes || S.slot_bytes < pe_spill_slot_bytes(S.cap_bars, S.cap_holes))
+        return hipErrorInvalidValue;
+    spill_grid = (unsigned)std::min<int64_t>(S.n_slots, blocks);
+    return hipMemsetAsync(A.counters + 2, 0, sizeof(unsigned long long), stream);
+}
+
+}  // namespace
+
 hipError_t launch_pe_backward(const PeLaunch& A, const PeGradScratch& S, bool exact, double* grad,
                               hipStream_t stream) {
     const int64_t blocks = (int64_t)A.geom.g.n_det * S.groups;
     if (blocks <= 0) return hipSuccess;
-    if (exact)
+    unsigned sg = 0;
+    hipError_t e = spill_begin(A, blocks, stream, sg);
+    if (e != hipSuccess) return e;
+    if (exact) {
         hipLaunchKernelGGL(pe_backward_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A, S);
-    else
+        if (sg) hipLaunchKernelGGL(pe_backward_spill_kernel<true>, dim3(sg), dim3(64), 0, stream, A, S);
+    } else {
         hipLaunchKernelGGL(pe_backward_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A, S);
-    hipError_t e = hipGetLastError();
+        if (sg) hipLaunchKernelGGL(pe_backward_spill_kernel<false>, dim3(sg), dim3(64), 0, stream, A, S);
+    }
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int n_out = A.geom.g.n_layers + 2 * A.geom.g.n_det + 1;
     hipLaunchKernelGGL(pe_grad_reduce, dim3(n_out), dim3(256), 0, stream, S, A.geom.g.n_det,
@@ -889,10 +1027,16 @@ hipError_t launch_pe_forward(const PeLaunch& A, bool exact, hipStream_t stream) 
     PeLaunch B = A;
     B.bar0 = 0;
     B.cell_slices = 1;
-    if (exact)
+    unsigned sg = 0;
+    hipError_t e = spill_begin(B, blocks, stream, sg);
+    if (e != hipSuccess) return e;
+    if (exact) {
         hipLaunchKernelGGL(pe_forward_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, B);
-    else
+        if (sg) hipLaunchKernelGGL((pe_forward_spill_kernel<true, false>), dim3(sg), dim3(64), 0, stream, B);
+    } else {
         hipLaunchKernelGGL(pe_forward_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, B);
+        if (sg) hipLaunchKernelGGL((pe_forward_spill_kernel<false, false>), dim3(sg), dim3(64), 0, stream, B);
+    }
     return hipGetLastError();
 }
 
@@ -909,10 +1053,16 @@ hipError_t launch_pe_points(const PeLaunch& A, bool exact, int bar0, int n_bars,
     slices = std::min<int64_t>(slices, std::max(1, n_cells / 64));
     B.cell_slices = (int)std::max<int64_t>(slices, 1);
     const int64_t blocks = waves * B.cell_slices;
-    if (exact)
+    unsigned sg = 0;
+    hipError_t e = spill_begin(B, blocks, stream, sg);
+    if (e != hipSuccess) return e;
+    if (exact) {
         hipLaunchKernelGGL(pe_points_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, B);
-    else
+        if (sg) hipLaunchKernelGGL((pe_forward_spill_kernel<true, true>), dim3(sg), dim3(64), 0, stream, B);
+    } else {
         hipLaunchKernelGGL(pe_points_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, B);
+        if (sg) hipLaunchKernelGGL((pe_forward_spill_kernel<false, true>), dim3(sg), dim3(64), 0, stream, B);
+    }
     return hipGetLastError();
 }
 
@@ -936,11 +1086,19 @@ hipError_t launch_pe_project(const PeLaunch& A, bool exact, const float* x, doub
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
     const int64_t blocks = (int64_t)A.geom.g.n_det * n_tiles;
     if (blocks <= 0) return hipSuccess;
-    if (exact)
+    unsigned sg = 0;
+    hipError_t e = spill_begin(A, blocks, stream, sg);
+    if (e != hipSuccess) return e;
+    if (exact) {
         hipLaunchKernelGGL(pe_project_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A, x, partial);
-    else
+        if (sg) hipLaunchKernelGGL((pe_project_spill_kernel<true, false>), dim3(sg), dim3(64), 0, stream, A, x, 0,
+                           partial);
+    } else {
         hipLaunchKernelGGL(pe_project_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A, x, partial);
-    hipError_t e = hipGetLastError();
+        if (sg) hipLaunchKernelGGL((pe_project_spill_kernel<false, false>), dim3(sg), dim3(64), 0, stream, A, x, 0,
+                           partial);
+    }
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int64_t n = A.geom.g.n_det;  // y[d] = sum over tiles of partial[d * n_tiles + t]
     hipLaunchKernelGGL(sum_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, partial,
@@ -956,13 +1114,21 @@ hipError_t launch_pe_backproject(const PeLaunch& A, bool exact, const float* y, 
     const int n_grp = (g.n_det + grp - 1) / grp;
     const int64_t blocks = (int64_t)n_grp * n_tiles;
     if (blocks <= 0) return hipSuccess;
-    if (exact)
+    unsigned sg = 0;
+    hipError_t e = spill_begin(A, blocks, stream, sg);
+    if (e != hipSuccess) return e;
+    if (exact) {
         hipLaunchKernelGGL(pe_backproject_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A, y, grp,
                            partial);
-    else
+        if (sg) hipLaunchKernelGGL((pe_project_spill_kernel<true, true>), dim3(sg), dim3(64), 0, stream, A, y, grp,
+                           partial);
+    } else {
         hipLaunchKernelGGL(pe_backproject_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A, y, grp,
                            partial);
-    hipError_t e = hipGetLastError();
+        if (sg) hipLaunchKernelGGL((pe_project_spill_kernel<false, true>), dim3(sg), dim3(64), 0, stream, A, y, grp,
+                           partial);
+    }
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int64_t n = g.n_vox;  // x[v] = sum over groups of partial[k * V + v]
     hipLaunchKernelGGL(sum_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, partial,

@@ -96,7 +96,18 @@ smg_ctx* context_for(int cuda_id, const float* coll, const float* det, const flo
     return c.ctx;
 }
 
-int rotation_of(const float* img) { return (int)(img[20] + 0.5f); }
+// img[20] is the rotation index the drivers write (PEGen_CircularHole.cpp:133);
+// the reference multiplies it, as a float, by the angle step
+// (PESysMatGen.cu:1004-1006).  The C ABI takes whole indices, so an index
+// that is not a non-negative whole number is refused rather than rounded.
+int rotation_of(const float* img) {
+    const float r = img[20];
+    if (!(r >= 0.0f) || r != std::floor(r) || r > 1.0e6f) {
+        std::fprintf(stderr, "rotation index img[20] = %g is not a non-negative whole number\n", (double)r);
+        std::exit(EXIT_FAILURE);
+    }
+    return (int)r;
+}
 
 int64_t n_voxels(smg_ctx* ctx, int* n_det = nullptr, int* n_holes = nullptr) {
     int nd = 0, nh = 0, nr = 0;

@@ -61,7 +61,8 @@ typedef struct smg_stats {
     uint64_t elements;         /* matrix elements produced */
     uint64_t rays;             /* element x sub-cell (x scatterer) terms */
     uint64_t fp64_fallbacks;   /* FP32 terms recomputed in FP64 */
-    uint64_t overflow_waves;   /* waves whose candidate lists overflowed */
+    uint64_t overflow_waves;   /* work items whose candidate lists overflowed LDS and were
+                                  redone with lists in global memory (same result) */
 } smg_stats;
 
 typedef struct smg_ctx smg_ctx;

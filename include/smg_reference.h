@@ -15,7 +15,8 @@
  *                 (defined ScatterGen_Crystal/scatter.cu:1528-1738)
  *
  * Same argument meaning: parameter arrays of 80000/80000/100/100 floats,
- * img[20] = rotation index, dst receives the N x V block of that rotation,
+ * img[20] = rotation index (a non-negative whole number: any other value
+ * prints a message and exits), dst receives the N x V block of that rotation,
  * phys[8]/phys[9] == 1 computes a geometry bitmap and writes it to the named
  * cache file, otherwise the file is read.  Returns the number of voxels.
  *

@@ -80,6 +80,11 @@ def elliptic_hole_shapes(n_holes, seed=0, circles_every=4):
 @pytest.fixture(scope="session")
 def lane_lib():
     """Host build of the GPU lane arithmetic (tests/lane_harness.cpp)."""
+    return load_lane_lib()
+
+
+def load_lane_lib():
+    """Builds (when a source is newer) and loads tests/lane_harness.cpp."""
     import ctypes as C
     out_dir = os.path.join(ROOT, "tests", "_build")
     os.makedirs(out_dir, exist_ok=True)
@@ -155,4 +160,35 @@ def narrow_plate_geometry(params_mod):
     for m in range(2):
         p.coll[(m + 1) * 10 + 1] = 9.0
         p.coll[(m + 1) * 10 + 3] = 7.0
+    return p
+
+
+def overflow_geometry(params_mod, n_layers=24, bars_per_side=12):
+    """A geometry whose voxel tiles need more candidates than the PE kernels'
+    LDS lists hold (512 bars, 256 holes): a 20 mm tungsten plate with 1,600
+    holes on a 1 mm grid, a deep block of n_layers layers of 2 mm cubic bars
+    right behind it, and 8 mm voxels, so a 4x4x4-voxel tile sees a wide
+    frustum through a thick plate.  The kernels then redo such tiles with
+    candidate lists in global memory."""
+    p = params_mod.Params.empty()
+    p.coll[0] = 1
+    T, nh = 20.0, 40
+    params_mod.set_layer(p, 0, nh * nh, 80.0, T, 80.0, 0.0, **params_mod.TUNGSTEN)
+    h = 0
+    for iz in range(nh):
+        for ix in range(nh):
+            params_mod.set_hole(p, h, (ix - (nh - 1) / 2) * 1.0, -T / 2, T / 2,
+                                (iz - (nh - 1) / 2) * 1.0, 0.3)
+            h += 1
+    n = bars_per_side
+    p.det[0] = n_layers * n * n
+    k = 0
+    for layer in range(n_layers):
+        for iz in range(n):
+            for ix in range(n):
+                params_mod.set_bar(p, k, (ix - (n - 1) / 2) * 2.0, T / 2 + 6.0 + 2.0 * layer,
+                                   (iz - (n - 1) / 2) * 2.0, 2.0, 2.0, 2.0, res=0.1, **params_mod.BAR_MU)
+                k += 1
+    params_mod.set_image(p, 8, 8, 4, 8.0, 8.0, 8.0, fov2coll=40.0)
+    params_mod.set_physics(p)
     return p

@@ -6,6 +6,7 @@
 // GPU), then the sub-cell rays in FP32 with the FP64 recompute of flagged
 // rays, or everything in FP64.  The GPU kernel adds the wave-level candidate
 // narrowing, which is exact by construction and is checked on the GPU.
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -125,7 +126,9 @@ extern "C" {
 // (bars) kept by a culling test of some lane, stats[6] = share of (sub-cell,
 // layer) plate gates skipped; the narrowed terms are computed with those
 // gates skipped; stats[7] = share of (wave, layer) gates cleared for the
-// whole bar (plate_gate_clear_ball), the rest tested per sub-cell.
+// whole bar (plate_gate_clear_ball), the rest tested per sub-cell;
+// stats[8], stats[9] = the largest tile candidate lists (bars, holes) over the
+// waves, which the kernels hold in LDS up to 512 bars and 256 holes.
 int lane_narrow_check(const float* coll, const float* det, const float* img, const float* phys,
                       int rot, int sx, int sy, int sz, int tx, int ty, int tz, int n_waves,
                       unsigned seed, double* stats) {
@@ -144,7 +147,7 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
         return st;
     };
     double bad = 0, checked = 0, nsb = 0, nsh = 0, nfull = 0, cells = 0, ntile = 0, nwav = 0, gates = 0,
-           bar_gates = 0;
+           bar_gates = 0, max_tb = 0, max_th = 0;
     for (int w = 0; w < n_waves; ++w) {
         int d = (int)(next() % g.n_det);
         int t = (int)(next() % ((uint64_t)nt[0] * nt[1] * nt[2]));
@@ -206,7 +209,12 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
             for (int h = 0; h < g.n_holes; ++h)
                 if (cull_hole<true>(G, h, P[l], pf, sc, udf, float(sc.crit))) hmask[h] |= 1ull << l;
         }
-        for (int k = 0; k < g.n_det; ++k) ntile += tb[k] && bmask[k];
+        double wtb = 0, wth = 0;
+        for (int k = 0; k < g.n_det; ++k) wtb += tb[k] && bmask[k];
+        for (int h = 0; h < g.n_holes; ++h) wth += th[h] && hmask[h];
+        ntile += wtb;
+        max_tb = std::max(max_tb, wtb);
+        max_th = std::max(max_th, wth);
         for (int cz = 0; cz < sz; ++cz)
             for (int a = 0; a < sx; ++a)
                 for (int b = 0; b < sy; ++b) {
@@ -266,6 +274,8 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
     stats[5] = nwav > 0 ? ntile / nwav : 0;
     stats[6] = cells > 0 ? gates / (cells * (g.n_layers > 0 ? g.n_layers : 1)) : 0;
     stats[7] = nwav > 0 ? bar_gates / (nwav * (g.n_layers > 0 ? g.n_layers : 1)) : 0;
+    stats[8] = max_tb;
+    stats[9] = max_th;
     return 0;
 }
 

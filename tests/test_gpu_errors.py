@@ -73,3 +73,35 @@ def test_bad_device_index(smg_mod):
     h = C.c_void_p()
     s = smg_mod.lib().smg_create(1000, C.byref(h))
     assert s != 0 and not h.value
+
+
+def test_reference_shim_refuses_fractional_rotation(params_mod, tmp_path):
+    """The reference multiplies img[20], as a float, by the angle step
+    (PEGen_RayTracing_CircularHole/PESysMatGen.cu:1004-1006); the C ABI takes
+    whole rotation indices, so the PESysMatGen shim refuses any other value
+    with a message and a failing exit status instead of rounding it."""
+    import os
+    import subprocess
+    import sys
+    import textwrap
+
+    from conftest import PKG
+    p = params_mod.config(1)
+    p.img[20] = 0.5
+    p.write(str(tmp_path))
+    code = textwrap.dedent(f"""
+        import ctypes as C, numpy as np, sys
+        sys.path.insert(0, {PKG!r})
+        import smg
+        L = smg.lib()
+        f = getattr(L, "_Z11PESysMatGenPfS_S_S_i")
+        arrs = [np.fromfile({str(tmp_path)!r} + "/Params_" + n + ".dat", np.float32)
+                for n in ("Collimator", "Detector", "Image")]
+        dst = np.zeros(64 * 1024, np.float32)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        f(ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]), ptr(dst), 0)
+        print("returned")
+    """)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "returned" not in r.stdout
+    assert "rotation index" in r.stderr

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from conftest import (TOL_FP32, TOL_FP64, assert_parity, elliptic_hole_shapes, narrow_plate_geometry,
-                      small_geometry)
+                      overflow_geometry, small_geometry)
 
 
 def lane_pe(L, p, d, v, sub, exact, rot=0):
@@ -88,7 +88,7 @@ def test_lane_narrowing_is_exact(lane_lib, params_mod, case):
         rot = 2
     else:
         p, sub, tile, waves = narrow_plate_geometry(params_mod), (3, 4, 2), (8, 4, 2), 60
-    st = np.zeros(8)
+    st = np.zeros(10)
     assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, rot, *sub, *tile, waves, 5, st) == 0
     print(f"narrowing {case}: {int(st[1])} terms, {st[2]:.2f} bars and {st[3]:.2f} holes per "
           f"sub-cell (full sets {st[4]:.1f} bars, tile candidates {st[5]:.1f}), "
@@ -220,3 +220,27 @@ def test_window_reject_cos(lane_lib):
     # nothing below the window: never rejects; window above e0 (1 + r): all
     assert lane_lib.lane_window_reject_cos(140.5, 0.0, 0.1, 1e-5) == -2.0
     assert lane_lib.lane_window_reject_cos(140.5, 1.2 * 140.5, 0.1, 1e-5) == 2.0
+
+
+def test_lane_overflow_geometry(lane_lib, oracle_mod, params_mod):
+    """The overflow geometry's tiles need more candidates than the kernels'
+    LDS lists hold (512 bars, 256 holes), so the kernels take their spill
+    path (lists in global memory with room for every bar and hole); the
+    narrowing stays exact there, and the lane arithmetic with full candidate
+    lists, which is what the spill path computes, matches the oracle."""
+    p = overflow_geometry(params_mod)
+    st = np.zeros(10)
+    assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, 0, 1, 2, 1, 4, 4, 4, 24, 7, st) == 0
+    print(f"overflow geometry: largest tile lists {st[8]:.0f} bars, {st[9]:.0f} holes", flush=True)
+    assert st[8] > 512 and st[9] > 256
+    assert st[1] > 0 and st[0] == 0
+    o = oracle_mod.Oracle(p)
+    rng = np.random.default_rng(44)
+    d = rng.integers(p.n_det // 2, o.n_det, 400)
+    v = rng.integers(0, o.n_vox, 400)
+    ref = o.pe_pairs(d, v, sub=(1, 2, 1))
+    assert (ref > 0).mean() > 0.5
+    for exact in (0, 1):
+        got, fb = lane_pe(lane_lib, p, d, v, (1, 2, 1), exact)
+        assert_parity(got, ref, TOL_FP64 if exact else TOL_FP32,
+                      f"lane PE overflow geometry exact={exact} fallbacks={fb}")
