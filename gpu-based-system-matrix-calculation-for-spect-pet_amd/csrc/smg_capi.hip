@@ -14,6 +14,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -47,8 +49,17 @@ struct DevBuf {
 
 }  // namespace
 
+// Scratch of smg_pe_forward_noalloc, the entry point for callers that cannot
+// allocate: the launch counters (first 256 bytes) and the sub-cell table.
+// Each GPU holds its own copy for the life of the process, shared by every
+// context on that GPU.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr size_t kNoAllocCounterBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 struct smg_ctx {
     int device = 0;
+    unsigned char* scratch_dev = nullptr;  // this device's address of `scratch`
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
@@ -188,6 +199,32 @@ smg_status ensure_spill(smg_ctx* c, PeLaunch& A) {
     S.queue = c->spill_queue.ptr;
     S.slots = c->spill_slots.ptr;
     return SMG_OK;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (hipGetSymbolAddress): host code uses this address, never the
+// symbol itself, which on the host is only a stand-in.
+hipError_t scratch_address(int device, unsigned char** ptr) {
+    static std::mutex mu;
+    static std::map<int, unsigned char*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) {
+        *ptr = it->second;
+        return hipSuccess;
+    }
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = (unsigned char*)p;
+    *ptr = (unsigned char*)p;
+    return hipSuccess;
+}
+
+// Clears n 64-bit counters (a kernel rather than a memset node, so the
+// no-alloc entry point's work can be recorded into a HIP graph).
+__global__ void clear_counters_kernel(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
 smg_status check_ready(smg_ctx* c, int rotation) {
@@ -457,7 +494,7 @@ smg_status smg_create(int device, smg_ctx** out) {
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        c->counters.reserve(4) != hipSuccess) {
+        c->counters.reserve(4) != hipSuccess || scratch_address(device, &c->scratch_dev) != hipSuccess) {
         smg_destroy(c);
         return SMG_ERR_HIP;
     }
@@ -695,6 +732,66 @@ smg_status smg_pe_forward(smg_ctx* c, int rotation, smg_precision prec, float* d
         stats->fp64_fallbacks = cnt[0];
         stats->overflow_waves = cnt[1];
     }
+    return SMG_OK;
+}
+
+// smg_pe_forward into a caller's device buffer with no allocation: the
+// counters and the sub-cell table live in this device's `scratch`, and tiles
+// beyond the LDS candidate lists (which smg_pe_forward redoes with lists it
+// allocates) are refused.
+smg_status smg_pe_forward_noalloc(smg_ctx* c, int rotation, smg_precision prec, float* dst_device,
+                                  smg_stats* stats) {
+    double t_start = now_ms();
+    smg_status s = check_ready(c, rotation);
+    if (s != SMG_OK) return s;
+    if (!dst_device) return fail(c, SMG_ERR_BAD_PARAM, "dst is NULL");
+    if (!c->scratch_dev) return fail(c, SMG_ERR_STATE, "no scratch address for device %d", c->device);
+    const GeomDesc& g = c->hg.g;
+    const size_t n_el = (size_t)g.n_det * (size_t)g.n_vox;
+    const size_t n_cells = (size_t)c->pe_sub[0] * c->pe_sub[1] * c->pe_sub[2];
+    const size_t need = kNoAllocCounterBytes + (size_t)g.n_det * n_cells * sizeof(CellCtx);
+    if (need > kNoAllocScratchBytes)
+        return fail(c, SMG_ERR_CAPACITY,
+                    "%d bars x %zu sub-cells need %zu bytes of scratch; smg_pe_forward_noalloc has %zu",
+                    g.n_det, n_cells, need, kNoAllocScratchBytes);
+    unsigned long long* counters = (unsigned long long*)c->scratch_dev;
+    CellCtx* cells = (CellCtx*)(c->scratch_dev + kNoAllocCounterBytes);
+    PeLaunch A{};
+    A.geom = device_geom(c);
+    A.dst = dst_device;
+    A.counters = counters;
+    const double phi = (double)rotation * g.dphi;
+    A.cphi = std::cos(phi);
+    A.sphi = std::sin(phi);
+    for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
+    pick_tile(g, A.tile, A.tiles);
+    A.cells = cells;
+    A.cells_bar0 = 0;
+    A.spill = PeSpill{};  // no spill area: overflowing work items are only counted
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, c->stream, counters, 4);
+    SMG_HIP(c, hipGetLastError());
+    SMG_HIP(c, launch_cell_table(A.geom, 0, g.n_det, A.sub, cells, c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    SMG_HIP(c, launch_pe_forward(A, pe_exact(c, prec), c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    SMG_HIP(c, hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    SMG_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t_start;
+        stats->elements = n_el;
+        stats->rays = n_el * (uint64_t)n_cells;
+        stats->fp64_fallbacks = cnt[0];
+        stats->overflow_waves = cnt[2];
+    }
+    if (cnt[2])
+        return fail(c, SMG_ERR_CAPACITY,
+                    "%llu voxel tiles need more than %d bars or %d holes of candidates; "
+                    "smg_pe_forward_noalloc has no spill area (smg_pe_forward handles them)",
+                    cnt[2], kCapBars, kCapHoles);
     return SMG_OK;
 }
 

@@ -38,6 +38,7 @@ EXPORTS = (
     "smg_bitmap_collimator", "smg_load_bitmap_crystal", "smg_load_bitmap_collimator",
     "smg_scatter_crystal", "smg_scatter_collimator", "smg_set_mu_tables",
     "smg_scatter_collimator_grid", "smg_scatter_crystal_grid", "smg_set_hole_shapes",
+    "smg_pe_forward_noalloc",
 )
 
 
@@ -96,6 +97,7 @@ def lib():
         "smg_set_mu_tables": (i32, [vp, i32, dp, dp, dp, dp, dp]),
         "smg_set_hole_shapes": (i32, [vp, fp, i32]),
         "smg_pe_forward": (i32, [vp, i32, i32, vp, i32, st]),
+        "smg_pe_forward_noalloc": (i32, [vp, i32, i32, vp, st]),
         "smg_pe_backward": (i32, [vp, i32, i32, vp, i32, dp, dp, st]),
         "smg_pe_project": (i32, [vp, i32, i32, vp, i32, vp, i32, st]),
         "smg_pe_backproject": (i32, [vp, i32, i32, vp, i32, vp, i32, st]),

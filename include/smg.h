@@ -131,6 +131,21 @@ smg_status smg_set_hole_shapes(smg_ctx* ctx, const float* shapes, int n_holes);
 smg_status smg_pe_forward(smg_ctx* ctx, int rotation, smg_precision prec, float* dst,
                           int dst_is_device, smg_stats* stats);
 
+/* smg_pe_forward with no allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  dst_device: a device
+ * buffer of N x V floats (the caller's); the geometry and sub-cells must be
+ * set beforehand (those calls allocate).  Same kernels and results as
+ * smg_pe_forward.  Its counters and sub-cell table live in a fixed 32 MiB
+ * scratch array per GPU, held until the process ends and shared by every
+ * context on that GPU, so calls that overlap on one GPU race.  Limits
+ * (SMG_ERR_CAPACITY): bars x sub-cells x 136 bytes must fit in 32 MiB - 256
+ * bytes (2048 bars at 3 x 3 x 3 sub-cells take 7.5 MB), and a voxel tile
+ * that needs more than 512 bars or 256 holes of candidates is refused (its
+ * elements are then left unwritten; smg_pe_forward handles such tiles). */
+smg_status smg_pe_forward_noalloc(smg_ctx* ctx, int rotation, smg_precision prec, float* dst_device,
+                                  smg_stats* stats);
+
 /* dL/dmu of L = sum_dv w[d][v] A[d][v] (adjoint of smg_pe_forward), for the
  * mu of each collimator layer, mu_total and mu_pe of each bar:
  * grad[0 .. Lc-1], grad[Lc .. Lc+N-1], grad[Lc+N .. Lc+2N-1].  w is N x V.
