@@ -72,10 +72,22 @@ def cpu_baseline(p, sub, target_s: float):
         "value": done / t_used,
         "unit": "matrix elements/s",
         "cores": threads,
+        "cpu_model": _cpu_model(),
         "kind": "port",
         "sample": f"{done} seeded random (bar, voxel) elements of the same geometry and "
                   f"sub-cells, FP64 oracle, {t_used:.1f} s on {threads} host threads",
     }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def ops_per_ray(p, sub):
@@ -166,6 +178,7 @@ def _crystal_scatter(np, smg, params, ex, precs):
         c.set_pe_subsamples(*p4.meta["pe_subsamples"])
         pe, _ = c.pe_forward(0, "fp32")
         c.set_scatter_subsamples(*p4.meta["subsamples"])
+        c.set_mu_tables(*params.synthetic_mu_tables(True))
         c.scatter_crystal(pe, 0, "fp32")  # builds the bitmap, warms up
         for prec in precs:
             _, st = c.scatter_crystal(pe, 0, prec)
@@ -176,7 +189,8 @@ def _crystal_scatter(np, smg, params, ex, precs):
                 "kernel_ms": round(ms, 3), "fp64_fallbacks": st["fp64_fallbacks"]}
             log(f"crystal scatter {prec}: {ms:.2f} ms")
         ex["crystal_scatter_workload"] = ("32x32x1 voxels x 2048 bars, scatter points at the "
-                                          "2047 other bar centres (reference semantics), 1x1x1 cells")
+                                          "2047 other bar centres (reference semantics), 1x1x1 cells, "
+                                          "64-node mu(E) tables")
 
 
 def _collimator_scatter(np, smg, params, ex, precs):
@@ -184,6 +198,7 @@ def _collimator_scatter(np, smg, params, ex, precs):
     p3 = params.config(3)
     with smg.open_context(p3) as c:
         c.set_scatter_subsamples(*p3.meta["subsamples"])
+        c.set_mu_tables(*params.synthetic_mu_tables(True))
         c.scatter_collimator(0, "combined", "fp32")
         for prec in precs:
             _, st = c.scatter_collimator(0, "combined", prec)
@@ -194,7 +209,8 @@ def _collimator_scatter(np, smg, params, ex, precs):
                 "kernel_ms": round(ms, 3), "fp64_fallbacks": st["fp64_fallbacks"]}
             log(f"collimator scatter {prec}: {ms:.2f} ms")
         ex["collimator_scatter_workload"] = ("32x32x16 voxels x 64 bars, scatter points at the 16 "
-                                             "hole centres (reference semantics), 1x1x1 cells")
+                                             "hole centres (reference semantics), 1x1x1 cells, "
+                                             "64-node mu(E) tables")
 
 
 def _collimator_grid(np, smg, params, ex, precs):
@@ -202,6 +218,7 @@ def _collimator_grid(np, smg, params, ex, precs):
     # collimator attenuation), 32x32x16 voxels, 64 bars
     p3 = params.config(3)
     with smg.open_context(p3) as c:
+        c.set_mu_tables(*params.synthetic_mu_tables(True))
         for prec in precs:
             _, st, n = c.scatter_collimator_grid(0.5, 0, prec)
             ex[f"collimator_grid_scatter_{prec}"] = {
@@ -211,7 +228,7 @@ def _collimator_grid(np, smg, params, ex, precs):
             log(f"collimator grid scatter {prec}: {st['kernel_ms']:.0f} ms, {n} points")
         ex["collimator_grid_scatter_workload"] = (f"32x32x16 voxels x 64 bars, {n} scatter points on "
                                                   "a 0.5 mm grid over the 60x5x60 mm tungsten, "
-                                                  "1x1x1 cells, power-law mu(E)")
+                                                  "1x1x1 cells, 64-node mu(E) tables")
 
 
 def _crystal_grid(np, smg, params, ex, precs):
@@ -220,6 +237,7 @@ def _crystal_grid(np, smg, params, ex, precs):
     p4 = params.config(4)
     rows = np.arange(0, p4.n_det, 256, dtype=np.int32)
     with smg.open_context(p4) as c:
+        c.set_mu_tables(*params.synthetic_mu_tables(True))
         for prec in precs:
             _, st = c.scatter_crystal_grid((10, 32, 10), rows, 0, prec)
             ex[f"crystal_grid_scatter_{prec}"] = {
@@ -230,7 +248,7 @@ def _crystal_grid(np, smg, params, ex, precs):
         ex["crystal_grid_scatter_workload"] = (
             f"32x32x1 voxels x {len(rows)} detector rows of 2048 bars, scatter points at the "
             f"centres of 10x32x10 cells (0.25 mm) of every bar ({p4.n_det * 3200} points), "
-            "1x1x1 cells, power-law mu(E)")
+            "1x1x1 cells, 64-node mu(E) tables")
 
 
 def _bitmaps(np, smg, params, ex):

@@ -282,6 +282,24 @@ def large_detector(n_layers: int = 3, nx: int = 50, nz: int = 40) -> Params:
     return p
 
 
+def synthetic_mu_tables(power_law: bool = True, e0: float = 140.5):
+    """mu(E) tables of the scatter workloads: 64 log-spaced nodes on 20-200
+    keV, returned as (energy_kev, crystal_pe, crystal_compton, coll_pe,
+    coll_compton) in 1/mm.  power_law: the configs' mu at e0 scaled as
+    (e0/E)^3 (photoelectric) and e0/E (Compton), which log-log interpolation
+    reproduces exactly; otherwise smooth tables with other slopes and a
+    modulation."""
+    E = 20.0 * 10.0 ** (np.arange(64) / 63.0)
+    r = e0 / E
+    cpe, cc = BAR_MU["mu_pe"], BAR_MU["mu_c"]
+    wpe, wc = TUNGSTEN["mu_pe"], TUNGSTEN["mu_c"]
+    if power_law:
+        return E, cpe * r ** 3, cc * r, wpe * r ** 3, wc * r
+    lE = np.log(E)
+    return (E, cpe * r ** 2.7 * (1 + 0.08 * np.sin(3 * lE)), cc * r ** 0.85 * (1 + 0.05 * np.cos(2 * lE)),
+            wpe * r ** 2.9 * (1 + 0.06 * np.sin(2.5 * lE)), wc * r ** 1.1)
+
+
 def grad_param_count(p: Params) -> int:
     """Gradient vector length: one mu per collimator layer, then mu_total and
     mu_pe of every bar."""

@@ -67,6 +67,10 @@ def lib():
                                 _dp, C.c_int]
     L.smg_o_pe_pairs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip,
                                  C.c_int, _dp, C.c_int]
+    L.smg_o_scatter_ops_per_term.restype = C.c_double
+    L.smg_o_scatter_ops_per_term.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_uint,
+                                             C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, _ip, _ip, C.c_int, C.POINTER(C.c_double)]
     L.smg_o_pe_ops_per_ray.restype = C.c_double
     L.smg_o_pe_ops_per_ray.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip,
                                        C.c_int]
@@ -232,6 +236,20 @@ class Oracle:
         d = np.ascontiguousarray(d, dtype=np.int32)
         v = np.ascontiguousarray(v, dtype=np.int32)
         return lib().smg_o_pe_ops_per_ray(self._g, rot, *sub, d, v, d.size)
+
+    def scatter_ops_per_term(self, kind, d, v, bitmap=None, pitch=0.5, layers=1, grid=(1, 1, 1),
+                             rot=0, sub=(1, 1, 1)):
+        """(algorithmic ops per scatter term, terms counted) of the reference's
+        scatter element body for rows d and voxels v; kind 0 crystal, 1
+        collimator (hole centres), 2 collimator grid, 3 crystal grid."""
+        d = np.ascontiguousarray(d, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        bm = None if bitmap is None else np.ascontiguousarray(bitmap, np.uint32)
+        terms = C.c_double()
+        ops = lib().smg_o_scatter_ops_per_term(
+            self._g, None if bm is None else bm.ctypes.data_as(C.c_void_p), kind, pitch, layers,
+            *grid, rot, *sub, d, v, d.size, C.byref(terms))
+        return ops, terms.value
 
     def cull_counts(self, d, v, rot=0):
         nc, nh = C.c_int(), C.c_int()

@@ -35,6 +35,7 @@
  * and the culling scans are not counted. */
 static _Thread_local int g_count_ops = 0;
 static _Thread_local double g_ops = 0.0;
+static _Thread_local double g_terms = 0.0; /* scatter terms evaluated while counting */
 #define OPS(n)                          \
     do {                                \
         if (g_count_ops) g_ops += (n);  \
@@ -382,6 +383,7 @@ static int hole_ellipse(const smg_o_geom* g, int h, double* xr, double* zr, doub
  * factorised form of SC:962-977: with c = cos(theta), a = E0/511,
  * f1 = a(1-c), f2 = 1+c^2:  f2/(1+f1)^2 * (1 + f1^2/(f2 (1+f1))). */
 double smg_o_kn(double theta, double e0) {
+    OPS(16); /* cos, alpha, f1, f2, the quotient as written (1 + f1 three times) */
     double c = cos(theta);
     double alpha = e0 / 511.0;
     double f1 = alpha * (1.0 - c);
@@ -394,8 +396,10 @@ double smg_o_kn(double theta, double e0) {
  * (SC:980-993). */
 double smg_o_kn_sum(double e0, double lo, double hi, double step) {
     int n = (int)((hi - lo) / step);
+    OPS(2);
     double s = 0.0;
     for (int i = 0; i <= n; ++i) {
+        OPS(6); /* node angle, sin, two products, sum */
         double th = lo + i * step;
         s += smg_o_kn(th, e0) * sin(th) * step;
     }
@@ -405,6 +409,7 @@ double smg_o_kn_sum(double e0, double lo, double hi, double step) {
 /* Angle between src -> a and a -> det, acos of the clamped cosine
  * (SC:996-1013). */
 double smg_o_cone_angle(const double src[3], const double a[3], const double det[3]) {
+    OPS(26); /* 6 differences, dot, two norms with sqrt, quotient, acos */
     double u[3] = {a[0] - src[0], a[1] - src[1], a[2] - src[2]};
     double w[3] = {det[0] - a[0], det[1] - a[1], det[2] - a[2]};
     double dot = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
@@ -418,12 +423,14 @@ double smg_o_cone_angle(const double src[3], const double a[3], const double det
 
 /* E' = E0 / (1 + (E0/511)(1 - cos theta))  (SC:1016-1023). */
 double smg_o_scatter_energy(double theta, double e0) {
+    OPS(6);
     return e0 / (1.0 + (e0 / 511.0) * (1.0 - cos(theta)));
 }
 
 /* Probability that a Gaussian of FWHM res*e centred on e falls in [lo, hi]
  * (SC:1025-1033). */
 double smg_o_gauss_window(double e, double res, double lo, double hi) {
+    OPS(12); /* sigma, sqrt(2) sigma, two arguments, two erf, difference, half */
     double sigma = res / FWHM_SIGMA * e;
     double s2 = sigma * sqrt(2.0);
     double z1 = (lo - e) / s2;
@@ -461,6 +468,7 @@ static bar_t get_bar(const smg_o_geom* g, int k) {
 /* World point -> bar frame: rotate by -beta about y around the bar centre
  * (PE:1071-1073, PE:1309-1311). */
 static void to_bar_frame(const bar_t* b, const double u[3], double out[3]) {
+    OPS(9); /* rotation trig of the bar is loop-invariant: not counted */
     double cb = cos(-b->beta), sb = sin(-b->beta);
     double x = u[0] - b->c[0], z = u[2] - b->c[2];
     out[0] = x * cb - z * sb;
@@ -470,6 +478,7 @@ static void to_bar_frame(const bar_t* b, const double u[3], double out[3]) {
 
 /* Bar frame -> world (PE:1211-1217). */
 static void from_bar_frame(const bar_t* b, const double s[3], double out[3]) {
+    OPS(9);
     double cb = cos(b->beta), sb = sin(b->beta);
     out[0] = b->c[0] + s[0] * cb - s[2] * sb;
     out[1] = b->c[1] + s[1];
@@ -505,6 +514,7 @@ static void cell_of(const bar_t* b, int a, int bb, int c, int sx, int sy, int sz
 }
 
 static double dist3(const double a[3], const double b[3]) {
+    OPS(9);
     double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
     return sqrt(dx * dx + dy * dy + dz * dz);
 }
@@ -766,6 +776,7 @@ void smg_o_pe_grad(const smg_o_geom* g, int rot, int sx, int sy, int sz, const d
         double* gp = part + (size_t)d * P;
         for (int v = 0; v < V; ++v) {
             double wv = w[(size_t)d * V + v];
+            if (wv == 0.0) continue; /* every term of this element is a multiple of wv */
             lrow[d] += wv * pe_element_core(g, d, v, rot, sx, sy, sz, wv, gp);
         }
     }
@@ -937,9 +948,11 @@ static double tab_lmu_at(const smg_o_geom* g, int which, double le) {
  * (SC:1371-1375). */
 double smg_o_mu_ratio(const smg_o_geom* g, int which, double e, double e0) {
     if (g->n_tab == 0) {
+        OPS((which & 1) ? 1 : 3);
         double r = e0 / e;
         return (which & 1) ? r : r * r * r;
     }
+    OPS(18); /* two logs, two interpolations, difference, exp */
     return exp(tab_lmu_at(g, which, log(e)) - tab_lmu_at(g, which, log(e0)));
 }
 
@@ -954,6 +967,7 @@ double smg_o_mu_ratio(const smg_o_geom* g, int which, double e, double e0) {
 
 /* Length of the segment a -> b inside the box [lo, hi]. */
 double smg_o_seg_box(const double a[3], const double b[3], const double lo[3], const double hi[3]) {
+    OPS(16); /* per axis: difference, two offsets, two quotients; the length */
     double t0 = 0.0, t1 = 1.0;
     for (int k = 0; k < 3; ++k) {
         double D = b[k] - a[k];
@@ -973,6 +987,7 @@ double smg_o_seg_box(const double a[3], const double b[3], const double lo[3], c
  * line x = xc, z = zc, for y1 <= y <= y2. */
 double smg_o_seg_cyl(const double a[3], const double b[3], double xc, double y1, double y2,
                      double zc, double r) {
+    OPS(35); /* direction, y slab, offsets, quadratic, sqrt, roots, length */
     double D[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
     double t0 = 0.0, t1 = 1.0;
     if (D[1] == 0.0) {
@@ -1130,28 +1145,34 @@ static double scatter_body(const smg_o_geom* g, const bar_t* bd, int d, const do
         for (int a = 0; a < sx; ++a)
             for (int bb = 0; bb < sy; ++bb) {
                 double qs[3], clo[3], chi[3], q[3];
+                if (g_count_ops) g_terms += 1.0;
+                OPS(30); /* cell centre and box */
                 cell_of(bd, a, bb, c, sx, sy, sz, qs, clo, chi);
                 from_bar_frame(bd, qs, q);
                 double theta = smg_o_cone_angle(p, qsw, q);
                 double ep = smg_o_scatter_energy(theta, ph->e0);
+                OPS(3 + 8); /* resolution at E', the two window tests */
                 double resp = bd->res * sqrt(ep / ph->e0);
                 /* 2-sigma energy-window early out (SC:1263-1269) */
                 if (ep * (1 + 2 * resp / FWHM_SIGMA) <= ph->lo || ep * (1 - 2 * resp / FWHM_SIGMA) >= ph->hi)
                     continue;
                 double pe_win = smg_o_gauss_window(ep, resp, ph->lo, ph->hi);
                 double L = dist3(qsw, q);
+                OPS(13 + 3); /* cell radius, the 2 pi test */
                 double rdet = sqrt(bd->w * bd->w / sx / sx + bd->h * bd->h / sz / sz +
                                    bd->t * bd->t / sy / sy) / 2.0;
                 double phi;
                 if (L * sin(theta) * 2.0 <= rdet) {
                     phi = 2.0 * M_PI;
                 } else {
+                    OPS(6);
                     double r = rdet / 2.0 / L / sin(theta);
                     phi = 4.0 * asin(r <= 1.0 ? r : 1.0);
                 }
                 /* polar range: cone angles to the 8 cell corners (SC:1290-1335) */
                 double dis = dist3(qss, ps);
                 double ext[3];
+                OPS(12);
                 for (int k = 0; k < 3; ++k) ext[k] = qss[k] + 1000.0 * (qss[k] - ps[k]) / dis;
                 double lthru = smg_o_box_ray(ps, ext, clo, chi);
                 double tmin = 0.0, tmax = 0.0;
@@ -1163,10 +1184,12 @@ static double scatter_body(const smg_o_geom* g, const bar_t* bd, int d, const do
                     if (corner == 0 || th < tmin) tmin = th;
                 }
                 if (lthru > (double)0.001f) tmin = 0.0;
+                OPS(1);
                 double ratio = smg_o_kn_sum(ph->e0, tmin, tmax, KN_STEP) / ph->norm;
                 /* second leg: bars flagged in the bitmap, mu scaled to E' (SC:1343-1401) */
                 double e3, e1; /* mu(E')/mu(E0): photoelectric, Compton (crystal) */
                 if (g->n_tab == 0) {
+                    OPS(6);
                     e3 = ph->e0 * ph->e0 * ph->e0 / ep / ep / ep;
                     e1 = ph->e0 / ep;
                 } else {
@@ -1187,6 +1210,7 @@ static double scatter_body(const smg_o_geom* g, const bar_t* bd, int d, const do
                     }
                     if (!bit) continue;
                     bar_t bk = get_bar(g, k);
+                    OPS(3 + 2); /* mu at E', its product with the chord and the sum */
                     double mut = bk.mupe * e3 + bk.muc * e1;
                     double sa[3], qa[3];
                     to_bar_frame(&bk, qsw, sa);
@@ -1202,6 +1226,7 @@ static double scatter_body(const smg_o_geom* g, const bar_t* bd, int d, const do
                     to_bar_frame(&bs, q, qa);
                     double lo[3] = {-0.5 * bs.w, -0.5 * bs.t, -0.5 * bs.h};
                     double hi[3] = {0.5 * bs.w, 0.5 * bs.t, 0.5 * bs.h};
+                    OPS(5);
                     att += smg_o_seg_box(sa, qa, lo, hi) * (bs.mupe * e3 + bs.muc * e1);
                 }
                 if (kind == 2) { /* collimator material on the second leg, at E' */
@@ -1211,6 +1236,7 @@ static double scatter_body(const smg_o_geom* g, const bar_t* bd, int d, const do
                 }
                 double l1 = smg_o_box_ray_inside(qss, qs, blo, bhi);
                 double l2 = smg_o_box_ray_inside(qss, qs, clo, chi);
+                OPS(4 + 3 + 12 + 14); /* own-bar mu at E', its path, extension, the term */
                 double mupe_d = bd->mupe * e3, mut_d = mupe_d + bd->muc * e1;
                 att += (l1 - l2) * mut_d;
                 double ext2[3];
@@ -1311,6 +1337,10 @@ void smg_o_coll_grid_scatter_pairs(const smg_o_geom* g, double pitch, unsigned l
  * a Compton interaction in that cell, dv mu_c,b / (4 pi |s - p|^2) exp(-att),
  * att being the PE ray's attenuation to s (PE:1240-1346 with the sub-cell's
  * own chord added back).  An extension with no reference counterpart. */
+static double grid_weight_culled(const smg_o_geom* g, const bar_t* bbp, const cull_t* clp,
+                                 const double p[3], const double ps[3], int gx, int gy, int gz,
+                                 int cell);
+
 double smg_o_crystal_grid_weight(const smg_o_geom* g, int b, int v, int rot, int gx, int gy,
                                  int gz, int cell) {
     bar_t bb = get_bar(g, b);
@@ -1320,6 +1350,18 @@ double smg_o_crystal_grid_weight(const smg_o_geom* g, int b, int v, int rot, int
     to_bar_frame(&bb, p, ps);
     cull_t cl;
     pe_cull(g, b, p, &cl);
+    double w = grid_weight_culled(g, &bb, &cl, p, ps, gx, gy, gz, cell);
+    cull_free(g, &cl);
+    return w;
+}
+
+/* The weight above with bar b's culling lists for the voxel already formed
+ * (they depend on the bar and the voxel only, not on the cell). */
+static double grid_weight_culled(const smg_o_geom* g, const bar_t* bbp, const cull_t* clp,
+                                 const double p[3], const double ps[3], int gx, int gy, int gz,
+                                 int cell) {
+    const bar_t bb = *bbp;
+    const cull_t cl = *clp;
     const int by = cell % gy, a = (cell / gy) % gx, c = cell / (gy * gx);
     double qs[3], clo[3], chi[3], q[3];
     cell_of(&bb, a, by, c, gx, gy, gz, qs, clo, chi);
@@ -1333,7 +1375,6 @@ double smg_o_crystal_grid_weight(const smg_o_geom* g, int b, int v, int rot, int
     double att = pe_ray_att(g, &bb, &cl, p, ps, q, qs, clo, chi, cosang, lay_len, bar_len, &l1, &l2);
     free(lay_len);
     free(bar_len);
-    cull_free(g, &cl);
     double dv = bb.w * bb.t * bb.h / ((double)gx * gy * gz);
     return exp(-(att + l2 * bb.mut)) * dv * bb.muc / (4.0 * M_PI * dsq);
 }
@@ -1357,14 +1398,22 @@ double smg_o_crystal_grid_scatter_element(const smg_o_geom* g, const uint32_t* b
     for (int b = 0; b < g->n_det; ++b) {
         if (b == d) continue;
         bar_t bb = get_bar(g, b);
+        const int live = bb.mut > (double)0.01f;
+        double pb[3];
+        cull_t cl;
+        if (live) {  /* bar b's culling for this voxel, shared by its cells */
+            to_bar_frame(&bb, p, pb);
+            pe_cull(g, b, p, &cl);
+        }
         for (int cell = 0; cell < nc; ++cell) {
             const int by = cell % gy, a = (cell / gy) % gx, c = cell / (gy * gx);
             double qs[3], clo[3], chi[3], s[3];
             cell_of(&bb, a, by, c, gx, gy, gz, qs, clo, chi);
             from_bar_frame(&bb, qs, s);
-            double w = smg_o_crystal_grid_weight(g, b, v, rot, gx, gy, gz, cell);
+            double w = live ? grid_weight_culled(g, &bb, &cl, p, pb, gx, gy, gz, cell) : 0.0;
             acc += scatter_body(g, &bd, d, p, s, w, bitmap, 3, b, sx, sy, sz, &ph);
         }
+        if (live) cull_free(g, &cl);
     }
     return acc;
 }
@@ -1396,4 +1445,70 @@ void smg_o_collimator_scatter_pairs(const smg_o_geom* g, const uint32_t* bitmap,
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads)
     for (int i = 0; i < n; ++i)
         out[i] = smg_o_collimator_scatter_element(g, bitmap, d[i], v[i], rot, variant, sx, sy, sz);
+}
+
+/* Algorithmic operations per scatter term (one (detector row, voxel,
+ * scatter point, detection cell) term of the scatter element, SC:1226-1428):
+ * the element body of the reference formula on the branches it takes, for
+ * the rows d[i] and voxels v[i], counted on the calling thread, with unit
+ * weights (the first-leg weights of the grid scatterers, their own kernels'
+ * work, are not counted).  kind: 0 crystal scatter (other bar centres, crystal
+ * bitmap), 1 collimator scatter (layer-0 hole centres, collimator bitmap),
+ * 2 collimator grid (points of a pitch grid over the layers in layer_mask),
+ * 3 crystal grid (centres of gx x gy x gz cells of the other bars, crystal
+ * bitmap).  *terms receives the number of terms. */
+double smg_o_scatter_ops_per_term(const smg_o_geom* g, const uint32_t* bitmap, int kind,
+                                  double pitch, unsigned layer_mask, int gx, int gy, int gz, int rot,
+                                  int sx, int sy, int sz, const int* d, const int* v, int n,
+                                  double* terms) {
+    double* pts = NULL;
+    int64_t np = 0;
+    if (kind == 2) {
+        np = smg_o_coll_grid_points(g, pitch, layer_mask, NULL, 0);
+        pts = (double*)malloc(sizeof(double) * 4 * (size_t)(np > 0 ? np : 1));
+        smg_o_coll_grid_points(g, pitch, layer_mask, pts, np);
+    }
+    g_count_ops = 1;
+    g_ops = 0.0;
+    g_terms = 0.0;
+    for (int i = 0; i < n; ++i) {
+        bar_t bd = get_bar(g, d[i]);
+        phys_t ph = phys_for(g, &bd);
+        if (ph.compton != 1 || !(bd.mut > (double)0.01f)) continue;
+        double p[3];
+        voxel_pos(g, v[i], rot, p);
+        if (kind == 0 || kind == 3) {
+            const int nc = kind == 0 ? 1 : gx * gy * gz;
+            for (int b = 0; b < g->n_det; ++b) {
+                if (b == d[i]) continue;
+                bar_t bb = get_bar(g, b);
+                for (int cell = 0; cell < nc; ++cell) {
+                    double s3[3];
+                    if (kind == 0) {
+                        s3[0] = bb.c[0]; s3[1] = bb.c[1]; s3[2] = bb.c[2];
+                    } else {
+                        const int by = cell % gy, a = (cell / gy) % gx, c = cell / (gy * gx);
+                        double qs[3], clo[3], chi[3];
+                        cell_of(&bb, a, by, c, gx, gy, gz, qs, clo, chi);
+                        from_bar_frame(&bb, qs, s3);
+                    }
+                    scatter_body(g, &bd, d[i], p, s3, 1.0, bitmap, kind, b, sx, sy, sz, &ph);
+                }
+            }
+        } else if (kind == 1) {
+            for (int h = 0; h < g->n_holes_l0; ++h) {
+                const float* hp = g->coll + 100 + 9 * h;
+                double qsw[3] = {hp[0], ((double)hp[1] + (double)hp[2]) / 2.0 + g->F, hp[3]};
+                OPS(20); /* hole centre, distance, the virtual-disc weight */
+                scatter_body(g, &bd, d[i], p, qsw, 1.0, bitmap, 1, h, sx, sy, sz, &ph);
+            }
+        } else {
+            for (int64_t k = 0; k < np; ++k)
+                scatter_body(g, &bd, d[i], p, pts + 4 * k, 1.0, NULL, 2, -1, sx, sy, sz, &ph);
+        }
+    }
+    g_count_ops = 0;
+    free(pts);
+    if (terms) *terms = g_terms;
+    return g_terms > 0 ? g_ops / g_terms : 0.0;
 }

@@ -75,6 +75,14 @@ void smg_o_pe_cull_counts(const smg_o_geom* g, int d, int v, int rot, int* n_cry
 /* Algorithmic arithmetic operations per sub-cell ray of the reference PE
  * formula (branches taken; culling scans and loop-invariant trig excluded),
  * averaged over the listed elements. */
+/* Algorithmic operations per scatter term (row, voxel, scatter point,
+ * detection cell) of the reference's scatter element body on the branches
+ * taken; kind 0 crystal, 1 collimator (hole centres), 2 collimator grid,
+ * 3 crystal grid.  Counted on the calling thread; *terms = terms counted. */
+double smg_o_scatter_ops_per_term(const smg_o_geom* g, const uint32_t* bitmap, int kind,
+                                  double pitch, unsigned layer_mask, int gx, int gy, int gz, int rot,
+                                  int sx, int sy, int sz, const int* d, const int* v, int n,
+                                  double* terms);
 double smg_o_pe_ops_per_ray(const smg_o_geom* g, int rot, int sx, int sy, int sz, const int* d,
                             const int* v, int n);
 

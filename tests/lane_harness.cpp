@@ -128,7 +128,10 @@ extern "C" {
 // gates skipped; stats[7] = share of (wave, layer) gates cleared for the
 // whole bar (plate_gate_clear_ball), the rest tested per sub-cell;
 // stats[8], stats[9] = the largest tile candidate lists (bars, holes) over the
-// waves, which the kernels hold in LDS up to 512 bars and 256 holes.
+// waves, which the kernels hold in LDS up to 512 bars and 256 holes;
+// stats[10] = the most narrowed bars of one sub-cell, stats[11] = the share of
+// sub-cells with more than 8 (the adjoint keeps the chords of the first 8 in
+// LDS and recomputes the others).
 int lane_narrow_check(const float* coll, const float* det, const float* img, const float* phys,
                       int rot, int sx, int sy, int sz, int tx, int ty, int tz, int n_waves,
                       unsigned seed, double* stats) {
@@ -147,7 +150,7 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
         return st;
     };
     double bad = 0, checked = 0, nsb = 0, nsh = 0, nfull = 0, cells = 0, ntile = 0, nwav = 0, gates = 0,
-           bar_gates = 0, max_tb = 0, max_th = 0;
+           bar_gates = 0, max_tb = 0, max_th = 0, max_nsb = 0, over8 = 0;
     for (int w = 0; w < n_waves; ++w) {
         int d = (int)(next() % g.n_det);
         int t = (int)(next() % ((uint64_t)nt[0] * nt[1] * nt[2]));
@@ -247,6 +250,8 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
                     }
                     nsb += nar.b.size();
                     nsh += nar.h.size();
+                    max_nsb = std::max(max_nsb, (double)nar.b.size());
+                    over8 += nar.b.size() > 8;
                     cells += 1;
                     // the same sub-cell with the tile's plate gates skipped
                     CellCtx cg = cc;
@@ -276,6 +281,8 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
     stats[7] = nwav > 0 ? bar_gates / (nwav * (g.n_layers > 0 ? g.n_layers : 1)) : 0;
     stats[8] = max_tb;
     stats[9] = max_th;
+    stats[10] = max_nsb;
+    stats[11] = cells > 0 ? over8 / cells : 0;
     return 0;
 }
 

@@ -209,16 +209,9 @@ def test_compton_off_gives_zero(smg_mod, params_mod):
 
 def mu_tables(power_law: bool, e0: float = 140.5):
     """64 log-spaced nodes on 20-200 keV (crystal pe, crystal Compton,
-    collimator pe, collimator Compton).  power_law: mu(E0) (E0/E)^3 and
-    mu(E0) E0/E, which log-log interpolation reproduces exactly; otherwise
-    smooth tables with other slopes and a modulation."""
-    E = 20.0 * 10.0 ** (np.arange(64) / 63.0)
-    r = e0 / E
-    if power_law:
-        return E, 0.18 * r ** 3, 0.07 * r, 3.3 * r ** 3, 0.35 * r
-    lE = np.log(E)
-    return (E, 0.18 * r ** 2.7 * (1 + 0.08 * np.sin(3 * lE)), 0.07 * r ** 0.85 * (1 + 0.05 * np.cos(2 * lE)),
-            3.3 * r ** 2.9 * (1 + 0.06 * np.sin(2.5 * lE)), 0.35 * r ** 1.1)
+    collimator pe, collimator Compton): smg.params.synthetic_mu_tables."""
+    from smg import params
+    return params.synthetic_mu_tables(power_law, e0)
 
 
 @pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("fp64", 1e-12)])
@@ -345,20 +338,72 @@ def test_crystal_grid_rows_match_full_matrix(smg_mod, params_mod):
     np.testing.assert_allclose(part, full[rows], rtol=2e-6, atol=0.0)
 
 
-def test_collimator_grid_scatter_half_mm_pitch(smg_mod, oracle_mod, params_mod):
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
+def test_collimator_grid_scatter_half_mm_pitch(smg_mod, oracle_mod, params_mod, prec, tol):
     """The benchmark's collimator grid: 0.5 mm pitch over the 60x5x60 mm
-    tungsten (143,440 points), config-3 image (32x32x16) and 64 bars, FP32
-    path, sampled elements against the oracle."""
+    tungsten (143,440 points), config-3 image (32x32x16) and 64 bars, with
+    the benchmark's 64-node mu(E) tables, sampled elements against the
+    oracle."""
     p = params_mod.config(3)
     o = oracle_mod.Oracle(p)
     d, v = _sample(o, 48, 17)
+    tabs = mu_tables(True)
     with smg_mod.open_context(p) as ctx:
-        got, st, n = ctx.scatter_collimator_grid(0.5, 0, "fp32")
+        ctx.set_mu_tables(*tabs)
+        got, st, n = ctx.scatter_collimator_grid(0.5, 0, prec)
     assert n > 140000
+    o.set_mu_tables(*tabs)
     ref = o.coll_grid_scatter_pairs(0.5, d, v)
     assert (ref > 0).all()
-    assert_parity(got[d, v], ref, TOL_FP32,
-                  f"collimator grid 0.5 mm fp32 (fallbacks {st['fp64_fallbacks']})")
+    assert_parity(got[d, v], ref, tol,
+                  f"collimator grid 0.5 mm, mu(E) tables, {prec} (fallbacks {st['fp64_fallbacks']})")
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
+def test_crystal_grid_scatter_bench_size(smg_mod, params_mod, prec, tol):
+    """The benchmark's crystal grid workload itself: scatter points at the
+    centres of 10x32x10 cells (0.25 mm) of all 2048 bars (6,553,600 points),
+    the 8 detector rows it times, the 64-node mu(E) tables, against committed
+    FP64-oracle values of 12 elements (tests/golden/make_scatter_golden.py:
+    the oracle needs ~1 min per element here)."""
+    import os
+    from conftest import ROOT
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "cfg4_crystal_grid_0p25mm.npz"))
+    p = params_mod.config(4)
+    rows = fx["rows"]
+    with smg_mod.open_context(p) as ctx:
+        ctx.set_mu_tables(*mu_tables(True))
+        got, st = ctx.scatter_crystal_grid(tuple(int(x) for x in fx["grid"]), rows, 0, prec)
+    ri = np.searchsorted(rows, fx["d"])
+    assert np.array_equal(rows[ri], fx["d"])
+    assert (fx["value"] > 0).all()
+    assert_parity(got[ri, fx["v"]], fx["value"], tol,
+                  f"crystal grid 0.25 mm, 2048 bars, {prec} (fallbacks {st['fp64_fallbacks']})")
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])
+def test_crystal_scatter_2048_bars(smg_mod, oracle_mod, params_mod, prec, tol):
+    """The benchmark's bar-centre crystal scatter (ScatterGen_Crystal path):
+    32x32x1 voxels x 2048 bars, 2047 scatter points per element, PE block
+    from the GPU, power laws and 64-node mu(E) tables, sampled elements
+    against the oracle."""
+    p = params_mod.config(4)
+    o = oracle_mod.Oracle(p)
+    d, v = _sample(o, 300, 61)
+    with smg_mod.open_context(p) as ctx:
+        ctx.set_pe_subsamples(*p.meta["pe_subsamples"])
+        pe, _ = ctx.pe_forward(0, "fp32")
+        got, st = ctx.scatter_crystal(pe, 0, prec)
+        ctx.set_mu_tables(*mu_tables(True))
+        got_t, _ = ctx.scatter_crystal(pe, 0, prec)
+        bm = ctx.bitmap_crystal()
+    ref = o.crystal_scatter_pairs(pe, bm, d, v)
+    assert (ref > 0).mean() > 0.5
+    assert_parity(got[d, v], ref, tol, f"crystal scatter, 2048 bars, {prec} "
+                                        f"(fallbacks {st['fp64_fallbacks']})")
+    o.set_mu_tables(*mu_tables(True))
+    ref_t = o.crystal_scatter_pairs(pe, bm, d, v)
+    assert_parity(got_t[d, v], ref_t, tol, f"crystal scatter, 2048 bars, mu(E) tables, {prec}")
 
 
 @pytest.mark.parametrize("prec,tol", [("fp32", TOL_FP32), ("fp64", TOL_FP64)])

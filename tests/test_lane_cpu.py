@@ -88,7 +88,7 @@ def test_lane_narrowing_is_exact(lane_lib, params_mod, case):
         rot = 2
     else:
         p, sub, tile, waves = narrow_plate_geometry(params_mod), (3, 4, 2), (8, 4, 2), 60
-    st = np.zeros(10)
+    st = np.zeros(12)
     assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, rot, *sub, *tile, waves, 5, st) == 0
     print(f"narrowing {case}: {int(st[1])} terms, {st[2]:.2f} bars and {st[3]:.2f} holes per "
           f"sub-cell (full sets {st[4]:.1f} bars, tile candidates {st[5]:.1f}), "
@@ -229,7 +229,7 @@ def test_lane_overflow_geometry(lane_lib, oracle_mod, params_mod):
     narrowing stays exact there, and the lane arithmetic with full candidate
     lists, which is what the spill path computes, matches the oracle."""
     p = overflow_geometry(params_mod)
-    st = np.zeros(10)
+    st = np.zeros(12)
     assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, 0, 1, 2, 1, 4, 4, 4, 24, 7, st) == 0
     print(f"overflow geometry: largest tile lists {st[8]:.0f} bars, {st[9]:.0f} holes", flush=True)
     assert st[8] > 512 and st[9] > 256
@@ -244,3 +244,18 @@ def test_lane_overflow_geometry(lane_lib, oracle_mod, params_mod):
         got, fb = lane_pe(lane_lib, p, d, v, (1, 2, 1), exact)
         assert_parity(got, ref, TOL_FP64 if exact else TOL_FP32,
                       f"lane PE overflow geometry exact={exact} fallbacks={fb}")
+
+
+def test_lane_full_size_long_candidate_lists(lane_lib, params_mod):
+    """On the full-size geometry with a 16x16x8 image (the reduced image of
+    the full-size adjoint test), some sub-cells keep more than 8 narrowed
+    bars, so that test covers the adjoint's recomputed chords beyond the 8 it
+    keeps in LDS; the narrowing is exact there."""
+    p = params_mod.config(2)
+    params_mod.set_image(p, 16, 16, 8, 0.5, 0.5, 0.5)
+    st = np.zeros(12)
+    assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, 0, 3, 3, 3, 4, 4, 4, 40, 9, st) == 0
+    print(f"full size, 16x16x8 image: up to {st[10]:.0f} narrowed bars per sub-cell, "
+          f"{100 * st[11]:.1f}% of sub-cells above 8", flush=True)
+    assert st[0] == 0 and st[1] > 0
+    assert st[10] > 8 and st[11] > 0.01

@@ -175,3 +175,24 @@ def test_window_excludes_far_energies(oracle_mod, params_mod):
     bm = o.bitmap_collimator()
     out = o.collimator_scatter_pairs(bm, np.arange(64), np.arange(64) * 16)
     assert np.all(out == 0.0)
+
+
+def test_scatter_op_counter_counts_terms(oracle_mod, params_mod):
+    """The scatter op counter visits every (scatter point, detection cell)
+    term of the sampled elements once: the 63 other bar centres, the 16 hole
+    centres, the grid cells of the other bars, times the detection cells."""
+    p = params_mod.config(3)
+    o = oracle_mod.Oracle(p)
+    d = np.array([3, 40], np.int32)
+    v = np.array([100, 9000], np.int32)
+    bm_c = o.bitmap_crystal()
+    bm_l = o.bitmap_collimator()
+    ops0, t0 = o.scatter_ops_per_term(0, d, v, bitmap=bm_c)
+    ops1, t1 = o.scatter_ops_per_term(1, d, v, bitmap=bm_l)
+    ops3, t3 = o.scatter_ops_per_term(3, d, v, bitmap=bm_c, grid=(1, 2, 1))
+    assert t0 == 2 * 63 and t1 == 2 * 16 and t3 == 2 * 63 * 2
+    ops_s, t_s = o.scatter_ops_per_term(0, d, v, bitmap=bm_c, sub=(2, 1, 2))
+    assert t_s == 4 * t0
+    # the cheapest term is the window-rejected one (cell, cone angle, E', tests)
+    assert 60 < min(ops0, ops1, ops3)
+    assert ops0 < 20000 and ops1 < 20000
