@@ -38,6 +38,16 @@ PEAK_HBM_GBS = 8000.0
 # measured HBM traffic of one FP32 launch of the full-size PE step (rocprofv3 PMC
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a plain 1 GiB copy)
 TRAFFIC_BYTES = 2.24e9
+# executed work of the same launch (rocprofv3 PMC, FP32 path): FLOPs per sub-cell ray
+# the lanes execute (64 x SQ_INSTS_VALU_FLOPS_FP32 + _FP64 per ray) and VALU wave
+# instructions per ray x 64
+EXEC_SOURCE = "profiles/pe_forward_instruction_mix_fp32_fp64_64x64x32_2048bars.md"
+EXEC_FLOPS_PER_RAY = 362.0 + 267.9
+VALU_INSTS_PER_RAY = 1136.7
+# unpacked FP32 (= FP64) vector FMA peak, and the VALU issue peak (256 CUs x 4 SIMDs x
+# 2.4 GHz, one wave64 instruction per 4 cycles)
+PEAK_FP32_UNPACKED_TFLOPS = 78.6
+PEAK_VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def log(*a):
@@ -104,6 +114,22 @@ def ops_per_ray(p, sub):
     return o.pe_ops_per_ray(d, v, sub=sub)
 
 
+def scatter_roofline(params, p, kind, d, v, kernel_ms, terms, sample, **kw):
+    """Roofline of a scatter launch: algorithmic operations per (row, voxel,
+    scatter point, cell) term of the reference's scatter element body, counted
+    by the oracle's op counter on a seeded sample, times the launch's terms."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    o = oracle.Oracle(p)
+    o.set_mu_tables(*params.synthetic_mu_tables(True))
+    f, n = o.scatter_ops_per_term(kind, d, v, **kw)
+    achieved = f * terms / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "alg_ops_per_term": round(f, 1), "achieved": round(achieved, 3),
+            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "op_count_sample": f"{sample} ({int(n)} terms)"}
+
+
 def timed(fn, reps):
     """Median kernel time (ms) of reps calls of fn() -> (result, stats)."""
     ks = sorted(fn()[1]["kernel_ms"] for _ in range(reps))
@@ -155,6 +181,24 @@ def _headline_extras(np, pe_ctx, out, precision):
     ex["pe_fp64_over_fp32_time"] = round((ms_other / ms_this) if other == "fp64" else (ms_this / ms_other), 3)
     log(f"PE {other}: {ms_other:.1f} ms")
 
+    # generation time as the drivers see it: the same step into host memory
+    # (kernels + device-to-host copy of the 1 GiB block, overlapped by row chunks)
+    host = np.empty((pe_ctx.dims.n_det, pe_ctx.dims.n_vox), np.float32)
+    host.fill(0.0)  # first touch outside the timing
+    pe_ctx.pe_forward(0, precision, out=host)
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        pe_ctx.pe_forward(0, precision, out=host)
+        walls.append((time.perf_counter() - t0) * 1e3)
+    wall = sorted(walls)[1]
+    ex["pe_generation_incl_d2h"] = {
+        "ms": round(wall, 2), "elements_per_s": round(host.size / wall * 1e3, 1),
+        "bytes_to_host": host.nbytes,
+        "what": "wall time of smg_pe_forward with a pageable host output: kernels plus the "
+                "device-to-host copy, which overlaps the kernels in 8 row chunks"}
+    log(f"PE into host memory: {wall:.1f} ms")
+
     # matrix-free projections with the same matrix (elements formed in-kernel, never stored)
     rng = np.random.default_rng(4)
     xv = rng.uniform(0, 1, pe_ctx.dims.n_vox).astype(np.float32)
@@ -188,6 +232,11 @@ def _crystal_scatter(np, smg, params, ex, precs):
                 "pairs_per_s": round(st["rays"] / ms * 1e3, 1),
                 "kernel_ms": round(ms, 3), "fp64_fallbacks": st["fp64_fallbacks"]}
             log(f"crystal scatter {prec}: {ms:.2f} ms")
+            if prec == "fp32":
+                rng = np.random.default_rng(13)
+                d, v = rng.integers(0, p4.n_det, 16), rng.integers(0, p4.n_vox, 16)
+                ex["crystal_scatter_fp32"]["roofline"] = scatter_roofline(
+                    params, p4, 0, d, v, ms, st["rays"], "16 seeded elements", bitmap=c.bitmap_crystal())
         ex["crystal_scatter_workload"] = ("32x32x1 voxels x 2048 bars, scatter points at the "
                                           "2047 other bar centres (reference semantics), 1x1x1 cells, "
                                           "64-node mu(E) tables")
@@ -208,6 +257,11 @@ def _collimator_scatter(np, smg, params, ex, precs):
                 "pairs_per_s": round(st["rays"] / ms * 1e3, 1),
                 "kernel_ms": round(ms, 3), "fp64_fallbacks": st["fp64_fallbacks"]}
             log(f"collimator scatter {prec}: {ms:.2f} ms")
+            if prec == "fp32":
+                rng = np.random.default_rng(14)
+                d, v = rng.integers(0, p3.n_det, 64), rng.integers(0, p3.n_vox, 64)
+                ex["collimator_scatter_fp32"]["roofline"] = scatter_roofline(
+                    params, p3, 1, d, v, ms, st["rays"], "64 seeded elements", bitmap=c.bitmap_collimator())
         ex["collimator_scatter_workload"] = ("32x32x16 voxels x 64 bars, scatter points at the 16 "
                                              "hole centres (reference semantics), 1x1x1 cells, "
                                              "64-node mu(E) tables")
@@ -226,6 +280,11 @@ def _collimator_grid(np, smg, params, ex, precs):
                 "pairs_per_s": round(st["rays"] / st["kernel_ms"] * 1e3, 1),
                 "kernel_ms": round(st["kernel_ms"], 1), "fp64_fallbacks": st["fp64_fallbacks"]}
             log(f"collimator grid scatter {prec}: {st['kernel_ms']:.0f} ms, {n} points")
+            if prec == "fp32":
+                rng = np.random.default_rng(15)
+                d, v = rng.integers(0, p3.n_det, 2), rng.integers(0, p3.n_vox, 2)
+                ex["collimator_grid_scatter_fp32"]["roofline"] = scatter_roofline(
+                    params, p3, 2, d, v, st["kernel_ms"], st["rays"], "2 seeded elements", pitch=0.5)
         ex["collimator_grid_scatter_workload"] = (f"32x32x16 voxels x 64 bars, {n} scatter points on "
                                                   "a 0.5 mm grid over the 60x5x60 mm tungsten, "
                                                   "1x1x1 cells, 64-node mu(E) tables")
@@ -245,6 +304,14 @@ def _crystal_grid(np, smg, params, ex, precs):
                 "pairs_per_s": round(st["rays"] / st["kernel_ms"] * 1e3, 1),
                 "kernel_ms": round(st["kernel_ms"], 1), "fp64_fallbacks": st["fp64_fallbacks"]}
             log(f"crystal grid scatter {prec}: {st['kernel_ms']:.0f} ms")
+            if prec == "fp32":
+                rng = np.random.default_rng(16)
+                d, v = rows[rng.integers(0, len(rows), 2)], rng.integers(0, p4.n_vox, 2)
+                ex["crystal_grid_scatter_fp32"]["roofline"] = scatter_roofline(
+                    params, p4, 3, d, v, st["kernel_ms"], st["rays"],
+                    "2 seeded elements of the timed rows on a 2x8x2 grid (the per-term count does not "
+                    "depend on the grid pitch; the 10x32x10 count takes ~30 s per element)",
+                    bitmap=c.bitmap_crystal(), grid=(2, 8, 2))
         ex["crystal_grid_scatter_workload"] = (
             f"32x32x1 voxels x {len(rows)} detector rows of 2048 bars, scatter points at the "
             f"centres of 10x32x10 cells (0.25 mm) of every bar ({p4.n_det * 3200} points), "
@@ -393,6 +460,20 @@ def main():
         "rays_per_launch": rays,
         "write_GBps": round(n_el * 4 / avg_kernel_s / 1e9, 2),
     }
+    if args.precision == "fp32":
+        ex_achieved = EXEC_FLOPS_PER_RAY * rays / avg_kernel_s / 1e12
+        issue = VALU_INSTS_PER_RAY * rays / 64 / avg_kernel_s
+        roofline["executed"] = {
+            "what": "FLOPs the lanes execute (PMC), against the unpacked FP32/FP64 vector peak",
+            "flops_per_ray": EXEC_FLOPS_PER_RAY,
+            "achieved": round(ex_achieved, 2),
+            "peak": PEAK_FP32_UNPACKED_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(ex_achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
+            "valu_insts_per_ray": VALU_INSTS_PER_RAY,
+            "valu_issue_frac": round(issue / PEAK_VALU_WAVE_INSTS_PER_S, 4),
+            "source": EXEC_SOURCE,
+        }
     ex = None if args.no_extras else extras(smg, params, ctx, out, args.precision)
     cpu = cpu_baseline(p, sub, args.cpu_seconds) if args.cpu_seconds > 0 else None
     line = {

@@ -49,6 +49,10 @@ struct DevBuf {
 
 }  // namespace
 
+// A PE call with a host output of at least 128 MiB computes the rows in up to
+// this many chunks and copies each chunk to the host while the next is computed.
+constexpr int kCopyChunks = 8;
+
 // Scratch of smg_pe_forward_noalloc, the entry point for callers that cannot
 // allocate: the launch counters (first 256 bytes) and the sub-cell table.
 // Each GPU holds its own copy for the life of the process, shared by every
@@ -61,7 +65,9 @@ struct smg_ctx {
     int device = 0;
     unsigned char* scratch_dev = nullptr;  // this device's address of `scratch`
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // device-to-host copies that overlap the kernels
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_rows[kCopyChunks] = {};  // end of each row chunk of a host-output PE call
     std::string err;
     bool have_geom = false;
     HostGeom hg;
@@ -76,10 +82,12 @@ struct smg_ctx {
     DevBuf<GroupBox> bar_groups, hole_groups;
     DevBuf<float> out;        // staging for host outputs
     DevBuf<float> in;         // staging for host inputs (PE weights)
+    DevBuf<float> out2, in2;  // second output and input of smg_scatter_sum
     DevBuf<unsigned long long> counters;
     DevBuf<uint32_t> bm_crystal, bm_coll;
     bool have_bm_crystal = false, have_bm_coll = false;
     DevBuf<double> scratch;   // adjoint partials
+    DevBuf<double> tan_v;     // tangent direction
     DevBuf<double> grad;
     DevBuf<Sphere<float>> sph_f;       // bars, raw frame (bitmaps)
     DevBuf<Sphere<double>> sph_d;
@@ -99,6 +107,8 @@ struct smg_ctx {
     bool elliptic_holes = false;  // smg_set_hole_shapes made a hole elliptical
     MuTables mu_tab{};                 // host copy; n = 0: power laws
     DevBuf<MuTables> mu_tab_d;
+    MuTablesF mu_tabf{};               // their FP32 form for the current E0
+    DevBuf<MuTablesF> mu_tabf_d;
     DevBuf<ScatterPt<double>> pts_grid;  // grid scatterers (one chunk)
     DevBuf<float> w1;                    // their first-leg weights
     DevBuf<int> rows;                    // detector rows of a row-subset call
@@ -366,43 +376,54 @@ hipError_t pe_cell_table(smg_ctx* c, PeLaunch& A) {
     return hipSuccess;
 }
 
-smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
-                         smg_precision prec, const float* pe, int pe_is_device, float* dst,
-                         int dst_is_device, smg_stats* stats) {
-    double t_start = now_ms();
-    smg_status s = check_ready(c, rotation);
-    if (s != SMG_OK) return s;
-    if (!dst) return fail(c, SMG_ERR_BAD_PARAM, "dst is NULL");
-    if (!collimator && !pe) return fail(c, SMG_ERR_BAD_PARAM, "PE block is NULL");
+// The mu(E) tables for photopeak e0 on the device: the tables with their
+// values at e0, and their FP32 form.
+hipError_t upload_mu_tables(smg_ctx* c, double e0) {
+    if (c->mu_tab.n == 0) return hipSuccess;
+    const double le0 = std::log(e0);
+    for (int t = 0; t < 4; ++t) c->mu_tab.lmu0[t] = mu_table_log(c->mu_tab, t, le0);
+    make_mu_tables_f(c->mu_tab, e0, c->mu_tabf);
+    hipError_t e = c->mu_tab_d.reserve(1);
+    if (e == hipSuccess) e = c->mu_tabf_d.reserve(1);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->mu_tab_d.ptr, &c->mu_tab, sizeof(MuTables), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->mu_tabf_d.ptr, &c->mu_tabf, sizeof(MuTablesF), hipMemcpyHostToDevice,
+                           c->stream);
+    return e;
+}
+
+// Per-call set-up of the bar-centre and hole-centre scatter launches, before
+// their timed region: the KN prefix table of the photopeak energy (built when
+// it changes) and the mu(E) tables with their values at E0.
+smg_status scatter_prep(smg_ctx* c) {
+    ScatterLaunch A{};
+    A.e0 = c->hg.phys[7];
+    SMG_HIP(c, kn_table(c, A));
+    SMG_HIP(c, upload_mu_tables(c, A.e0));
+    return SMG_OK;
+}
+
+// Enqueues one scatter matrix (crystal scatter driven by the device PE block
+// pe, or collimator scatter of `variant`) into the device block out, added to
+// it when accumulate; no synchronisation.  Adds its terms to *rays.
+smg_status scatter_launch(smg_ctx* c, bool collimator, int rotation, int variant, smg_precision prec,
+                          const float* pe, float* out, bool accumulate, uint64_t* rays) {
     const HostGeom& hg = c->hg;
     const GeomDesc& g = hg.g;
-    const size_t n_el = (size_t)g.n_det * (size_t)g.n_vox;
+    smg_status s;
     if ((s = ensure_bitmap(c, collimator)) != SMG_OK) return s;
-    ScatterLaunch A;
+    ScatterLaunch A{};
     A.geom = device_geom(c);
     A.kind = collimator ? 1 : 0;
     A.variant = variant;
     A.pts_d = collimator ? c->pts_hole.ptr : c->pts_bar.ptr;
     A.n_pts = collimator ? hg.n_holes_l0 : g.n_det;
     A.bitmap = collimator ? c->bm_coll.ptr : c->bm_crystal.ptr;
-    A.pe = nullptr;
-    if (!collimator) {
-        if (pe_is_device) {
-            A.pe = pe;
-        } else {
-            SMG_HIP(c, c->in.reserve(n_el));
-            SMG_HIP(c, hipMemcpyAsync(c->in.ptr, pe, n_el * sizeof(float), hipMemcpyHostToDevice, c->stream));
-            A.pe = c->in.ptr;
-        }
-    }
+    A.pe = collimator ? nullptr : pe;
     const int ncell = c->sc_sub[0] * c->sc_sub[1] * c->sc_sub[2];
     SMG_HIP(c, c->pair.reserve((size_t)g.n_det * (size_t)(A.n_pts > 0 ? A.n_pts : 1) * ncell));
     A.pair = c->pair.ptr;
-    float* out = dst;
-    if (!dst_is_device) {
-        SMG_HIP(c, c->out.reserve(n_el));
-        out = c->out.ptr;
-    }
     A.dst = out;
     A.counters = c->counters.ptr;
     double phi = (double)rotation * g.dphi;
@@ -416,29 +437,64 @@ smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
     A.win_hi = ph[6];
     A.e0 = ph[7];
     A.kn_norm = kn_norm(A.e0);
-    SMG_HIP(c, kn_table(c, A));
+    A.kn_table = c->kn_tab.ptr;  // scatter_prep
     A.coll_ratio = (double)hg.coll[17] / (double)hg.coll[15];
     if (collimator && !(hg.coll[15] != 0.0f)) A.coll_ratio = 0.0;
-    A.tables = nullptr;
-    if (c->mu_tab.n > 0) {
-        const double le0 = std::log(A.e0);
-        for (int t = 0; t < 4; ++t) c->mu_tab.lmu0[t] = mu_table_log(c->mu_tab, t, le0);
-        SMG_HIP(c, c->mu_tab_d.reserve(1));
-        SMG_HIP(c, hipMemcpyAsync(c->mu_tab_d.ptr, &c->mu_tab, sizeof(MuTables), hipMemcpyHostToDevice,
-                                  c->stream));
-        A.tables = c->mu_tab_d.ptr;
-    }
+    A.tables = c->mu_tab.n > 0 ? c->mu_tab_d.ptr : nullptr;
+    A.tables_f = c->mu_tab.n > 0 ? c->mu_tabf_d.ptr : nullptr;
     A.w1 = nullptr;
-    A.accumulate = 0;
+    A.accumulate = accumulate ? 1 : 0;
     A.rows = nullptr;
     A.n_rows = 0;
     A.pts_per_bar = 1;
     A.pt_bar0 = 0;
-    SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
-    SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
     SMG_HIP(c, launch_pair_terms(A, c->stream));
     SMG_HIP(c, scatter_partials(c, A));
     SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));
+    *rays += (uint64_t)g.n_det * (uint64_t)g.n_vox * (uint64_t)(A.n_pts > 0 ? A.n_pts : 0) * (uint64_t)ncell;
+    return SMG_OK;
+}
+
+// The PE block of a scatter call on the device (uploaded into buf when it is
+// on the host).
+smg_status device_block(smg_ctx* c, const float* blk, int is_device, DevBuf<float>& buf,
+                        const float** out) {
+    if (is_device) {
+        *out = blk;
+        return SMG_OK;
+    }
+    const size_t n_el = (size_t)c->hg.g.n_det * (size_t)c->hg.g.n_vox;
+    SMG_HIP(c, buf.reserve(n_el));
+    SMG_HIP(c, hipMemcpyAsync(buf.ptr, blk, n_el * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    *out = buf.ptr;
+    return SMG_OK;
+}
+
+smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
+                         smg_precision prec, const float* pe, int pe_is_device, float* dst,
+                         int dst_is_device, smg_stats* stats) {
+    double t_start = now_ms();
+    smg_status s = check_ready(c, rotation);
+    if (s != SMG_OK) return s;
+    if (!dst) return fail(c, SMG_ERR_BAD_PARAM, "dst is NULL");
+    if (!collimator && !pe) return fail(c, SMG_ERR_BAD_PARAM, "PE block is NULL");
+    const GeomDesc& g = c->hg.g;
+    const size_t n_el = (size_t)g.n_det * (size_t)g.n_vox;
+    const float* pe_d = nullptr;
+    if (!collimator && (s = device_block(c, pe, pe_is_device, c->in, &pe_d)) != SMG_OK) return s;
+    float* out = dst;
+    if (!dst_is_device) {
+        SMG_HIP(c, c->out.reserve(n_el));
+        out = c->out.ptr;
+    }
+    // the bitmap and the tables are built before the timed region
+    if ((s = ensure_bitmap(c, collimator)) != SMG_OK) return s;
+    if ((s = scatter_prep(c)) != SMG_OK) return s;
+    SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    uint64_t rays = 0;
+    if ((s = scatter_launch(c, collimator, rotation, variant, prec, pe_d, out, false, &rays)) != SMG_OK)
+        return s;
     SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long cnt[4] = {0, 0, 0, 0};
     SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
@@ -451,7 +507,7 @@ smg_status scatter_entry(smg_ctx* c, bool collimator, int rotation, int variant,
         stats->kernel_ms = ms;
         stats->total_ms = now_ms() - t_start;
         stats->elements = n_el;
-        stats->rays = n_el * (uint64_t)(A.n_pts > 0 ? A.n_pts : 0) * (uint64_t)ncell;
+        stats->rays = rays;
         stats->fp64_fallbacks = cnt[0];
         stats->overflow_waves = 0;
     }
@@ -492,8 +548,11 @@ smg_status smg_create(int device, smg_ctx** out) {
     if (device < 0 || device >= n) return SMG_ERR_NO_DEVICE;
     smg_ctx* c = new smg_ctx();
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+    bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreate(&c->stream) == hipSuccess &&
+              hipStreamCreate(&c->copy_stream) == hipSuccess;
+    for (int i = 0; ok && i < kCopyChunks; ++i)
+        ok = hipEventCreateWithFlags(&c->ev_rows[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok || hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         c->counters.reserve(4) != hipSuccess || scratch_address(device, &c->scratch_dev) != hipSuccess) {
         smg_destroy(c);
         return SMG_ERR_HIP;
@@ -534,14 +593,24 @@ smg_status smg_destroy(smg_ctx* c) {
     c->cell_tab.release();
     c->cell_tab_pts.release();
     c->mu_tab_d.release();
+    c->mu_tabf_d.release();
     c->pts_grid.release();
     c->w1.release();
     c->rows.release();
+    c->out2.release();
+    c->in2.release();
+    c->tan_v.release();
     c->spill_queue.release();
     c->spill_slots.release();
     c->use_rows.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t& e : c->ev_rows)
+        if (e) (void)hipEventDestroy(e);
+    if (c->copy_stream) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamDestroy(c->copy_stream);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SMG_OK;
@@ -715,13 +784,36 @@ smg_status smg_pe_forward(smg_ctx* c, int rotation, smg_precision prec, float* d
     SMG_HIP(c, pe_cell_table(c, A));
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
-    SMG_HIP(c, launch_pe_forward(A, pe_exact(c, prec), c->stream));
-    SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    const bool exact = pe_exact(c, prec);
+    if (dst_is_device) {
+        SMG_HIP(c, launch_pe_forward(A, exact, c->stream));
+        SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    } else {
+        // host output: rows in chunks of at least 64 MiB, each copied on the
+        // copy stream once computed, so the copies overlap the following
+        // chunks' kernels (a small block stays one launch)
+        const size_t bytes = n_el * sizeof(float);
+        const int n_chunks = (int)std::max<size_t>(
+            1, std::min<size_t>({(size_t)kCopyChunks, (size_t)g.n_det, bytes >> 26}));
+        int row0[kCopyChunks + 1];
+        for (int i = 0; i <= n_chunks; ++i) row0[i] = (int)((int64_t)g.n_det * i / n_chunks);
+        for (int i = 0; i < n_chunks; ++i) {
+            SMG_HIP(c, launch_pe_forward_rows(A, exact, row0[i], row0[i + 1] - row0[i], c->stream));
+            SMG_HIP(c, hipEventRecord(c->ev_rows[i], c->stream));
+        }
+        SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+        for (int i = 0; i < n_chunks; ++i) {
+            const size_t off = (size_t)row0[i] * (size_t)g.n_vox;
+            const size_t n = (size_t)(row0[i + 1] - row0[i]) * (size_t)g.n_vox;
+            SMG_HIP(c, hipStreamWaitEvent(c->copy_stream, c->ev_rows[i], 0));
+            SMG_HIP(c, hipMemcpyAsync(dst + off, out + off, n * sizeof(float), hipMemcpyDeviceToHost,
+                                      c->copy_stream));
+        }
+    }
     unsigned long long cnt[4] = {0, 0, 0, 0};
     SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
-    if (!dst_is_device)
-        SMG_HIP(c, hipMemcpyAsync(dst, out, n_el * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     SMG_HIP(c, hipStreamSynchronize(c->stream));
+    if (!dst_is_device) SMG_HIP(c, hipStreamSynchronize(c->copy_stream));
     float ms = 0.0f;
     SMG_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (stats) {
@@ -905,6 +997,81 @@ smg_status smg_scatter_collimator(smg_ctx* c, int rotation, smg_coll_variant var
     return scatter_entry(c, true, rotation, (int)variant, prec, nullptr, 0, dst, dst_is_device, stats);
 }
 
+smg_status smg_scatter_sum(smg_ctx* c, int rotation, unsigned parts, smg_precision prec,
+                           const float* pe_drive, const float* pe_add, int pe_is_device,
+                           float* scatter_dst, float* sum_dst, int dst_is_device, smg_stats* stats) {
+    double t_start = now_ms();
+    smg_status s = check_ready(c, rotation);
+    if (s != SMG_OK) return s;
+    const unsigned known = SMG_PART_CRYSTAL | SMG_PART_COLL_COMBINED | SMG_PART_COLL_STANDALONE;
+    if (parts == 0 || (parts & ~known))
+        return fail(c, SMG_ERR_BAD_PARAM, "scatter parts 0x%x (crystal 1, collimator 2 or 4)", parts);
+    if ((parts & SMG_PART_COLL_COMBINED) && (parts & SMG_PART_COLL_STANDALONE))
+        return fail(c, SMG_ERR_BAD_PARAM, "one collimator variant per call");
+    if (!scatter_dst && !sum_dst) return fail(c, SMG_ERR_BAD_PARAM, "no output");
+    if ((parts & SMG_PART_CRYSTAL) && !pe_drive)
+        return fail(c, SMG_ERR_BAD_PARAM, "crystal scatter needs the PE block that drives it");
+    if (sum_dst && !pe_add) return fail(c, SMG_ERR_BAD_PARAM, "the sum needs the PE block to add");
+    const GeomDesc& g = c->hg.g;
+    const size_t n_el = (size_t)g.n_det * (size_t)g.n_vox;
+    const float *drive_d = nullptr, *add_d = nullptr;
+    if ((parts & SMG_PART_CRYSTAL) && (s = device_block(c, pe_drive, pe_is_device, c->in, &drive_d)) != SMG_OK)
+        return s;
+    if (sum_dst) {
+        if (pe_add == pe_drive && drive_d)
+            add_d = drive_d;
+        else if ((s = device_block(c, pe_add, pe_is_device, c->in2, &add_d)) != SMG_OK)
+            return s;
+    }
+    float* sc = scatter_dst;
+    if (!dst_is_device || !scatter_dst) {
+        SMG_HIP(c, c->out.reserve(n_el));
+        sc = c->out.ptr;
+    }
+    float* sm = sum_dst;
+    if (sum_dst && !dst_is_device) {
+        SMG_HIP(c, c->out2.reserve(n_el));
+        sm = c->out2.ptr;
+    }
+    if ((parts & SMG_PART_CRYSTAL) && (s = ensure_bitmap(c, false)) != SMG_OK) return s;
+    if ((parts & ~SMG_PART_CRYSTAL) && (s = ensure_bitmap(c, true)) != SMG_OK) return s;
+    if ((s = scatter_prep(c)) != SMG_OK) return s;
+    SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    uint64_t rays = 0;
+    bool acc = false;
+    if (parts & SMG_PART_CRYSTAL) {
+        if ((s = scatter_launch(c, false, rotation, 0, prec, drive_d, sc, acc, &rays)) != SMG_OK) return s;
+        acc = true;
+    }
+    if (parts & (SMG_PART_COLL_COMBINED | SMG_PART_COLL_STANDALONE)) {
+        const int variant = (parts & SMG_PART_COLL_STANDALONE) ? 1 : 0;
+        if ((s = scatter_launch(c, true, rotation, variant, prec, nullptr, sc, acc, &rays)) != SMG_OK) return s;
+    }
+    if (sum_dst) SMG_HIP(c, launch_add_blocks(add_d, sc, sm, n_el, c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    if (!dst_is_device) {
+        if (scatter_dst)
+            SMG_HIP(c, hipMemcpyAsync(scatter_dst, sc, n_el * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        if (sum_dst)
+            SMG_HIP(c, hipMemcpyAsync(sum_dst, sm, n_el * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    SMG_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t_start;
+        stats->elements = n_el;
+        stats->rays = rays;
+        stats->fp64_fallbacks = cnt[0];
+        stats->overflow_waves = 0;
+    }
+    return SMG_OK;
+}
+
 smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm, unsigned layer_mask,
                                        smg_precision prec, float* dst, int dst_is_device,
                                        int64_t* n_points, smg_stats* stats) {
@@ -965,13 +1132,11 @@ smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm
     SMG_HIP(c, kn_table(c, A));
     A.coll_ratio = 0.0;
     A.tables = nullptr;
+    A.tables_f = nullptr;
     if (c->mu_tab.n > 0) {
-        const double le0 = std::log(A.e0);
-        for (int t = 0; t < 4; ++t) c->mu_tab.lmu0[t] = mu_table_log(c->mu_tab, t, le0);
-        SMG_HIP(c, c->mu_tab_d.reserve(1));
-        SMG_HIP(c, hipMemcpyAsync(c->mu_tab_d.ptr, &c->mu_tab, sizeof(MuTables), hipMemcpyHostToDevice,
-                                  c->stream));
+        SMG_HIP(c, upload_mu_tables(c, A.e0));
         A.tables = c->mu_tab_d.ptr;
+        A.tables_f = c->mu_tabf_d.ptr;
     }
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
@@ -1082,13 +1247,11 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
     A.kn_norm = kn_norm(A.e0);
     SMG_HIP(c, kn_table(c, A));
     A.tables = nullptr;
+    A.tables_f = nullptr;
     if (c->mu_tab.n > 0) {
-        const double le0 = std::log(A.e0);
-        for (int t = 0; t < 4; ++t) c->mu_tab.lmu0[t] = mu_table_log(c->mu_tab, t, le0);
-        SMG_HIP(c, c->mu_tab_d.reserve(1));
-        SMG_HIP(c, hipMemcpyAsync(c->mu_tab_d.ptr, &c->mu_tab, sizeof(MuTables), hipMemcpyHostToDevice,
-                                  c->stream));
+        SMG_HIP(c, upload_mu_tables(c, A.e0));
         A.tables = c->mu_tab_d.ptr;
+        A.tables_f = c->mu_tabf_d.ptr;
     }
     A.rows = rows ? c->rows.ptr : nullptr;
     A.n_rows = n_rows;
@@ -1164,6 +1327,62 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
         stats->total_ms = now_ms() - t_start;
         stats->elements = n_el;
         stats->rays = n_el * (uint64_t)(g.n_det - 1) * (uint64_t)npb * (uint64_t)ncell;
+        stats->fp64_fallbacks = cnt[0];
+        stats->overflow_waves = cnt[1];
+    }
+    return SMG_OK;
+}
+
+smg_status smg_pe_tangent(smg_ctx* c, int rotation, smg_precision prec, const double* dmu, float* dA,
+                          float* A_out, int dst_is_device, smg_stats* stats) {
+    double t_start = now_ms();
+    smg_status s = check_ready(c, rotation);
+    if (s != SMG_OK) return s;
+    if (!dmu || !dA) return fail(c, SMG_ERR_BAD_PARAM, "dmu and dA must not be NULL");
+    const GeomDesc& g = c->hg.g;
+    const size_t n_el = (size_t)g.n_det * (size_t)g.n_vox;
+    const int n_par = g.n_layers + 2 * g.n_det;
+    PeLaunch A{};
+    A.geom = device_geom(c);
+    A.dst = nullptr;
+    A.counters = c->counters.ptr;
+    const double phi = (double)rotation * g.dphi;
+    A.cphi = std::cos(phi);
+    A.sphi = std::sin(phi);
+    for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
+    pick_tile(g, A.tile, A.tiles);
+    if ((s = ensure_spill(c, A)) != SMG_OK) return s;
+    SMG_HIP(c, pe_cell_table(c, A));
+    SMG_HIP(c, c->tan_v.reserve((size_t)n_par));
+    SMG_HIP(c, hipMemcpyAsync(c->tan_v.ptr, dmu, n_par * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PeTangent TG{c->tan_v.ptr, dA, A_out};
+    if (!dst_is_device) {
+        SMG_HIP(c, c->out.reserve(n_el));
+        TG.d_out = c->out.ptr;
+        if (A_out) {
+            SMG_HIP(c, c->out2.reserve(n_el));
+            TG.a_out = c->out2.ptr;
+        }
+    }
+    SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
+    SMG_HIP(c, launch_pe_tangent(A, TG, pe_exact(c, prec), c->stream));
+    SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    if (!dst_is_device) {
+        SMG_HIP(c, hipMemcpyAsync(dA, TG.d_out, n_el * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        if (A_out)
+            SMG_HIP(c, hipMemcpyAsync(A_out, TG.a_out, n_el * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+    SMG_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    SMG_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t_start;
+        stats->elements = n_el;
+        stats->rays = n_el * (uint64_t)(c->pe_sub[0] * c->pe_sub[1] * c->pe_sub[2]);
         stats->fp64_fallbacks = cnt[0];
         stats->overflow_waves = cnt[1];
     }

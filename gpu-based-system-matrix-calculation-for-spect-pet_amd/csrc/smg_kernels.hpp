@@ -36,10 +36,10 @@ struct PeSpill {
 };
 
 // Bytes of one spill slot: masks, adjoint sums and indices of every bar and
-// hole, in 256-byte units.
+// hole, and the face records of the first narrowed bars, in 256-byte units.
 inline int64_t pe_spill_slot_bytes(int n_det, int n_holes) {
     const int64_t b = n_det > 0 ? n_det : 1, h = n_holes > 0 ? n_holes : 1;
-    const int64_t bytes = b * (8 + 8 + 2 + 2) + h * (8 + 2 + 2);
+    const int64_t bytes = b * (8 + 8 + 2 + 2) + h * (8 + 2 + 2) + 32 * (6 * 4 + 1);
     return (bytes + 255) / 256 * 256;
 }
 
@@ -80,7 +80,19 @@ hipError_t launch_pe_backproject(const PeLaunch& a, bool exact, const float* y, 
 hipError_t launch_cell_table(const PeGeom& geom, int bar0, int n_bars, const int sub[3], CellCtx* out,
                              hipStream_t stream);
 
+// Tangent (forward-mode derivative) of the PE matrix along a direction v over
+// the adjoint's parameters ([n_layers layer mu][N bar mu_total][N bar mu_pe]):
+// dA[d][v] = sum_k v_k dA[d][v]/dmu_k, and A itself when a_out is set.
+struct PeTangent {
+    const double* v;   // direction (device)
+    float* d_out;      // [N][V]
+    float* a_out;      // [N][V] or nullptr
+};
+hipError_t launch_pe_tangent(const PeLaunch& a, const PeTangent& t, bool exact, hipStream_t stream);
+
 hipError_t launch_pe_forward(const PeLaunch& a, bool exact, hipStream_t stream);
+// The rows (bars) bar0 .. bar0 + n_bars - 1 of the same matrix.
+hipError_t launch_pe_forward_rows(const PeLaunch& a, bool exact, int bar0, int n_bars, hipStream_t stream);
 // Crystal grid scatterers: first-leg weights of the sub-cell centres of bars
 // bar0 .. bar0 + n_bars - 1 (sub-cells = grid cells) into a.pts.
 hipError_t launch_pe_points(const PeLaunch& a, bool exact, int bar0, int n_bars, hipStream_t stream);
@@ -130,6 +142,7 @@ struct ScatterLaunch {
     double coll_ratio;               // mu_compton / mu_total of collimator layer 0
     int same_window, compton_on;
     const MuTables* tables;          // mu(E) tables, or nullptr for the power laws
+    const MuTablesF* tables_f;       // their FP32 form (when tables is set)
     const float* w1;                 // grid: first-leg weights [n_pts][V] of this chunk
     int accumulate;                  // add to dst instead of writing it
     const int* rows;                 // detector rows to compute (dst row r = bar rows[r]),
@@ -148,6 +161,8 @@ hipError_t launch_kn_table(double e0, double* tab, hipStream_t stream);
 hipError_t launch_first_leg(const ScatterLaunch& a, float* w1, hipStream_t stream);
 
 hipError_t launch_pair_terms(const ScatterLaunch& a, hipStream_t stream);
+// out[i] = a[i] + b[i], i < n (the PE + scatter block).
+hipError_t launch_add_blocks(const float* a, const float* b, float* out, size_t n, hipStream_t stream);
 hipError_t launch_scatter(const ScatterLaunch& a, bool exact, hipStream_t stream);
 // Doubles launch_scatter needs in `partial` for this launch (0: one wave per
 // element covers every scatter point, no partial sums).

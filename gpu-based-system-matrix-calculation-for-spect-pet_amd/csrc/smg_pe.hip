@@ -33,6 +33,10 @@ namespace {
 // Capacity of the per-wave queue of sub-cells to redo in FP64.
 constexpr int kRedoCap = 64;
 
+// Narrowed bars of a sub-cell whose faces relative to the sub-cell point are
+// kept for fast_bar_chord (the first kFastBars).
+constexpr int kFastBars = 32;
+
 // Candidate lists of one wave, in LDS.
 struct WaveLists {
     static constexpr int cap_bars = kCapBars, cap_holes = kCapHoles;
@@ -42,6 +46,8 @@ struct WaveLists {
     uint16_t hole_idx[kCapHoles];
     uint16_t bar_sel[kCapBars];
     uint16_t hole_sel[kCapHoles];
+    float faces[kFastBars][6];   // narrowed bar j: faces - sub-cell point (fast_faces)
+    uint8_t fast_ok[kFastBars];  // narrowed bar j qualifies for fast_bar_chord
 };
 
 // Candidate lists of a wave whose tile overflowed the LDS lists, in its own
@@ -52,10 +58,12 @@ struct SpillLists {
     uint64_t* bar_mask;
     uint64_t* hole_mask;
     double* cand_acc;
+    float (*faces)[6];
     uint16_t* bar_idx;
     uint16_t* hole_idx;
     uint16_t* bar_sel;
     uint16_t* hole_sel;
+    uint8_t* fast_ok;
 };
 
 __device__ __forceinline__ SpillLists spill_lists(const PeSpill& S, int slot) {
@@ -69,6 +77,8 @@ __device__ __forceinline__ SpillLists spill_lists(const PeSpill& S, int slot) {
     p += sizeof(uint64_t) * S.cap_holes;
     L.cand_acc = (double*)p;
     p += sizeof(double) * S.cap_bars;
+    L.faces = (float(*)[6])p;
+    p += sizeof(float) * 6 * kFastBars;
     L.bar_idx = (uint16_t*)p;
     p += sizeof(uint16_t) * S.cap_bars;
     L.hole_idx = (uint16_t*)p;
@@ -76,6 +86,8 @@ __device__ __forceinline__ SpillLists spill_lists(const PeSpill& S, int slot) {
     L.bar_sel = (uint16_t*)p;
     p += sizeof(uint16_t) * S.cap_bars;
     L.hole_sel = (uint16_t*)p;
+    p += sizeof(uint16_t) * S.cap_holes;
+    L.fast_ok = (uint8_t*)p;
     return L;
 }
 
@@ -92,8 +104,11 @@ __device__ __forceinline__ void spill_push(const PeLaunch& A, int lane) {
 // View of the narrowed lists for one lane.
 template <class Lists>
 struct ListCands {
+    static constexpr bool kFastFaces = true;
     const Lists* L;
     int nb, nh, lane;
+    __device__ bool fast(int j) const { return j < kFastBars && L->fast_ok[j]; }
+    __device__ const float* faces(int j) const { return L->faces[j]; }
     __device__ int nbars() const { return nb; }
     // the index is wave-uniform: readfirstlane lets the record loads go
     // through the scalar unit
@@ -299,10 +314,18 @@ __device__ void narrow_candidates(const PeLaunch& A, const double qd[3], const W
         }
         const uint64_t bb = __ballot(pass && is_bar), bh = __ballot(pass && !is_bar);
         if (pass) {
-            if (is_bar)
-                L.bar_sel[nsb + lanes_below(bb, lane)] = (uint16_t)j;
-            else
+            if (is_bar) {
+                const int pos = nsb + lanes_below(bb, lane);
+                L.bar_sel[pos] = (uint16_t)j;
+                if (pos < kFastBars) {
+                    float f[6];
+                    L.fast_ok[pos] = fast_faces(G, L.bar_idx[j], qd, W.tc, W.rho, f) ? 1 : 0;
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) L.faces[pos][i] = f[i];
+                }
+            } else {
                 L.hole_sel[nsh + lanes_below(bh, lane)] = (uint16_t)(j - nb);
+            }
         }
         nsb += __popcll(bb);
         nsh += __popcll(bh);
@@ -506,6 +529,143 @@ __global__ __launch_bounds__(64) void pe_forward_spill_kernel(PeLaunch A) {
     for (unsigned long long i = blockIdx.x; i < n; i += gridDim.x) {
         __syncthreads();  // the previous item's list reads are done
         pe_forward_body<kExact, kPoints>(A, A.spill.queue[i], L);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&A.counters[1], n);
+}
+
+// ---- tangent --------------------------------------------------------------------------
+//
+// d term of one sub-cell ray along the direction v (no counterpart in the
+// reference; the derivatives of its element formula, PESysMatGen.cu:1240-1357,
+// as in the adjoint): term = Omega e^-att abs,
+//   d att = sum_m v_m rest_m + sum_k v_t[k] l_k + v_t[d] (l1 - l2),
+//   d abs = v_t[d] mu_pe (l_a e^(-mu l_a)/mu - (1 - e^(-mu l_a))/mu^2) + v_pe[d] (1 - e^(-mu l_a))/mu,
+//   d term = -term d att + Omega e^-att d abs.
+// dlay, aux and tan_att as pe_cell_ray returns them.
+__device__ __forceinline__ double tangent_term(const PeGeom& G, int d, const double* v, double term,
+                                               const double* dlay, const double* aux, double tan_att) {
+    const GeomDesc& g = G.g;
+    const Bar<double>& bd = G.bars_d[d];
+    const double vt = v[g.n_layers + d], vpe = v[g.n_layers + g.n_det + d];
+    double datt = tan_att + vt * aux[1];
+#pragma unroll
+    for (int m = 0; m < kMaxLayers; ++m)
+        if (m < g.n_layers) datt += v[m] * dlay[m];
+    const double mut = bd.mut, la = aux[2], ea = aux[3];
+    const double dabs = vt * bd.mupe * (la * ea / mut - (1.0 - ea) / (mut * mut)) + vpe * (1.0 - ea) / mut;
+    return -term * datt + aux[0] * dabs;
+}
+
+// Work item `item` (bar, tile) of the tangent: as pe_forward_body, with the
+// derivative summed beside the element.  Returns false, having written
+// nothing, when the candidates overflow L.
+template <bool kExact, class Lists>
+__device__ __forceinline__ bool pe_tangent_body(const PeLaunch& A, const PeTangent& TG, int item, Lists& L) {
+    const PeGeom& G = A.geom;
+    const GeomDesc& g = G.g;
+    const int lane = threadIdx.x;
+    const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    const int d = item / n_tiles;
+    const int tile = item % n_tiles;
+    const int n_cells = A.sub[0] * A.sub[1] * A.sub[2];
+    const Bar<double>& bd = G.bars_d[d];
+    WaveSetup W;
+    wave_setup(A, d, tile, lane, W);
+    const int64_t o = (int64_t)d * g.n_vox + W.v;
+    if (!(bd.mut > (double)0.01f)) {  // PESysMatGen.cu:1116, :1194
+        if (W.valid) {
+            TG.d_out[o] = 0.0f;
+            if (TG.a_out) TG.a_out[o] = 0.0f;
+        }
+        return true;
+    }
+    int nb, nh;
+    if (!build_candidates<kExact>(A, d, lane, W, L, nb, nh)) return false;
+    __syncthreads();
+    using T = typename std::conditional<kExact, double, float>::type;
+    KahanSum<T> acc;
+    KahanSum<double> dacc;
+    unsigned fallbacks = 0;
+    const double* vbar = TG.v + g.n_layers;
+    __shared__ uint16_t redo_cell[kRedoCap];
+    __shared__ uint64_t redo_mask[kRedoCap];
+    int cell = 0;
+    while (cell < n_cells) {
+        int n_redo = 0;
+        for (; cell < n_cells && n_redo < kRedoCap; ++cell) {
+            CellCtx cc;
+            tile_cell(A, bd, d, cell, W, lane, cc);
+            int nsb, nsh;
+            __syncthreads();
+            narrow_candidates(A, cc.q, W, lane, L, nb, nh, nsb, nsh);
+            __syncthreads();
+            bool near = false;
+            if (W.valid) {
+                ListCands<Lists> C{&L, nsb, nsh, lane};
+                double dlay[kMaxLayers], aux[8], tatt = 0.0;
+                T val = pe_cell_ray<T, kExact>(G, d, W.p, W.ps, cc, C, near, dlay, aux, nullptr, 0, vbar, &tatt);
+                if (!near) {
+                    acc.add(val);
+                    dacc.add(tangent_term(G, d, TG.v, (double)val, dlay, aux, tatt));
+                }
+            }
+            if constexpr (!kExact) {
+                const uint64_t m = __ballot(near);
+                if (m) {
+                    if (lane == 0) {
+                        redo_cell[n_redo] = (uint16_t)cell;
+                        redo_mask[n_redo] = m;
+                    }
+                    ++n_redo;
+                }
+            }
+        }
+        if constexpr (!kExact) {
+            for (int i = 0; i < n_redo; ++i) {
+                CellCtx cc;
+                tile_cell(A, bd, d, redo_cell[i], W, lane, cc);
+                int rsb, rsh;
+                __syncthreads();
+                narrow_candidates(A, cc.q, W, lane, L, nb, nh, rsb, rsh);
+                __syncthreads();
+                if ((redo_mask[i] >> lane) & 1ull) {
+                    ListCands<Lists> C{&L, rsb, rsh, lane};
+                    bool n2 = false;
+                    double dlay[kMaxLayers], aux[8], tatt = 0.0;
+                    const double val = pe_cell_ray<double, kExact>(G, d, W.p, W.ps, cc, C, n2, dlay, aux,
+                                                                   nullptr, 0, vbar, &tatt);
+                    acc.add((T)val);
+                    dacc.add(tangent_term(G, d, TG.v, val, dlay, aux, tatt));
+                    ++fallbacks;
+                }
+            }
+        }
+    }
+    if (W.valid) {
+        TG.d_out[o] = float(dacc.value());
+        if (TG.a_out) TG.a_out[o] = float(acc.value());
+    }
+    if constexpr (!kExact) {
+        unsigned tot = fallbacks;
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+        if (lane == 0 && tot) atomicAdd(&A.counters[0], (unsigned long long)tot);
+    }
+    return true;
+}
+
+template <bool kExact>
+__global__ __launch_bounds__(64) void pe_tangent_kernel(PeLaunch A, PeTangent TG) {
+    __shared__ WaveLists L;
+    if (!pe_tangent_body<kExact>(A, TG, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
+}
+
+template <bool kExact>
+__global__ __launch_bounds__(64) void pe_tangent_spill_kernel(PeLaunch A, PeTangent TG) {
+    SpillLists L = spill_lists(A.spill, (int)blockIdx.x);
+    const unsigned long long n = A.counters[2];
+    for (unsigned long long i = blockIdx.x; i < n; i += gridDim.x) {
+        __syncthreads();
+        pe_tangent_body<kExact>(A, TG, A.spill.queue[i], L);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&A.counters[1], n);
 }
@@ -970,6 +1130,23 @@ hipError_t spill_begin(const PeLaunch& A, int64_t blocks, hipStream_t stream, un
 
 }  // namespace
 
+hipError_t launch_pe_tangent(const PeLaunch& A, const PeTangent& TG, bool exact, hipStream_t stream) {
+    const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    const int64_t blocks = (int64_t)n_tiles * A.geom.g.n_det;
+    if (blocks <= 0) return hipSuccess;
+    unsigned sg = 0;
+    hipError_t e = spill_begin(A, blocks, stream, sg);
+    if (e != hipSuccess) return e;
+    if (exact) {
+        hipLaunchKernelGGL(pe_tangent_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A, TG);
+        if (sg) hipLaunchKernelGGL(pe_tangent_spill_kernel<true>, dim3(sg), dim3(64), 0, stream, A, TG);
+    } else {
+        hipLaunchKernelGGL(pe_tangent_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A, TG);
+        if (sg) hipLaunchKernelGGL(pe_tangent_spill_kernel<false>, dim3(sg), dim3(64), 0, stream, A, TG);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_pe_backward(const PeLaunch& A, const PeGradScratch& S, bool exact, double* grad,
                               hipStream_t stream) {
     const int64_t blocks = (int64_t)A.geom.g.n_det * S.groups;
@@ -1021,11 +1198,16 @@ hipError_t launch_cell_table(const PeGeom& geom, int bar0, int n_bars, const int
 }
 
 hipError_t launch_pe_forward(const PeLaunch& A, bool exact, hipStream_t stream) {
+    return launch_pe_forward_rows(A, exact, 0, A.geom.g.n_det, stream);
+}
+
+hipError_t launch_pe_forward_rows(const PeLaunch& A, bool exact, int bar0, int n_bars, hipStream_t stream) {
     const int n_tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
-    const int64_t blocks = (int64_t)n_tiles * A.geom.g.n_det;
+    if (bar0 < 0 || n_bars < 0 || bar0 + n_bars > A.geom.g.n_det) return hipErrorInvalidValue;
+    const int64_t blocks = (int64_t)n_tiles * n_bars;
     if (blocks <= 0) return hipSuccess;
     PeLaunch B = A;
-    B.bar0 = 0;
+    B.bar0 = bar0;
     B.cell_slices = 1;
     unsigned sg = 0;
     hipError_t e = spill_begin(B, blocks, stream, sg);

@@ -373,6 +373,9 @@ SMG_FN void make_cell_ctx_linear(const Bar<double>& bd, int cell, const int sub[
 // reference culling set.
 //   int nbars() const; int bar(int j) const; bool bar_on(int j) const;
 //   int nholes() const; int hole(int j) const; bool hole_on(int j) const;
+//   static constexpr bool kFastFaces;  with it, bool fast(int j) const and
+//   const float* faces(int j) const: bar j qualifies for fast_bar_chord and
+//   its faces relative to the sub-cell point (fast_faces).
 
 // Chord of the ray voxel p -> sub-cell point q through another bar k
 // (PESysMatGen.cu:1290-1325).  FP32: slab parameters relative to the point of
@@ -414,6 +417,64 @@ SMG_FN T other_bar_chord(const PeGeom& G, int k, const double p[3], const double
     }
 }
 
+// ---- FP32 chords through unturned bars measured from the sub-cell point ---------------
+//
+// For an unturned bar k and the sub-cell point q, the box faces relative to q
+// (formed once per sub-cell and candidate, in FP64, rounded once) give the
+// slab parameters of every lane's ray measured from q: t = -(face - q) len /
+// D_k (D = q - p).  Each is a product of two correctly rounded factors, so
+// its relative error is a few ulp whatever the ray's angle, and the chord
+// t_out - t_in of a bar near q (the detector bars are near q, the voxels far)
+// keeps ~1e-7 mm.  A bar qualifies when it is more than 0.01 mm from q and
+// from the tile ball: then the line of a lane's ray meets it (if at all) in
+// an interval entirely in front of q (t_in >= 0.01), entirely behind q
+// (t_out <= -0.01: the reference's entry and exit lie past the segment's end,
+// chord 0) or entirely beyond the voxel (t_in >= len + 0.01), so the
+// reference's tests of the entry and exit against the segment's ends
+// (PESysMatGen.cu:297-304) are decided by the sign of t_in and of
+// len - t_in with a margin of 0.01, and the chord below eps is the only fine
+// decision (PESysMatGen.cu:291), taken with the tolerance of the two
+// parameters.
+
+// Faces of unturned bar k relative to q: out = (lo - q, hi - q) per axis;
+// true when the bar qualifies (unturned, more than 0.01 mm from q and from
+// the ball (tc, rho)).
+SMG_FN bool fast_faces(const PeGeom& G, int k, const double q[3], const double tc[3], double rho,
+                       float out[6]) {
+    const Bar<double>& b = G.bars_d[k];
+    const double c[3] = {b.cx, b.cy, b.cz}, h[3] = {0.5 * b.w, 0.5 * b.t, 0.5 * b.h};
+    double dq = 0.0, dt = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double lo = c[i] - h[i], hi = c[i] + h[i];
+        out[i] = float(lo - q[i]);
+        out[3 + i] = float(hi - q[i]);
+        const double eq = tmax2(tmax2(lo - q[i], q[i] - hi), 0.0);
+        const double et = tmax2(tmax2(lo - tc[i], tc[i] - hi), 0.0);
+        dq += eq * eq;
+        dt += et * et;
+    }
+    return b.sb == 0.0 && b.cb == 1.0 && dq > 1e-4 && sqrt(dt) > rho + 0.01;
+}
+
+// Chord of the ray through a qualifying bar from its faces relative to q
+// (length_box_ray, PESysMatGen.cu:236-312, for a segment with no direction
+// component below eps).
+SMG_FN float fast_bar_chord(const SegDir<float>& s, const float n[6], bool& near) {
+    float t_in = -INFINITY, t_out = INFINITY, e = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float a = -n[k] * s.inv[k], b = -n[3 + k] * s.inv[k];
+        t_in = tmax2(t_in, tmin2(a, b));
+        t_out = tmin2(t_out, tmax2(a, b));
+    }
+    const float chord = t_out - t_in;
+    if (!(chord > 0.0f) || !(t_in > 0.0f) || !(t_in < s.len)) return 0.0f;
+    const float tol = 4.0f * kUnit<float> * (tabs(t_in) + tabs(t_out)) + 1e-9f;
+    if (g_lt(chord, kSlabEps<float>, tol, near)) return 0.0f;
+    return chord;
+}
+
 // World-frame direction data of the ray voxel -> sub-cell point.
 template <class T>
 SMG_FN void ray_dir(const double p[3], const double q[3], double D[3], double& dd, SegDir<T>& sw,
@@ -431,10 +492,14 @@ constexpr int kKeepChords = 8;
 // the library runs whenever a hole is elliptical); otherwise every hole is
 // traced as the parameter file's circle, which keeps the ellipse's code, and
 // its registers, out of the FP32 kernels.
+// tan_bar (optional): a direction over the bars' mu_total; *tan_att then
+// receives sum_k tan_bar[k] l_k over the other bars on the ray (the tangent of
+// their attenuation).
 template <class T, bool kEll, class Cands>
 SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[3],
                      const CellCtx& cc, const Cands& C, bool& near, double* dl_layer = nullptr,
-                     double* aux = nullptr, float* keep = nullptr, int keep_stride = 0) {
+                     double* aux = nullptr, float* keep = nullptr, int keep_stride = 0,
+                     const double* tan_bar = nullptr, double* tan_att = nullptr) {
     const GeomDesc& g = G.g;
     const Bar<double>& bd = G.bars_d[d];
     double D[3] = {cc.q[0] - p[0], cc.q[1] - p[1], cc.q[2] - p[2]};
@@ -540,9 +605,18 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
             continue;
         }
         int k = C.bar(j);
-        T l = other_bar_chord<T>(G, k, p, D, dd, inv_dd, cc.q, sw, tol_d, near);
+        T l;
+        if constexpr (Cands::kFastFaces && std::is_same<T, float>::value) {
+            if (sw.ns == 0 && C.fast(j))
+                l = fast_bar_chord(sw, C.faces(j), near);
+            else
+                l = other_bar_chord<T>(G, k, p, D, dd, inv_dd, cc.q, sw, tol_d, near);
+        } else {
+            l = other_bar_chord<T>(G, k, p, D, dd, inv_dd, cc.q, sw, tol_d, near);
+        }
         if (keep && j < kKeepChords) keep[j * keep_stride] = float(l);
         att += double(l) * G.bars_d[k].mut;
+        if (tan_bar) *tan_att += double(l) * tan_bar[k];
     }
 
     // own bar from its surface to the sub-cell (PESysMatGen.cu:1329-1346), and the

@@ -168,28 +168,45 @@ int PESysMatGen(float* parameter_Collimator, float* parameter_Detector, float* p
     return (int)nv;
 }
 
-// Combined generator: crystal scatter + collimator scatter (r = 0.1 mm), summed.
+// Extension behind the three scatter() overloads (declared in smg_reference.h):
+// the scatter block of `which` and, in the same call on the GPU, PE + scatter.
+int scatter_and_sum(int which, float* parameter_Collimator, float* parameter_Detector,
+                    float* parameter_Image, float* parameter_Physics, float* PE_SysMat, float* PE_add,
+                    const char* FnameGeoCrystal, const char* FnameGeoCollimator, float* scatter_dst,
+                    float* sum_dst, int cuda_id) {
+    if (which < 1 || which > 3 || (!scatter_dst && !sum_dst)) {
+        std::fprintf(stderr, "scatter_and_sum: which = %d, outputs %p %p\n", which, (void*)scatter_dst,
+                     (void*)sum_dst);
+        std::exit(EXIT_FAILURE);
+    }
+    smg_ctx* ctx = context_for(cuda_id, which == 1 ? nullptr : parameter_Collimator, parameter_Detector,
+                               parameter_Image, parameter_Physics);
+    int64_t nv = n_voxels(ctx);
+    unsigned parts = 0;
+    if (which & 1) {
+        bitmap_from_flag(ctx, false, (int)std::floor(parameter_Physics[8]), FnameGeoCrystal);
+        parts |= SMG_PART_CRYSTAL;
+    }
+    if (which & 2) {
+        bitmap_from_flag(ctx, true, (int)std::floor(parameter_Physics[9] + 0.001f), FnameGeoCollimator);
+        parts |= which == 3 ? SMG_PART_COLL_COMBINED : SMG_PART_COLL_STANDALONE;
+    }
+    smg_stats st;
+    check(ctx, smg_scatter_sum(ctx, rotation_of(parameter_Image), parts, precision(), PE_SysMat, PE_add, 0,
+                               scatter_dst, sum_dst, 0, &st),
+          "smg_scatter_sum");
+    std::printf("%s scatter%s: %.1f ms\n", which == 3 ? "crystal + collimator" : (which == 1 ? "crystal" : "collimator"),
+                sum_dst ? " and PE + scatter" : "", st.kernel_ms);
+    return (int)nv;
+}
+
+// Combined generator: crystal scatter + collimator scatter (r = 0.1 mm), summed
+// on the GPU.
 int scatter(float* parameter_Collimator, float* parameter_Detector, float* parameter_Image,
             float* parameter_Physics, float* PE_SysMat, const char* FnameGeoCrystal,
             const char* FnameGeoCollimator, float* dst, int cuda_id) {
-    smg_ctx* ctx = context_for(cuda_id, parameter_Collimator, parameter_Detector, parameter_Image,
-                               parameter_Physics);
-    int nd = 0;
-    int64_t nv = n_voxels(ctx, &nd);
-    const size_t n = (size_t)nd * (size_t)nv;
-    const int rot = rotation_of(parameter_Image);
-    bitmap_from_flag(ctx, false, (int)std::floor(parameter_Physics[8]), FnameGeoCrystal);
-    std::vector<float> crystal(n), coll(n);
-    smg_stats st;
-    check(ctx, smg_scatter_crystal(ctx, rot, precision(), PE_SysMat, 0, crystal.data(), 0, &st),
-          "smg_scatter_crystal");
-    std::printf("crystal scatter: %.1f ms\n", st.kernel_ms);
-    bitmap_from_flag(ctx, true, (int)std::floor(parameter_Physics[9] + 0.001f), FnameGeoCollimator);
-    check(ctx, smg_scatter_collimator(ctx, rot, SMG_COLL_COMBINED, precision(), coll.data(), 0, &st),
-          "smg_scatter_collimator");
-    std::printf("collimator scatter: %.1f ms\n", st.kernel_ms);
-    for (size_t i = 0; i < n; ++i) dst[i] = coll[i] + crystal[i];
-    return (int)nv;
+    return scatter_and_sum(3, parameter_Collimator, parameter_Detector, parameter_Image, parameter_Physics,
+                           PE_SysMat, nullptr, FnameGeoCrystal, FnameGeoCollimator, dst, nullptr, cuda_id);
 }
 
 // Standalone collimator scatter (r = 0.2 mm virtual disc).

@@ -269,6 +269,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void sc
     }
     ph.norm = A.kn_norm;
     ph.tab = A.tables;
+    ph.tabf = A.tables_f;
     ph.knt = A.kn_table;
     ph.res = bd.res;
     ph.mupe = bd.mupe;
@@ -445,6 +446,39 @@ hipError_t launch_scatter(const ScatterLaunch& A, bool exact, hipStream_t stream
     const int64_t n = sh.rows * A.geom.g.n_vox;
     hipLaunchKernelGGL(scatter_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
                        A, sh);
+    return hipGetLastError();
+}
+
+namespace {
+
+// out = a + b, four floats per thread where the three pointers allow it.
+__global__ __launch_bounds__(256) void add_blocks_kernel(const float* a, const float* b, float* out,
+                                                         size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = a[i] + b[i];
+}
+
+__global__ __launch_bounds__(256) void add_blocks4_kernel(const float4* a, const float4* b, float4* out,
+                                                          size_t n4) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        const float4 x = a[i], y = b[i];
+        out[i] = make_float4(x.x + y.x, x.y + y.y, x.z + y.z, x.w + y.w);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_add_blocks(const float* a, const float* b, float* out, size_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const bool al = ((uintptr_t)a | (uintptr_t)b | (uintptr_t)out) % 16 == 0 && n % 4 == 0;
+    const size_t work = al ? n / 4 : n;
+    const unsigned blocks = (unsigned)std::min<size_t>((work + 255) / 256, 8192);
+    if (al)
+        hipLaunchKernelGGL(add_blocks4_kernel, dim3(blocks), dim3(256), 0, stream, (const float4*)a,
+                           (const float4*)b, (float4*)out, work);
+    else
+        hipLaunchKernelGGL(add_blocks_kernel, dim3(blocks), dim3(256), 0, stream, a, b, out, n);
     return hipGetLastError();
 }
 

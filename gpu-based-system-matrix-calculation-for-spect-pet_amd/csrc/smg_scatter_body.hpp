@@ -16,6 +16,8 @@
 // (d, s, sub-cell) by the FP64 pair-terms kernel and read here.
 #pragma once
 
+#include <cmath>
+
 #include "smg_pe_body.hpp"
 
 namespace smg {
@@ -150,6 +152,65 @@ SMG_FN void mu_ratios(const MuTables* t, double e0, double ep, double& r3, doubl
     r1 = exp(mu_table_log(*t, 1, le) - t->lmu0[1]);
 }
 
+// The tables in the form the FP32 term evaluates them.  E' lies in
+// [E0 / (1 + 2 alpha), E0], so each process's ratio mu(E')/mu(E0) is a
+// piecewise power law in L = ln(E0/E') on the few table segments that range
+// meets: on segment j, ln(mu(E')/mu(E0)) = c[p][j] - s[p][j] L, with lb the
+// segment breakpoints in L (ascending).  The FP32 term forms L from
+// E0/E' = 1 + alpha (1 - cos theta) and needs no FP64 log or exp: its ratios
+// are good to ~3e-7 relative (the FP64 term keeps the table search in FP64).
+// n = 0: not available (too many segments in range); the FP32 term then
+// takes the FP64 form.
+constexpr int kMaxMuSegF = 24;
+struct MuTablesF {
+    int n;                       // segments
+    float lb[kMaxMuSegF];        // lb[j], j < n - 1: upper end of segment j in L
+    float c[4][kMaxMuSegF];
+    float s[4][kMaxMuSegF];
+};
+
+// Segment of L (branch-free count of the breakpoints below it).
+SMG_FN int mu_seg_f(const MuTablesF& t, float L) {
+    int j = 0;
+    for (int k = 0; k < t.n - 1; ++k) j += L > t.lb[k];
+    return j;
+}
+
+SMG_FN float mu_ratio_f(const MuTablesF& t, int which, int j, float L) {
+    return expf(t.c[which][j] - t.s[which][j] * L);
+}
+
+// Host: the FP32 form of t for photopeak e0 (alpha = e0 / 511).
+inline void make_mu_tables_f(const MuTables& t, double e0, MuTablesF& f) {
+    f.n = 0;
+    if (t.n < 2) return;
+    const double alpha = e0 / 511.0, lmax = std::log(1.0 + 2.0 * alpha) * (1.0 + 1e-6), le0 = std::log(e0);
+    double br[kMaxMuNodes];
+    int nb = 0;
+    for (int k = t.n - 1; k >= 0; --k) {  // nodes inside (E0 e^-lmax, E0), ascending in L
+        const double L = le0 - t.le[k];
+        if (L > 0.0 && L < lmax) br[nb++] = L;
+    }
+    if (nb + 1 > kMaxMuSegF) return;
+    for (int j = 0; j <= nb; ++j) {
+        const double a = j == 0 ? 0.0 : br[j - 1], b = j == nb ? lmax : br[j];
+        const double le = le0 - 0.5 * (a + b);  // inside segment j
+        int lo = 0, hi = t.n - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (t.le[mid] <= le) lo = mid;
+            else hi = mid;
+        }
+        if (j < nb) f.lb[j] = float(br[j]);
+        for (int p = 0; p < 4; ++p) {
+            const double slope = (t.lmu[p][lo + 1] - t.lmu[p][lo]) / (t.le[lo + 1] - t.le[lo]);
+            f.c[p][j] = float(t.lmu[p][lo] + slope * (le0 - t.le[lo]) - t.lmu0[p]);
+            f.s[p][j] = float(slope);
+        }
+    }
+    f.n = nb + 1;
+}
+
 struct ScatterPhys {
     double e0, lo, hi;     // photopeak energy and energy window (keV)
     double norm;           // sum of KN sin over [0, pi] (scatter.cu:1169)
@@ -157,6 +218,7 @@ struct ScatterPhys {
     double rcell;          // bounding radius of a detection sub-cell
     double mupe, muc;      // bar d
     const MuTables* tab = nullptr;  // mu(E) tables, or power laws
+    const MuTablesF* tabf = nullptr;  // their FP32 form (FP32 term), or none
     const double* knt = nullptr;    // KN prefix table (FP32 path), or the node loop
     double c_rej = -2.0;            // cos(theta) below this is outside the window for
                                     // sure (window_reject_cos with the path's margin)
@@ -561,16 +623,28 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
     // second leg and absorption at E' (scatter.cu:1371-1427); the FP32 path
     // takes E0/E' = 1 + alpha (1 - cos) for the power laws
     double e3, e1;
-    if (std::is_same<T, float>::value && !(ph.tab && ph.tab->n > 0)) {
+    const bool tables = ph.tab && ph.tab->n > 0;
+    const bool tables_f = std::is_same<T, float>::value && tables && ph.tabf && ph.tabf->n > 0;
+    float Lf = 0.0f;
+    int jf = 0;
+    if (std::is_same<T, float>::value && !tables) {
         e1 = 1.0 + alpha * omc;
         e3 = e1 * e1 * e1;
+    } else if (tables_f) {
+        Lf = logf(float(1.0 + alpha * omc));  // ln(E0/E')
+        jf = mu_seg_f(*ph.tabf, Lf);
+        e3 = mu_ratio_f(*ph.tabf, 0, jf, Lf);
+        e1 = mu_ratio_f(*ph.tabf, 1, jf, Lf);
     } else {
         mu_ratios(ph.tab, ph.e0, ep, e3, e1);
     }
     double att2 = e3 * sc.pt.s_pe + e1 * sc.pt.s_c;
     if (sc.pt.s_wpe != 0.0 || sc.pt.s_wc != 0.0) {  // collimator material (grid scatterers)
         double w3 = e3, w1 = e1;
-        if (ph.tab && ph.tab->n > 0) {
+        if (tables_f) {
+            w3 = mu_ratio_f(*ph.tabf, 2, jf, Lf);
+            w1 = mu_ratio_f(*ph.tabf, 3, jf, Lf);
+        } else if (tables) {
             const double le = log(ep);
             w3 = exp(mu_table_log(*ph.tab, 2, le) - ph.tab->lmu0[2]);
             w1 = exp(mu_table_log(*ph.tab, 3, le) - ph.tab->lmu0[3]);

@@ -15,10 +15,10 @@ int main(int argc, char* argv[]) {
     drv::ScatterArgs a = drv::parse_scatter_args(argc, argv, p, 3);
     drv::Dims d = drv::dims_of(p);
     drv::scatter_rotations(p, d, a, "Scatter_SysMat", p.phys[2] == 1, "SysMat_withScatter",
-                           [&](int, float* pe_block, float* dst) {
-                               int q = scatter(p.coll.data(), p.det.data(), p.img.data(), p.phys.data(),
-                                               pe_block, a.geo_crystal.c_str(), a.geo_coll.c_str(), dst,
-                                               a.cuda_id);
+                           [&](int, float* pe_drive, float* pe_add, float* dst, float* sum) {
+                               int q = scatter_and_sum(3, p.coll.data(), p.det.data(), p.img.data(),
+                                                       p.phys.data(), pe_drive, pe_add, a.geo_crystal.c_str(),
+                                                       a.geo_coll.c_str(), dst, sum, a.cuda_id);
                                std::printf("numImagebin = %d\n", q);
                            });
     return 0;

@@ -14,9 +14,10 @@ int main(int argc, char* argv[]) {
     drv::ScatterArgs a = drv::parse_scatter_args(argc, argv, p, 2);
     drv::Dims d = drv::dims_of(p);
     drv::scatter_rotations(p, d, a, "Scatter_SysMat", true, "SysMat",
-                           [&](int, float* pe_block, float* dst) {
-                               int q = scatter(p.coll.data(), p.det.data(), p.img.data(), p.phys.data(),
-                                               pe_block, a.geo_coll.c_str(), dst, a.cuda_id);
+                           [&](int, float* pe_drive, float* pe_add, float* dst, float* sum) {
+                               int q = scatter_and_sum(2, p.coll.data(), p.det.data(), p.img.data(),
+                                                       p.phys.data(), pe_drive, pe_add, nullptr,
+                                                       a.geo_coll.c_str(), dst, sum, a.cuda_id);
                                std::printf("numImagebin = %d\n", q);
                            });
     return 0;

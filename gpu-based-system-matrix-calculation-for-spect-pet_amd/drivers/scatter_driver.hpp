@@ -81,11 +81,12 @@ inline ScatterArgs parse_scatter_args(int argc, char* argv[], Params& p, int whi
 }
 
 // Rotation loop of the scatter drivers with one rotation block of each matrix
-// in memory.  compute(r, pe_block, dst) fills rotation r's scatter block from
-// the PE block that drives it: rotation 0's for every rotation, as the
-// reference drivers pass it, or rotation r's with -PErot.  Writes scatter_name
-// when write_scatter, and sum_name = PE (rotation r) + scatter when
-// Params_Physics[3] == 1, the sums the reference drivers form at the end.
+// in memory.  compute(r, pe_drive, pe_add, scatter_dst, sum_dst) fills rotation
+// r's scatter block from the PE block that drives it (rotation 0's for every
+// rotation, as the reference drivers pass it, or rotation r's with -PErot) and,
+// when sum_dst is not null, the PE (rotation r) + scatter block, the sum the
+// reference drivers form at the end; both blocks come from the GPU.  Writes
+// scatter_name when write_scatter, and sum_name when Params_Physics[3] == 1.
 template <class Compute>
 void scatter_rotations(Params& p, const Dims& d, const ScatterArgs& a, const char* scatter_name,
                        bool write_scatter, const char* sum_name, Compute compute) {
@@ -96,7 +97,7 @@ void scatter_rotations(Params& p, const Dims& d, const ScatterArgs& a, const cha
     pe_file.read(0, pe0.data());
     BlockWriter* ws = write_scatter ? new BlockWriter(sysmat_name(scatter_name, p), d.block()) : nullptr;
     BlockWriter* wsum = write_sum ? new BlockWriter(sysmat_name(sum_name, p), d.block()) : nullptr;
-    if (!ws) out.assign(d.block(), 0.0f);
+    if (!ws && !wsum) out.assign(d.block(), 0.0f);  // computed and not written, as the reference does
     for (int r = 0; r < d.n_rot; ++r) {
         std::printf("rotation %d\n", r);
         p.img[20] = (float)r;
@@ -105,13 +106,9 @@ void scatter_rotations(Params& p, const Dims& d, const ScatterArgs& a, const cha
             pe_file.read(r, pe_r.data());
             pe_here = pe_r.data();
         }
-        float* dst = ws ? ws->buffer() : out.data();
-        compute(r, a.pe_per_rotation ? pe_here : pe0.data(), dst);
-        if (wsum) {
-            float* s = wsum->buffer();
-            for (size_t i = 0; i < d.block(); ++i) s[i] = pe_here[i] + dst[i];
-            wsum->commit();
-        }
+        float* dst = ws ? ws->buffer() : (wsum ? nullptr : out.data());
+        compute(r, a.pe_per_rotation ? pe_here : pe0.data(), pe_here, dst, wsum ? wsum->buffer() : nullptr);
+        if (wsum) wsum->commit();
         if (ws) ws->commit();
     }
     if (ws) ws->close();

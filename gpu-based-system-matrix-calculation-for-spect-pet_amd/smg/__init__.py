@@ -38,7 +38,7 @@ EXPORTS = (
     "smg_bitmap_collimator", "smg_load_bitmap_crystal", "smg_load_bitmap_collimator",
     "smg_scatter_crystal", "smg_scatter_collimator", "smg_set_mu_tables",
     "smg_scatter_collimator_grid", "smg_scatter_crystal_grid", "smg_set_hole_shapes",
-    "smg_pe_forward_noalloc",
+    "smg_pe_forward_noalloc", "smg_scatter_sum", "smg_pe_tangent",
 )
 
 
@@ -99,6 +99,7 @@ def lib():
         "smg_pe_forward": (i32, [vp, i32, i32, vp, i32, st]),
         "smg_pe_forward_noalloc": (i32, [vp, i32, i32, vp, st]),
         "smg_pe_backward": (i32, [vp, i32, i32, vp, i32, dp, dp, st]),
+        "smg_pe_tangent": (i32, [vp, i32, i32, dp, vp, vp, i32, st]),
         "smg_pe_project": (i32, [vp, i32, i32, vp, i32, vp, i32, st]),
         "smg_pe_backproject": (i32, [vp, i32, i32, vp, i32, vp, i32, st]),
         "smg_bitmap_crystal_words": (i64, [i32]),
@@ -109,6 +110,7 @@ def lib():
         "smg_load_bitmap_collimator": (i32, [vp, up, i64]),
         "smg_scatter_crystal": (i32, [vp, i32, i32, vp, i32, vp, i32, st]),
         "smg_scatter_collimator": (i32, [vp, i32, i32, i32, vp, i32, st]),
+        "smg_scatter_sum": (i32, [vp, i32, C.c_uint, i32, vp, vp, i32, vp, vp, i32, st]),
         "smg_scatter_collimator_grid": (i32, [vp, i32, C.c_double, C.c_uint, i32, vp, i32,
                                               C.POINTER(C.c_int64), st]),
         "smg_scatter_crystal_grid": (i32, [vp, i32, i32, i32, i32, vp, i32, i32, vp, i32, st]),
@@ -262,6 +264,23 @@ class Context:
                                             C.byref(loss), C.byref(st)))
         return loss.value, grad, st.as_dict()
 
+    def pe_tangent(self, dmu: np.ndarray, rotation: int = 0, precision: str = "fp32",
+                   with_matrix: bool = False):
+        """dA = sum_k dmu[k] dA/dmu_k (dmu in gradient order); returns (dA, A or
+        None, stats)."""
+        n_par = self._n_layers + 2 * self.dims.n_det
+        dmu = np.ascontiguousarray(dmu, np.float64).reshape(n_par)
+        shape = (self.dims.n_det, self.dims.n_vox)
+        dA = np.empty(shape, np.float32)
+        A = np.empty(shape, np.float32) if with_matrix else None
+        st = Stats()
+        self._check(self._L.smg_pe_tangent(self._h, rotation, PREC[precision],
+                                           dmu.ctypes.data_as(C.POINTER(C.c_double)),
+                                           dA.ctypes.data_as(C.c_void_p),
+                                           None if A is None else A.ctypes.data_as(C.c_void_p), 0,
+                                           C.byref(st)))
+        return dA, A, st.as_dict()
+
     def pe_project(self, x: np.ndarray, rotation: int = 0,
                    precision: str = "fp32") -> tuple[np.ndarray, dict]:
         """y = A x with the PE matrix of `rotation`, matrix-free (x: V values)."""
@@ -343,6 +362,29 @@ class Context:
                                                    out.ctypes.data_as(C.c_void_p), 0,
                                                    C.byref(st)))
         return out, st.as_dict()
+
+    def scatter_sum(self, parts, pe_drive=None, pe_add=None, rotation: int = 0,
+                    precision: str = "fp32", scatter: bool = True, total: bool = True):
+        """Scatter block of `parts` ("crystal", "combined", "standalone"; the
+        crystal part driven by pe_drive) and pe_add + scatter, both formed on
+        the GPU.  Returns (scatter or None, sum or None, stats)."""
+        bits = {"crystal": 1, "combined": 2, "standalone": 4}
+        mask = 0
+        for name in parts:
+            mask |= bits[name]
+        shape = (self.dims.n_det, self.dims.n_vox)
+
+        def host(a):
+            return None if a is None else np.ascontiguousarray(a, np.float32).reshape(shape)
+
+        pe_drive, pe_add = host(pe_drive), host(pe_add)
+        sc = np.empty(shape, np.float32) if scatter else None
+        sm = np.empty(shape, np.float32) if total else None
+        ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        st = Stats()
+        self._check(self._L.smg_scatter_sum(self._h, rotation, mask, PREC[precision], ptr(pe_drive),
+                                            ptr(pe_add), 0, ptr(sc), ptr(sm), 0, C.byref(st)))
+        return sc, sm, st.as_dict()
 
     def scatter_collimator_grid(self, pitch_mm: float, rotation: int = 0,
                                 precision: str = "fp32", layers: int = 1):

@@ -12,11 +12,17 @@ parameters are theta = log mu.  Everything runs through the C ABI
 (smg_pe_forward / smg_pe_backward) on the GPU.
 
 Methods:
-  "levenberg-marquardt" (default): damped Gauss-Newton steps
-      (J^T J + lam diag(J^T J)) delta = -g / 2,
-  with g = dL/dtheta from the adjoint and J^T J from forward differences of
-  the GPU forward (one forward per parameter); a step is taken when it lowers
-  L (lam / 3), else lam * 4 and retry.  The problem is ill-conditioned (the
+  "gauss-newton" (default): damped Gauss-Newton steps
+      (J^T J + lam I) delta = -J^T r
+  solved by conjugate gradients on the products J^T (J p): J p from the
+  tangent (smg_pe_tangent, forward-mode derivative of the matrix along
+  mu * p) and J^T u from the adjoint (smg_pe_backward with w = u / Y), two
+  passes per product, so the pass count does not depend on the number of
+  parameters; a step is taken when it lowers L (lam / 3), else lam * 4 and
+  retry.
+  "levenberg-marquardt": the same steps with J^T J formed from forward
+  differences of the GPU forward (one forward per parameter) and
+  lam diag(J^T J) damping, solved directly (small parameter counts).  The problem is ill-conditioned (the
   Jacobian's condition number is ~1.6e4 on the 64-bar geometry: a common
   scale of the bars' mu_pe against the plate mu is the weakest direction), so
   first-order methods crawl along that valley; the Gauss-Newton curvature
@@ -67,6 +73,8 @@ class CalibrationResult:
     backward_ms: float = 0.0
     n_forward: int = 0
     n_backward: int = 0
+    n_tangent: int = 0
+    cg_iterations: list = field(default_factory=list)
 
 
 class _Problem:
@@ -93,6 +101,24 @@ class _Problem:
         self.res.n_forward += 1
         return (A[:, self.sources].astype(np.float64) - self.target) / self.target
 
+    def jvp(self, mu32: np.ndarray, p: np.ndarray) -> np.ndarray:
+        """J_theta p (relative residuals) at the context's geometry: the
+        tangent along dmu = mu * p on the source columns."""
+        dA, _, st = self.ctx.pe_tangent(mu32 * p, 0, self.prec)
+        self.res.forward_ms += st["kernel_ms"]
+        self.res.n_tangent += 1
+        return dA[:, self.sources].astype(np.float64) / self.target
+
+    def vjp(self, mu32: np.ndarray, u: np.ndarray) -> np.ndarray:
+        """J_theta^T u at the context's geometry: the adjoint with w = u / Y on
+        the source columns, times mu."""
+        self.w[:] = 0.0
+        self.w[:, self.sources] = (u / self.target).astype(np.float32)
+        _, g, st = self.ctx.pe_backward(self.w, precision=self.prec)
+        self.res.backward_ms += st["kernel_ms"]
+        self.res.n_backward += 1
+        return g * mu32
+
     def grad_theta(self, mu32: np.ndarray, r: np.ndarray) -> np.ndarray:
         """dL/dtheta at the mu of the last residuals call (the context's geometry)."""
         self.w[:] = 0.0
@@ -109,8 +135,9 @@ def _f32(mu: np.ndarray) -> np.ndarray:
 
 def calibrate(p_start: Params, target: np.ndarray, sources: np.ndarray, sub=(4, 4, 4),
               steps: int = 20, lr: float = 0.02, device: int = 0,
-              method: str = "levenberg-marquardt", precision: str = "fp32",
-              fd_step: float = 1e-4, lam0: float = 0.1) -> CalibrationResult:
+              method: str = "gauss-newton", precision: str = "fp32",
+              fd_step: float = 1e-4, lam0: float = 0.1, cg_iters: int = 60,
+              cg_tol: float = 1e-4) -> CalibrationResult:
     """target: [N][n_sources] projections of the sources (columns of the true
     matrix).  steps: iterations of the method (for "lbfgs", evaluations).
     Returns the fitted mu (gradient order) and the loss after each step."""
@@ -118,7 +145,9 @@ def calibrate(p_start: Params, target: np.ndarray, sources: np.ndarray, sub=(4, 
     res = CalibrationResult(mu=mu.copy())
     prob = _Problem(p_start, target, sources, sub, precision, device, res)
     try:
-        if method == "levenberg-marquardt":
+        if method == "gauss-newton":
+            mu = _gn_cg(prob, mu, steps, lam0, cg_iters, cg_tol)
+        elif method == "levenberg-marquardt":
             mu = _lm(prob, mu, steps, fd_step, lam0)
         elif method == "adam":
             mu = _adam(prob, mu, steps, lr)
@@ -130,6 +159,71 @@ def calibrate(p_start: Params, target: np.ndarray, sources: np.ndarray, sub=(4, 
         prob.close()
     res.mu = mu
     return res
+
+
+def _cg(matvec, b: np.ndarray, iters: int, tol: float) -> tuple[np.ndarray, int]:
+    """Conjugate gradients for M x = b with M symmetric positive definite,
+    from x = 0, until |r| <= tol |b| or iters products."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    pdir = r.copy()
+    rr = float(r @ r)
+    bb = rr
+    k = 0
+    while k < iters and rr > tol * tol * bb:
+        q = matvec(pdir)
+        alpha = rr / float(pdir @ q)
+        x += alpha * pdir
+        r -= alpha * q
+        rr_new = float(r @ r)
+        pdir = r + (rr_new / rr) * pdir
+        rr = rr_new
+        k += 1
+    return x, k
+
+
+def _gn_cg(prob: _Problem, mu: np.ndarray, steps: int, lam: float, cg_iters: int,
+           cg_tol: float) -> np.ndarray:
+    """Damped Gauss-Newton in theta = log mu with CG on J^T J products (two
+    GPU passes each, tangent and adjoint).  lam is relative to the mean
+    diagonal of J^T J, estimated from the first products."""
+    r = prob.residuals(mu)
+    L = float(np.sum(r * r))
+    prob.res.losses.append(L)
+    scale = None
+    for _ in range(steps):
+        g = prob.vjp(mu, r)  # J^T r = dL/dtheta / 2
+        if scale is None:
+            # mean diagonal of J^T J from a few random +-1 probes (E[z (M z)] = diag M)
+            rng = np.random.default_rng(0)
+            est = np.zeros_like(mu)
+            for _k in range(4):
+                z = rng.choice([-1.0, 1.0], mu.size)
+                est += z * prob.vjp(mu, prob.jvp(mu, z))
+            scale = max(float(np.mean(np.abs(est))) / 4.0, 1e-30)
+
+        def matvec(v, lam_abs):
+            return prob.vjp(mu, prob.jvp(mu, v)) + lam_abs * v
+
+        while True:
+            lam_abs = lam * scale
+            delta, k = _cg(lambda v: matvec(v, lam_abs), -g, cg_iters, cg_tol)
+            prob.res.cg_iterations.append(k)
+            mu_new = _f32(mu * np.exp(delta))
+            r_new = prob.residuals(mu_new)
+            L_new = float(np.sum(r_new * r_new))
+            if L_new < L:
+                mu, r, L = mu_new, r_new, L_new
+                lam = max(lam / 3.0, 1e-12)
+                break
+            # the residuals call moved the context to mu_new: back to mu
+            prob.residuals(mu)
+            lam *= 4.0
+            if lam > 1e8:  # no step lowers L: converged to the arithmetic's resolution
+                prob.res.losses.append(L)
+                return mu
+        prob.res.losses.append(L)
+    return mu
 
 
 def _lm(prob: _Problem, mu: np.ndarray, steps: int, h: float, lam: float) -> np.ndarray:

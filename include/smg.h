@@ -55,6 +55,13 @@ typedef enum smg_coll_variant {
                                 (ScatterGen_Collimator/scatter.cu:1207-1215) */
 } smg_coll_variant;
 
+/* Parts of smg_scatter_sum (bit mask). */
+typedef enum smg_scatter_part {
+    SMG_PART_CRYSTAL = 1,          /* crystal scatter (smg_scatter_crystal) */
+    SMG_PART_COLL_COMBINED = 2,    /* collimator scatter, r = 0.1 mm (combined driver) */
+    SMG_PART_COLL_STANDALONE = 4   /* collimator scatter, r = 0.2 mm (standalone driver) */
+} smg_scatter_part;
+
 typedef struct smg_stats {
     double kernel_ms;          /* GPU time of the main kernel(s), HIP events */
     double total_ms;           /* host wall time of the call */
@@ -131,6 +138,16 @@ smg_status smg_set_hole_shapes(smg_ctx* ctx, const float* shapes, int n_holes);
 smg_status smg_pe_forward(smg_ctx* ctx, int rotation, smg_precision prec, float* dst,
                           int dst_is_device, smg_stats* stats);
 
+/* Tangent (forward-mode derivative) of smg_pe_forward along a direction dmu
+ * over the parameters of smg_pe_backward's gradient (dmu[0 .. Lc-1] layers,
+ * dmu[Lc .. Lc+N-1] mu_total and dmu[Lc+N .. Lc+2N-1] mu_pe of each bar, host
+ * memory): dA[d][v] = sum_k dmu[k] dA[d][v]/dmu_k, N x V floats, and the
+ * matrix itself into A (optional, NULL to skip).  With smg_pe_backward it
+ * gives Gauss-Newton products J^T (J v) at two passes per product, whatever
+ * the number of parameters.  No counterpart in the reference. */
+smg_status smg_pe_tangent(smg_ctx* ctx, int rotation, smg_precision prec, const double* dmu, float* dA,
+                          float* A, int dst_is_device, smg_stats* stats);
+
 /* smg_pe_forward with no allocation, for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  No counterpart in the reference.  dst_device: a device
@@ -195,6 +212,20 @@ smg_status smg_scatter_crystal(smg_ctx* ctx, int rotation, smg_precision prec, c
 smg_status smg_scatter_collimator(smg_ctx* ctx, int rotation, smg_coll_variant variant,
                                   smg_precision prec, float* dst, int dst_is_device,
                                   smg_stats* stats);
+/* The scatter block of the parts in `parts` (smg_scatter_part bits: crystal
+ * scatter driven by pe_drive, and one collimator variant), summed on the GPU,
+ * into scatter_dst, and pe_add + scatter into sum_dst: the blocks the reference
+ * drivers form on the host (scatter() of the combined generator adds crystal
+ * and collimator scatter, ScatterGen_RayTracing_CircularHole/scatter.cu:
+ * 2354-2357; every driver writes PE + scatter, ScatterGen_CircularHole.cpp:
+ * 211-230, ScatterGen_Collimator.cpp and ScatterGen_Crystal.cpp likewise).
+ * Either output may be NULL (not both); pe_drive and pe_add may be the same
+ * block; both inputs are host or both device memory (pe_is_device), both
+ * outputs likewise (dst_is_device).  Same values as the separate calls: the
+ * parts are added in float32 in the order crystal, collimator, then PE. */
+smg_status smg_scatter_sum(smg_ctx* ctx, int rotation, unsigned parts, smg_precision prec,
+                           const float* pe_drive, const float* pe_add, int pe_is_device,
+                           float* scatter_dst, float* sum_dst, int dst_is_device, smg_stats* stats);
 
 /* Collimator scatter with scatter points on a grid over the tungsten of the
  * layers in layer_mask (bit m: layer m; 1 = the layer-0 plate), pitch_mm,

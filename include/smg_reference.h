@@ -50,4 +50,17 @@ int scatter(float* parameter_Collimator, float* parameter_Detector, float* param
 int scatter(float* parameter_Detector, float* parameter_Image, float* parameter_Physics,
             float* PE_SysMat, const char* FnameGeo, float* dst, int cuda_id);
 
+/* Extension with no reference counterpart, for the drivers: the scatter block
+ * of which = 1 (crystal, as the 7-argument scatter), 2 (collimator r = 0.2 mm,
+ * as the 8-argument one) or 3 (combined, as the 9-argument one) into
+ * scatter_dst, and PE_add + scatter into sum_dst, both formed on the GPU
+ * (smg_scatter_sum); the reference drivers add PE and scatter on the host
+ * (ScatterGen_RayTracing_CircularHole/ScatterGen_CircularHole.cpp:211-230).
+ * Either output may be NULL, not both; parameter_Collimator and
+ * FnameGeoCollimator are unused for which = 1, FnameGeoCrystal for which = 2. */
+int scatter_and_sum(int which, float* parameter_Collimator, float* parameter_Detector,
+                    float* parameter_Image, float* parameter_Physics, float* PE_SysMat, float* PE_add,
+                    const char* FnameGeoCrystal, const char* FnameGeoCollimator, float* scatter_dst,
+                    float* sum_dst, int cuda_id);
+
 #endif /* SMG_REFERENCE_H */

@@ -110,6 +110,8 @@ def load_lane_lib():
     L.lane_scatter_pairs.restype = C.c_long
     L.lane_scatter_pairs.argtypes = [fp, fp, fp, fp, fp, up] + [C.c_int] * 7 + [ip, ip, C.c_int, dp]
     L.lane_bitmap.restype = C.c_long
+    L.lane_mu_ratio_f.restype = C.c_int
+    L.lane_mu_ratio_f.argtypes = [C.c_double, C.c_int] + [dp] * 6 + [C.c_int, dp, dp]
     L.lane_window_reject_cos.restype = C.c_double
     L.lane_window_reject_cos.argtypes = [C.c_double] * 4
     L.lane_set_hole_shapes.restype = None

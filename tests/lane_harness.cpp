@@ -7,6 +7,7 @@
 // rays, or everything in FP64.  The GPU kernel adds the wave-level candidate
 // narrowing, which is exact by construction and is checked on the GPU.
 #include <algorithm>
+#include <array>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -21,6 +22,7 @@ using namespace smg;
 namespace {
 
 struct VecCands {
+    static constexpr bool kFastFaces = false;
     std::vector<int> bars, holes;
     int nbars() const { return (int)bars.size(); }
     int bar(int j) const { return bars[j]; }
@@ -100,6 +102,7 @@ double element(const Ctx& c, int d, int v, int rot, int sx, int sy, int sz, long
 // Candidate lists narrowed for one sub-cell: a wave-uniform list of bars and
 // holes with per-lane masks, as the GPU kernel holds them in LDS.
 struct MaskCands {
+    static constexpr bool kFastFaces = false;
     std::vector<int> b, h;
     std::vector<uint64_t> bm, hm;
     int lane = 0;
@@ -109,6 +112,17 @@ struct MaskCands {
     int nholes() const { return (int)h.size(); }
     int hole(int j) const { return h[j]; }
     bool hole_on(int j) const { return (hm[j] >> lane) & 1ull; }
+};
+
+// MaskCands with the face records of the first 32 narrowed bars, as the GPU
+// kernels keep them in LDS (fast_faces), so pe_cell_ray<float> takes
+// fast_bar_chord for the bars that qualify.
+struct FastMaskCands : MaskCands {
+    static constexpr bool kFastFaces = true;
+    std::vector<char> ok;
+    std::vector<std::array<float, 6>> f;
+    bool fast(int j) const { return j < 32 && j < (int)ok.size() && ok[j]; }
+    const float* faces(int j) const { return f[j].data(); }
 };
 
 }  // namespace
@@ -131,7 +145,11 @@ extern "C" {
 // waves, which the kernels hold in LDS up to 512 bars and 256 holes;
 // stats[10] = the most narrowed bars of one sub-cell, stats[11] = the share of
 // sub-cells with more than 8 (the adjoint keeps the chords of the first 8 in
-// LDS and recomputes the others).
+// LDS and recomputes the others); the FP32 term with the narrowed lists and
+// the face records of fast_bar_chord against the FP64 term: stats[12] = the
+// largest relative difference over the terms it does not flag, stats[13] =
+// the share of narrowed bars (the first 32) that qualify, stats[14] = the
+// share of flagged terms.
 int lane_narrow_check(const float* coll, const float* det, const float* img, const float* phys,
                       int rot, int sx, int sy, int sz, int tx, int ty, int tz, int n_waves,
                       unsigned seed, double* stats) {
@@ -150,7 +168,8 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
         return st;
     };
     double bad = 0, checked = 0, nsb = 0, nsh = 0, nfull = 0, cells = 0, ntile = 0, nwav = 0, gates = 0,
-           bar_gates = 0, max_tb = 0, max_th = 0, max_nsb = 0, over8 = 0;
+           bar_gates = 0, max_tb = 0, max_th = 0, max_nsb = 0, over8 = 0, max_fast = 0, n_fast = 0,
+           n_fastable = 0, n_near = 0;
     for (int w = 0; w < n_waves; ++w) {
         int d = (int)(next() % g.n_det);
         int t = (int)(next() % ((uint64_t)nt[0] * nt[1] * nt[2]));
@@ -260,14 +279,29 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
                         if (!((bar_gate >> m) & 1u) && plate_gate_clear(G.layers_d[m], cc.q, tc, rho))
                             cg.gate_clear |= 1u << m;
                     gates += __builtin_popcount(cg.gate_clear);
+                    FastMaskCands fm;
+                    static_cast<MaskCands&>(fm) = nar;
+                    for (int j = 0; j < (int)nar.b.size() && j < 32; ++j) {
+                        std::array<float, 6> fr;
+                        fm.ok.push_back(fast_faces(G, nar.b[j], cc.q, tc, rho, fr.data()) ? 1 : 0);
+                        fm.f.push_back(fr);
+                        n_fast += fm.ok.back();
+                        n_fastable += 1;
+                    }
                     for (int l = 0; l < tx * ty * tz; ++l) {
                         if (!valid[l]) continue;
-                        full.lane = nar.lane = l;
-                        bool n1 = false, n2 = false;
+                        full.lane = nar.lane = fm.lane = l;
+                        bool n1 = false, n2 = false, n3 = false;
                         double v1 = pe_cell_ray<double, true>(G, d, P[l], PS[l], cc, full, n1);
                         double v2 = pe_cell_ray<double, true>(G, d, P[l], PS[l], cg, nar, n2);
                         checked += 1;
                         if (!(v1 == v2)) bad += 1;
+                        double v3 = (double)pe_cell_ray<float, false>(G, d, P[l], PS[l], cg, fm, n3);
+                        if (n3) {
+                            n_near += 1;
+                        } else if (v1 != 0.0 || v3 != 0.0) {
+                            max_fast = std::max(max_fast, std::fabs(v3 - v1) / std::fabs(v1));
+                        }
                     }
                 }
     }
@@ -283,6 +317,9 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
     stats[9] = max_th;
     stats[10] = max_nsb;
     stats[11] = cells > 0 ? over8 / cells : 0;
+    stats[12] = max_fast;
+    stats[13] = n_fastable > 0 ? n_fast / n_fastable : 0;
+    stats[14] = checked > 0 ? n_near / checked : 0;
     return 0;
 }
 
@@ -493,6 +530,36 @@ void lane_kn_sum_table(double e0, const float* lo, const float* hi, int n, doubl
         out[i] = kn_sum_table(knt.data(), lo[i], hi[i], nr);
         near[i] = nr;
     }
+}
+
+// mu(E')/mu(E0) of the four tables at E' = ep[i]: out32 from the FP32 form
+// the FP32 scatter term uses (piecewise power law in ln(E0/E')), out64 from the
+// FP64 table search, 4 values per point (crystal pe, crystal Compton,
+// collimator pe, collimator Compton).  Returns the FP32 form's segment count
+// (0: not available).
+int lane_mu_ratio_f(double e0, int n_nodes, const double* energy, const double* cpe, const double* cc,
+                    const double* wpe, const double* wc, const double* ep, int n, double* out32,
+                    double* out64) {
+    MuTables t{};
+    t.n = n_nodes;
+    const double* v[4] = {cpe, cc, wpe, wc};
+    for (int i = 0; i < n_nodes; ++i) {
+        t.le[i] = std::log(energy[i]);
+        for (int k = 0; k < 4; ++k) t.lmu[k][i] = std::log(v[k][i]);
+    }
+    for (int k = 0; k < 4; ++k) t.lmu0[k] = mu_table_log(t, k, std::log(e0));
+    MuTablesF f{};
+    make_mu_tables_f(t, e0, f);
+    const double alpha = e0 / 511.0;
+    for (int i = 0; i < n; ++i) {
+        const float L = logf(float(1.0 + alpha * (e0 / ep[i] - 1.0) / alpha));
+        const int j = f.n > 0 ? mu_seg_f(f, L) : 0;
+        for (int k = 0; k < 4; ++k) {
+            out64[4 * i + k] = std::exp(mu_table_log(t, k, std::log(ep[i])) - t.lmu0[k]);
+            out32[4 * i + k] = f.n > 0 ? (double)mu_ratio_f(f, k, j, L) : 0.0;
+        }
+    }
+    return f.n;
 }
 
 // The FP32 scatter term's lower-window threshold on cos(theta).

@@ -449,3 +449,29 @@ def test_collimator_grid_scatter_three_layers(smg_mod, oracle_mod, params_mod, p
     assert_parity(got[d, v], ref, tol, f"three-layer collimator grid scatter {prec}")
     total = sum(q[0].astype(np.float64) for q in parts)
     np.testing.assert_allclose(got, total, rtol=1e-5, atol=1e-7 * np.abs(total).max())
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_scatter_sum_on_gpu(smg_mod, params_mod, prec):
+    """smg_scatter_sum forms the combined generator's crystal + collimator
+    scatter and the drivers' PE + scatter block on the GPU: the same floats
+    as the separate calls added on the host (ScatterGen_RayTracing_CircularHole/
+    scatter.cu:2354-2357, ScatterGen_CircularHole.cpp:211-230)."""
+    p = params_mod.config(3)
+    with smg_mod.open_context(p) as ctx:
+        ctx.set_pe_subsamples(2, 2, 2)
+        pe, _ = ctx.pe_forward(0, "fp32")
+        pe_add = (pe * np.float32(1.25)).astype(np.float32)
+        cry, _ = ctx.scatter_crystal(pe, 0, prec)
+        comb, _ = ctx.scatter_collimator(0, "combined", prec)
+        stand, _ = ctx.scatter_collimator(0, "standalone", prec)
+        sc, sm, st = ctx.scatter_sum(["crystal", "combined"], pe, pe_add, 0, prec)
+        sc2, sm2, _ = ctx.scatter_sum(["standalone"], None, pe_add, 0, prec)
+        _, sm3, _ = ctx.scatter_sum(["crystal"], pe, pe, 0, prec, scatter=False)
+        with pytest.raises(smg_mod.SmgError):
+            ctx.scatter_sum(["combined", "standalone"], None, pe, 0, prec)
+    assert np.array_equal(sc, cry + comb)
+    assert np.array_equal(sm, pe_add + (cry + comb))
+    assert np.array_equal(sc2, stand) and np.array_equal(sm2, pe_add + stand)
+    assert np.array_equal(sm3, pe + cry)
+    assert st["rays"] > 0 and (sc > 0).mean() > 0.5

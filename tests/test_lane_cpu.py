@@ -88,13 +88,17 @@ def test_lane_narrowing_is_exact(lane_lib, params_mod, case):
         rot = 2
     else:
         p, sub, tile, waves = narrow_plate_geometry(params_mod), (3, 4, 2), (8, 4, 2), 60
-    st = np.zeros(12)
+    st = np.zeros(15)
     assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, rot, *sub, *tile, waves, 5, st) == 0
     print(f"narrowing {case}: {int(st[1])} terms, {st[2]:.2f} bars and {st[3]:.2f} holes per "
           f"sub-cell (full sets {st[4]:.1f} bars, tile candidates {st[5]:.1f}), "
-          f"{100 * st[6]:.1f}% of plate gates skipped, {100 * st[7]:.1f}% for the whole bar")
+          f"{100 * st[6]:.1f}% of plate gates skipped, {100 * st[7]:.1f}% for the whole bar; "
+          f"FP32 term with fast bar chords: {100 * st[13]:.0f}% of bars qualify, max rel "
+          f"{st[12]:.2e} over unflagged terms, {100 * st[14]:.2f}% flagged")
     assert st[1] > 0
     assert st[0] == 0
+    # the FP32 ray with the face-record chords, unflagged, within 2e-6 of the FP64 ray
+    assert st[12] <= 2e-6
     if case == "narrowplate":
         assert 0.05 < st[6] < 0.95  # both the skipped and the traced gates are exercised
 
@@ -229,7 +233,7 @@ def test_lane_overflow_geometry(lane_lib, oracle_mod, params_mod):
     narrowing stays exact there, and the lane arithmetic with full candidate
     lists, which is what the spill path computes, matches the oracle."""
     p = overflow_geometry(params_mod)
-    st = np.zeros(12)
+    st = np.zeros(15)
     assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, 0, 1, 2, 1, 4, 4, 4, 24, 7, st) == 0
     print(f"overflow geometry: largest tile lists {st[8]:.0f} bars, {st[9]:.0f} holes", flush=True)
     assert st[8] > 512 and st[9] > 256
@@ -253,9 +257,28 @@ def test_lane_full_size_long_candidate_lists(lane_lib, params_mod):
     keeps in LDS; the narrowing is exact there."""
     p = params_mod.config(2)
     params_mod.set_image(p, 16, 16, 8, 0.5, 0.5, 0.5)
-    st = np.zeros(12)
+    st = np.zeros(15)
     assert lane_lib.lane_narrow_check(p.coll, p.det, p.img, p.phys, 0, 3, 3, 3, 4, 4, 4, 40, 9, st) == 0
     print(f"full size, 16x16x8 image: up to {st[10]:.0f} narrowed bars per sub-cell, "
           f"{100 * st[11]:.1f}% of sub-cells above 8", flush=True)
     assert st[0] == 0 and st[1] > 0
     assert st[10] > 8 and st[11] > 0.01
+
+
+@pytest.mark.parametrize("power_law", [True, False])
+@pytest.mark.parametrize("e0", [140.5, 190.0, 250.0])
+def test_lane_mu_tables_fp32_form(lane_lib, params_mod, power_law, e0):
+    """The FP32 scatter term's form of the mu(E) tables (a piecewise power law
+    in ln(E0/E') over the segments E' can reach) against the FP64 table
+    search, over the whole range E0/(1 + 2 alpha) <= E' <= E0, including E0
+    beyond the last node (250 keV: the end segment continued)."""
+    tabs = [np.ascontiguousarray(t, np.float64) for t in params_mod.synthetic_mu_tables(power_law)]
+    alpha = e0 / 511.0
+    ep = np.linspace(e0 / (1 + 2 * alpha), e0, 5001)
+    o32 = np.zeros(4 * ep.size)
+    o64 = np.zeros(4 * ep.size)
+    n_seg = lane_lib.lane_mu_ratio_f(e0, len(tabs[0]), *tabs, ep, ep.size, o32, o64)
+    assert n_seg > 1
+    rel = np.abs(o32 / o64 - 1)
+    print(f"mu(E) FP32 form, E0 {e0}, {n_seg} segments: max rel {rel.max():.2e}", flush=True)
+    assert rel.max() < 6e-7
