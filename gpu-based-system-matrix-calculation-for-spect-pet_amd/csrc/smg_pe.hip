@@ -505,8 +505,11 @@ __device__ __forceinline__ bool pe_forward_body(const PeLaunch& A, int item, Lis
     return true;
 }
 
+// 3 waves per SIMD: uncapped, the FP32 allocation lands at 169 registers (2
+// waves per SIMD, 36% slower on the full-size step); the cap costs a 12-byte
+// spill.  The same holds for the projection below.
 template <bool kExact>
-__global__ __launch_bounds__(64) void pe_forward_kernel(PeLaunch A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_forward_kernel(PeLaunch A) {
     __shared__ WaveLists L;
     if (!pe_forward_body<kExact, false>(A, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
 }
@@ -1032,7 +1035,7 @@ __device__ __forceinline__ bool pe_project_body(const PeLaunch& A, const float* 
 }
 
 template <bool kExact>
-__global__ __launch_bounds__(64) void pe_project_kernel(PeLaunch A, const float* x, double* partial) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_project_kernel(PeLaunch A, const float* x, double* partial) {
     __shared__ WaveLists L;
     if (!pe_project_body<kExact>(A, x, partial, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
 }

@@ -502,10 +502,18 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
                      const double* tan_bar = nullptr, double* tan_att = nullptr) {
     const GeomDesc& g = G.g;
     const Bar<double>& bd = G.bars_d[d];
+    constexpr bool kF32 = std::is_same<T, float>::value;
     double D[3] = {cc.q[0] - p[0], cc.q[1] - p[1], cc.q[2] - p[2]};
     double dsq = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
-    double dd = sqrt(dsq);
-    double cosang = D[1] / dd;
+    // FP64 path: |D| and cos(alpha) as the reference forms them; the FP32
+    // path takes |D| and 1/|D| from a refined hardware reciprocal square root
+    // below, where the hole and bar chords need them
+    double dd = 0.0, inv_dd = 0.0, cosang = 0.0;
+    if constexpr (!kF32) {
+        dd = sqrt(dsq);
+        inv_dd = 1.0 / dd;
+        cosang = D[1] / dd;
+    }
 
     // solid angle: largest of the three projected cell faces (PESysMatGen.cu:1223-1231)
     T om;
@@ -519,7 +527,8 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
         // one reciprocal of 4 pi |D|^3 for the three faces
         const float cb = float(bd.cb), sb = float(bd.sb);
         const float Dx = float(D[0]), Dy = float(D[1]), Dz = float(D[2]);
-        const float r = trcp(float(4.0 * 3.14159265358979323846 * dsq * dd));
+        const float dsqf = float(dsq);
+        const float r = trcp(float(4.0 * 3.14159265358979323846) * dsqf * tsqrt(dsqf));
         float omx = float(cc.area[0]) * tabs(Dx * cb - Dz * sb);
         float omz = float(cc.area[2]) * tabs(Dx * sb + Dz * cb);
         float omy = float(cc.area[1]) * tabs(Dy);
@@ -558,14 +567,14 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
     // path forms T / cos(alpha) as T |D| / D_y and the hole-ray set-up from
     // the same two reciprocals, only when the sub-cell has hole candidates.
     CylRay cr;
-    double inv_dd, inv_cos;
+    double inv_cos;
     if constexpr (std::is_same<T, double>::value) {
         cyl_ray(D, dd, cr);
-        inv_dd = 1.0 / dd;
         inv_cos = 0.0;
     } else {
-        inv_dd = 1.0 / dd;
-        inv_cos = dd / D[1];
+        inv_dd = rsq_fast(dsq);
+        dd = dsq * inv_dd;
+        inv_cos = tabs(D[1]) > 1e-300 ? dd * rcp_fast(D[1]) : dd / D[1];
         if (C.nholes() > 0) cyl_ray_from(D, inv_dd, inv_cos, cr);
     }
     for (int m = 0; m < g.n_layers; ++m) {
@@ -586,7 +595,7 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
                 l = o.erx > 0.0 ? ell_chord(o, p[0], p[1], p[2], cr.kx, cr.kz, cr.inv_ky, dd)
                                 : cyl_chord(p, cr, dd, o.xc, o.y1, o.y2, o.zc, o.r);
             else
-                l = cyl_chord(p, cr, dd, o.xc, o.y1, o.y2, o.zc, o.r);
+                l = cyl_chord<kF32>(p, cr, dd, o.xc, o.y1, o.y2, o.zc, o.r);
             sum += l;
             att += o.mu * l;
         }

@@ -82,6 +82,34 @@ SMG_FN T tsqrt(T x) {
     return sqrt(x);
 }
 
+// FP64 reciprocal, and reciprocal and square root, for the FP32 path's FP64
+// set-up: the hardware approximations (v_rcp_f64, v_rsq_f64) refined by two
+// Newton steps, within a few ulp of the correctly rounded values at a third of
+// the instructions of IEEE division and square root.  Arguments: positive
+// normal numbers (callers guard the rest).  The host build (lane tests) and
+// the all-FP64 path use the operators.
+SMG_FN double rcp_fast(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+SMG_FN double rsq_fast(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * fma(-h * y, y, 1.5);
+    return y * fma(-h * y, y, 1.5);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
 // Element k of a 3-vector by selects: indexing with a run-time k would put
 // the whole array in scratch memory.
 template <class T>
@@ -427,7 +455,7 @@ SMG_FN void cyl_ray_from(const double D[3], double il, double inv_ky, CylRay& c)
     c.sxk = c.kx * inv_ky;
     c.szk = c.kz * inv_ky;
     c.qa = c.kx * c.kx + c.kz * c.kz;
-    c.inv_qa = 1.0 / c.qa;
+    c.inv_qa = c.qa > 1e-300 ? rcp_fast(c.qa) : 1.0 / c.qa;
 }
 
 // Chord through the finite cylinder of radius r, axis parallel to y at
@@ -435,6 +463,7 @@ SMG_FN void cyl_ray_from(const double D[3], double il, double inv_ky, CylRay& c)
 // (PESysMatGen.cu:688-799).  Evaluated in FP64: rays through a hole run
 // almost along its axis, where kx^2 + kz^2 -> 0 and the quadratic's roots
 // lose every FP32 digit.
+template <bool kFastSqrt = false>
 SMG_FN double cyl_chord(const double a[3], const CylRay& c, double len, double xc, double y1,
                         double y2, double zc, double r) {
     if (fabs(y1 - y2) < kCylDy) return 0.0;
@@ -459,7 +488,7 @@ SMG_FN double cyl_chord(const double a[3], const CylRay& c, double len, double x
     double cr = kx * z0 - kz * x0;
     double disc = qa * r2 - cr * cr;
     if (disc <= kCylDisc) return 0.0;
-    double sq = sqrt(disc);
+    double sq = kFastSqrt ? disc * rsq_fast(disc) : sqrt(disc);
     double qb = kx * x0 + kz * z0;
     double t1 = (-qb - sq) * c.inv_qa;
     double t2 = (-qb + sq) * c.inv_qa;
