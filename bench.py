@@ -14,8 +14,10 @@ the reference's scatter points (crystal scatter on 32x32x1 voxels x 2048 bars,
 collimator scatter on 32x32x16 voxels x 64 bars), with scatter points on a
 0.5 mm grid over the collimator tungsten and on a 0.25 mm grid in every bar
 (8 detector rows of 2048 bars), the matrix-free forward and back-projection
-times with the headline matrix, and the forward and adjoint (dL/dmu) times on
-the 32x32x1-voxel, 64-bar geometry and on the full-size geometry.
+times with the headline matrix, the forward and adjoint (dL/dmu) times on
+the 32x32x1-voxel, 64-bar geometry and on the full-size geometry, and one
+PE launch at a 200 x 200 2D field of view with 6,000 bars and 8 x 16 x 8
+sub-cells.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision fp32|fp64]
 """
@@ -137,7 +139,7 @@ def timed(fn, reps):
 
 
 WORKLOADS = ("crystal_scatter", "collimator_scatter", "collimator_grid", "crystal_grid",
-             "bitmaps", "adjoint")
+             "bitmaps", "adjoint", "large_fov_pe")
 
 
 def extras(smg, params, pe_ctx, out, precision, sections=None, precs=("fp32", "fp64")):
@@ -165,6 +167,8 @@ def extras(smg, params, pe_ctx, out, precision, sections=None, precs=("fp32", "f
         _bitmaps(np, smg, params, ex)
     if want("adjoint"):
         _adjoint(np, smg, params, ex)
+    if want("large_fov_pe"):
+        _large_fov_pe(smg, params, ex)
     if pe_ctx is not None:
         ms_this = ex["projection"]["matrix_kernel_ms"]
         _adjoint_full_size(np, pe_ctx, ms_this, ex)
@@ -353,6 +357,25 @@ def _adjoint(np, smg, params, ex):
                      "backward_over_forward": round(b_ms / f_ms, 2),
                      "workload": "32x32x1 voxels x 64 bars, 4x4x4 sub-cells, 1 + 128 mu"}
     log(f"adjoint: forward {f_ms:.3f} ms, backward {b_ms:.3f} ms")
+
+
+def _large_fov_pe(smg, params, ex):
+    # the PE matrix at the size the reference's README quotes (200 x 200 2D FOV, 6,000
+    # bars, 8 x 16 x 8 sub-cells): one FP32 launch into a 0.96 GB device buffer
+    import torch
+    p = params.large_fov_pe()
+    with smg.open_context(p) as c:
+        c.set_pe_subsamples(*p.meta["subsamples"])
+        buf = torch.empty(c.dims.n_det * c.dims.n_vox, dtype=torch.float32, device="cuda")
+        _, st = c.pe_forward(0, "fp32", device_ptr=buf.data_ptr())
+        del buf
+    ms = st["kernel_ms"]
+    ex["large_fov_pe"] = {"kernel_ms": round(ms, 1), "elements_per_s": round(st["elements"] / ms * 1e3, 1),
+                          "sub_cell_rays_per_s": round(st["rays"] / ms * 1e3, 1),
+                          "fp64_fallback_rays": st["fp64_fallbacks"], "overflow_waves": st["overflow_waves"],
+                          "workload": "200x200x1 voxels at 0.5 mm x 6000 bars (3 layers of 50x40), "
+                                      "400-hole plate, 8x16x8 sub-cells, FP32; one launch"}
+    log(f"PE 200x200 FOV, 6000 bars: {ms:.0f} ms")
 
 
 def _adjoint_full_size(np, pe_ctx, ms_this, ex):

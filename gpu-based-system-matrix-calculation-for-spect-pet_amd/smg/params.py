@@ -282,6 +282,18 @@ def large_detector(n_layers: int = 3, nx: int = 50, nz: int = 40) -> Params:
     return p
 
 
+def large_fov_pe() -> Params:
+    """The system size the reference's README quotes a PE time for (ReadMe.md:
+    200): a 200 x 200 2D field of view (0.5 mm voxels, one slice) and 6,000
+    bars (:func:`large_detector`), at the reference's default 8 x 16 x 8
+    sub-cells (PESysMatGen.cu:1037): 240,000,000 elements, 2.46e11 sub-cell
+    rays."""
+    p = large_detector()
+    set_image(p, 200, 200, 1, 0.5, 0.5, 0.5)
+    p.meta = dict(name="pe_200x200x1_6000bars", subsamples=(8, 16, 8))
+    return p
+
+
 def synthetic_mu_tables(power_law: bool = True, e0: float = 140.5):
     """mu(E) tables of the scatter workloads: 64 log-spaced nodes on 20-200
     keV, returned as (energy_kev, crystal_pe, crystal_compton, coll_pe,

@@ -117,3 +117,34 @@ def test_zero_mu_bar_gives_zero_row(smg_mod, params_mod):
         got, _ = ctx.pe_forward(0, "fp32")
     assert np.all(got[5] == 0.0)
     assert np.all(got[4] > 0.0)
+
+
+def test_large_fov_sampled(smg_mod, oracle_mod, params_mod):
+    """PE at the size the reference's README quotes a time for (ReadMe.md:200):
+    200 x 200 2D FOV, 6000 bars in three layers, 8 x 16 x 8 sub-cells, FP32.
+    Every element on the GPU into a device buffer; in the oracle 1000 random
+    elements and the 1000 largest of 400,000 random ones (rays through holes)."""
+    torch = pytest.importorskip("torch")
+    p = params_mod.large_fov_pe()
+    sub = p.meta["subsamples"]
+    with _ctx(smg_mod, p, sub) as ctx:
+        n_det, n_vox = ctx.dims.n_det, ctx.dims.n_vox
+        assert (n_det, n_vox) == (6000, 40000)
+        buf = torch.empty(n_det * n_vox, dtype=torch.float32, device="cuda")
+        _, st = ctx.pe_forward(0, "fp32", device_ptr=buf.data_ptr())
+        ctx.synchronize()
+        print(f"large FOV: kernel {st['kernel_ms']:.0f} ms, {st['fp64_fallbacks']} FP64 rays, "
+              f"{st['overflow_waves']} overflow waves", flush=True)
+        assert st["rays"] == n_det * n_vox * 1024
+        finite = bool(torch.isfinite(buf).all())
+        rng = np.random.default_rng(11)
+        cand = torch.from_numpy(rng.integers(0, n_det * n_vox, 400000)).to(buf.device)
+        top = cand[torch.topk(buf[cand], 1000).indices]
+        idx = torch.cat([cand[:1000], top])
+        got = buf[idx].cpu().numpy()
+        idx = idx.cpu().numpy()
+        del buf
+    assert finite
+    assert (got[1000:] > 0).all()
+    ref = oracle_mod.Oracle(p).pe_pairs(idx // n_vox, idx % n_vox, sub=sub)
+    assert_parity(got, ref, TOL_FP32, "large FOV PE fp32")
