@@ -91,13 +91,13 @@ __device__ __forceinline__ SpillLists spill_lists(const PeSpill& S, int slot) {
     return L;
 }
 
-// A wave whose candidates overflowed its LDS lists queues its work item
-// (its block index) for the spill kernel of the same launch; without a
-// queue (a launch with no spill area) it is only counted.
-__device__ __forceinline__ void spill_push(const PeLaunch& A, int lane) {
+// A wave whose candidates overflowed its LDS lists queues its work item for
+// the spill kernel of the same launch; without a queue (a launch with no
+// spill area) it is only counted.
+__device__ __forceinline__ void spill_push(const PeLaunch& A, int item, int lane) {
     if (lane == 0) {
         const unsigned long long i = atomicAdd(&A.counters[2], 1ull);
-        if (A.spill.queue) A.spill.queue[i] = (int)blockIdx.x;
+        if (A.spill.queue) A.spill.queue[i] = item;
     }
 }
 
@@ -511,7 +511,8 @@ __device__ __forceinline__ bool pe_forward_body(const PeLaunch& A, int item, Lis
 template <bool kExact>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_forward_kernel(PeLaunch A) {
     __shared__ WaveLists L;
-    if (!pe_forward_body<kExact, false>(A, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
+    const int item = xcd_item((int)blockIdx.x, (int)gridDim.x);
+    if (!pe_forward_body<kExact, false>(A, item, L)) spill_push(A, item, threadIdx.x);
 }
 
 // Point weights at 3 waves per SIMD (the register cap costs no spills here;
@@ -519,7 +520,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe
 template <bool kExact>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_points_kernel(PeLaunch A) {
     __shared__ WaveLists L;
-    if (!pe_forward_body<kExact, true>(A, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
+    const int item = xcd_item((int)blockIdx.x, (int)gridDim.x);
+    if (!pe_forward_body<kExact, true>(A, item, L)) spill_push(A, item, threadIdx.x);
 }
 
 // The work items the kernel above queued (counters[2] of them), redone with
@@ -659,7 +661,7 @@ __device__ __forceinline__ bool pe_tangent_body(const PeLaunch& A, const PeTange
 template <bool kExact>
 __global__ __launch_bounds__(64) void pe_tangent_kernel(PeLaunch A, PeTangent TG) {
     __shared__ WaveLists L;
-    if (!pe_tangent_body<kExact>(A, TG, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
+    if (!pe_tangent_body<kExact>(A, TG, (int)blockIdx.x, L)) spill_push(A, (int)blockIdx.x, threadIdx.x);
 }
 
 template <bool kExact>
@@ -882,7 +884,7 @@ template <bool kExact>
 __global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScratch S) {
     __shared__ WaveLists L;
     __shared__ double cand_acc[kCapBars];
-    if (!pe_backward_body<kExact>(A, S, (int)blockIdx.x, L, cand_acc)) spill_push(A, threadIdx.x);
+    if (!pe_backward_body<kExact>(A, S, (int)blockIdx.x, L, cand_acc)) spill_push(A, (int)blockIdx.x, threadIdx.x);
 }
 
 // Queued work items of pe_backward_kernel redone with lists in global memory
@@ -1037,7 +1039,7 @@ __device__ __forceinline__ bool pe_project_body(const PeLaunch& A, const float* 
 template <bool kExact>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_project_kernel(PeLaunch A, const float* x, double* partial) {
     __shared__ WaveLists L;
-    if (!pe_project_body<kExact>(A, x, partial, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
+    if (!pe_project_body<kExact>(A, x, partial, (int)blockIdx.x, L)) spill_push(A, (int)blockIdx.x, threadIdx.x);
 }
 
 // Back-projection of work item `item` (group of `group` consecutive bars,
@@ -1082,7 +1084,7 @@ template <bool kExact>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_backproject_kernel(
     PeLaunch A, const float* y, int group, double* partial) {
     __shared__ WaveLists L;
-    if (!pe_backproject_body<kExact>(A, y, group, partial, (int)blockIdx.x, L)) spill_push(A, threadIdx.x);
+    if (!pe_backproject_body<kExact>(A, y, group, partial, (int)blockIdx.x, L)) spill_push(A, (int)blockIdx.x, threadIdx.x);
 }
 
 // Queued work items of the projection kernels redone with lists in global

@@ -260,6 +260,20 @@ SMG_FN T exp_neg(double x) {
     }
 }
 
+// Work item of block b in a grid of n blocks.  The dispatcher deals
+// consecutive blocks round-robin over the 8 XCDs of the chip, each with its
+// own L2; this mapping gives XCD x the contiguous item range
+// [x q + min(x, r), ...) (q = n / 8, r = n % 8), in order, so the waves that
+// write neighbouring voxel runs of an output row (x-adjacent tiles) share an
+// L2, which merges their partial-line stores into whole lines before they go
+// to HBM.  A bijection on [0, n): XCD x holds q + (x < r) blocks.
+constexpr int kXcds = 8;
+SMG_FN int xcd_item(int b, int n) {
+    const int x = b % kXcds, local = b / kXcds;
+    const int q = n / kXcds, r = n % kXcds;
+    return x * q + (x < r ? x : r) + local;
+}
+
 // ---- one sub-cell ray ------------------------------------------------------------------
 
 // Per sub-cell quantities shared by every lane (detector-side geometry).

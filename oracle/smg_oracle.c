@@ -647,7 +647,8 @@ static double pe_ray_att(const smg_o_geom* g, const bar_t* bdp, const cull_t* cl
         to_bar_frame(&bk, q, qa);
         double lo[3] = {-0.5 * bk.w, -0.5 * bk.t, -0.5 * bk.h};
         double hi[3] = {0.5 * bk.w, 0.5 * bk.t, 0.5 * bk.h};
-        OPS(31); /* two frame transforms, box, attenuation */
+        OPS(13); /* box, attenuation (the two frame transforms, 18, are
+                  * counted by to_bar_frame) */
         double l = smg_o_box_ray(pa, qa, lo, hi);
         bar_len[i] = l;
         att += l * bk.mut;
@@ -671,10 +672,15 @@ static double pe_element_core(const smg_o_geom* g, int d, int v, int rot, int sx
     if (!(bd.mut > (double)0.01f)) return 0.0; /* PE:1116, PE:1194 */
     double p[3];
     voxel_pos(g, v, rot, p);
+    /* the element set-up (voxel in the bar frame, culling scan) is not
+     * counted: the op count is per sub-cell ray */
+    const int counting = g_count_ops;
+    g_count_ops = 0;
     double ps[3];
     to_bar_frame(&bd, p, ps);
     cull_t cl;
     pe_cull(g, d, p, &cl);
+    g_count_ops = counting;
     const int Lc = g->n_layers;
     double cb = cos(bd.beta), sb = sin(bd.beta);
     double acc = 0.0;
@@ -686,8 +692,9 @@ static double pe_element_core(const smg_o_geom* g, int d, int v, int rot, int sx
                 double qs[3], clo[3], chi[3], q[3];
                 cell_of(&bd, a, bb, c, sx, sy, sz, qs, clo, chi);
                 from_bar_frame(&bd, qs, q);
-                /* sub-cell point 24, distance 10, solid angle 32 */
-                OPS(66);
+                /* sub-cell point 24 (of which the turn into the world frame,
+                 * 9, is counted by from_bar_frame), distance 10, solid angle 32 */
+                OPS(57);
                 double D[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
                 double dsq = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
                 double dd = sqrt(dsq);

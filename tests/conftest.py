@@ -101,6 +101,8 @@ def load_lane_lib():
     ip = np.ctypeslib.ndpointer(np.int32, flags="C")
     dp = np.ctypeslib.ndpointer(np.float64, flags="C")
     up = np.ctypeslib.ndpointer(np.uint32, flags="C")
+    L.lane_xcd_items.restype = None
+    L.lane_xcd_items.argtypes = [C.c_int, ip]
     L.lane_pe_pairs.restype = C.c_long
     L.lane_pe_pairs.argtypes = [fp, fp, fp, fp] + [C.c_int] * 5 + [ip, ip, C.c_int, dp]
     L.lane_narrow_check.restype = C.c_int

@@ -129,6 +129,11 @@ struct FastMaskCands : MaskCands {
 
 extern "C" {
 
+// Work items of the blocks 0 .. n-1 of a PE launch (xcd_item) into out.
+void lane_xcd_items(int n, int* out) {
+    for (int b = 0; b < n; ++b) out[b] = xcd_item(b, n);
+}
+
 // Checks the wave-level candidate selection (bar_tile_may_hit per tile on
 // group boxes and members, then box_meets_frustum per sub-cell)
 // and the tile-level plate-gate skip (plate_gate_clear) on n_waves seeded

@@ -282,3 +282,16 @@ def test_lane_mu_tables_fp32_form(lane_lib, params_mod, power_law, e0):
     rel = np.abs(o32 / o64 - 1)
     print(f"mu(E) FP32 form, E0 {e0}, {n_seg} segments: max rel {rel.max():.2e}", flush=True)
     assert rel.max() < 6e-7
+
+
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 64, 1000, 4093, 4_194_304])
+def test_xcd_items_are_a_permutation(lane_lib, n):
+    """The PE kernels' block -> work-item map (xcd_item) covers every item once,
+    and the blocks the dispatcher deals to one XCD (b = x, x + 8, ...) take
+    consecutive items in order."""
+    got = np.empty(n, np.int32)
+    lane_lib.lane_xcd_items(n, got)
+    assert np.array_equal(np.sort(got), np.arange(n, dtype=np.int32))
+    for x in range(min(8, n)):
+        seq = got[x::8]
+        assert np.all(np.diff(seq) == 1), x

@@ -196,3 +196,25 @@ def test_scatter_op_counter_counts_terms(oracle_mod, params_mod):
     # the cheapest term is the window-rejected one (cell, cone angle, E', tests)
     assert 60 < min(ops0, ops1, ops3)
     assert ops0 < 20000 and ops1 < 20000
+
+
+def test_pe_op_counter_excludes_culling_scan(oracle_mod, params_mod):
+    """The PE op count is per sub-cell ray and leaves out the element's
+    culling scan: bars far behind the detector, which no element culls in,
+    add scan work per element but no operation to any ray."""
+    p = params_mod.config(1)
+    q = params_mod.config(1)
+    n = p.n_det
+    extra = 200
+    q.det[0] = n + extra
+    for i in range(extra):
+        r = 12 * (n + i)
+        q.det[r + 1:r + 13] = q.det[1:13]
+        q.det[r + 1] = 5000.0 + 3.0 * i  # x far off to the side
+        q.det[r + 2] = 9000.0            # y far behind
+    d = np.arange(0, n, 9, dtype=np.int32)
+    v = (np.arange(d.size) * 97) % p.n_vox
+    sub = (2, 2, 2)
+    a = oracle_mod.Oracle(p).pe_ops_per_ray(d, v, sub=sub)
+    b = oracle_mod.Oracle(q).pe_ops_per_ray(d, v, sub=sub)
+    assert a == b and 100 < a < 10000
