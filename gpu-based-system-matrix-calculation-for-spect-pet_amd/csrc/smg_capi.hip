@@ -184,12 +184,14 @@ double now_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-// The PE-family kernels run in FP64 when asked, and whenever a hole is
-// elliptical: only the all-FP64 kernels carry the elliptic-cylinder chord,
-// which keeps its code and registers out of the FP32 kernels.
+// The point-weight, tangent and adjoint kernels run in FP64 when asked, and
+// whenever a hole is elliptical: of their kernels only the all-FP64 ones
+// carry the elliptic-cylinder chord.  The matrix and projection launches have
+// FP32 kernels with the chord (PeLaunch::ell) and use pe_fp64.
 bool pe_exact(const smg_ctx* c, smg_precision prec) {
     return prec == SMG_PREC_FP64 || c->elliptic_holes;
 }
+bool pe_fp64(smg_precision prec) { return prec == SMG_PREC_FP64; }
 
 // Spill area of the PE-family launches (PeSpill): one queue entry per block
 // of the largest launch on this geometry (n_det x tiles blocks for the matrix
@@ -780,11 +782,12 @@ smg_status smg_pe_forward(smg_ctx* c, int rotation, smg_precision prec, float* d
     A.sphi = std::sin(phi);
     for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
     pick_tile(g, A.tile, A.tiles);
+    A.ell = c->elliptic_holes;
     if ((s = ensure_spill(c, A)) != SMG_OK) return s;
     SMG_HIP(c, pe_cell_table(c, A));
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
-    const bool exact = pe_exact(c, prec);
+    const bool exact = pe_fp64(prec);
     if (dst_is_device) {
         SMG_HIP(c, launch_pe_forward(A, exact, c->stream));
         SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -860,11 +863,12 @@ smg_status smg_pe_forward_noalloc(smg_ctx* c, int rotation, smg_precision prec, 
     A.cells = cells;
     A.cells_bar0 = 0;
     A.spill = PeSpill{};  // no spill area: overflowing work items are only counted
+    A.ell = c->elliptic_holes;
     hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, c->stream, counters, 4);
     SMG_HIP(c, hipGetLastError());
     SMG_HIP(c, launch_cell_table(A.geom, 0, g.n_det, A.sub, cells, c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
-    SMG_HIP(c, launch_pe_forward(A, pe_exact(c, prec), c->stream));
+    SMG_HIP(c, launch_pe_forward(A, pe_fp64(prec), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long cnt[4] = {0, 0, 0, 0};
     SMG_HIP(c, hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
@@ -905,6 +909,7 @@ static smg_status project_entry(smg_ctx* c, int rotation, smg_precision prec, bo
     A.sphi = std::sin(phi);
     for (int k = 0; k < 3; ++k) A.sub[k] = c->pe_sub[k];
     A.cell_slices = 1;
+    A.ell = c->elliptic_holes;
     pick_tile(g, A.tile, A.tiles);
     if ((s = ensure_spill(c, A)) != SMG_OK) return s;
     SMG_HIP(c, pe_cell_table(c, A));
@@ -923,9 +928,9 @@ static smg_status project_entry(smg_ctx* c, int rotation, smg_precision prec, bo
     SMG_HIP(c, hipMemsetAsync(c->counters.ptr, 0, 4 * sizeof(unsigned long long), c->stream));
     SMG_HIP(c, hipEventRecord(c->ev0, c->stream));
     if (back)
-        SMG_HIP(c, launch_pe_backproject(A, pe_exact(c, prec), in_d, c->proj.ptr, out_d, c->stream));
+        SMG_HIP(c, launch_pe_backproject(A, pe_fp64(prec), in_d, c->proj.ptr, out_d, c->stream));
     else
-        SMG_HIP(c, launch_pe_project(A, pe_exact(c, prec), in_d, c->proj.ptr, out_d, c->stream));
+        SMG_HIP(c, launch_pe_project(A, pe_fp64(prec), in_d, c->proj.ptr, out_d, c->stream));
     SMG_HIP(c, hipEventRecord(c->ev1, c->stream));
     unsigned long long cnt[4] = {0, 0, 0, 0};
     SMG_HIP(c, hipMemcpyAsync(cnt, c->counters.ptr, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
@@ -1083,8 +1088,6 @@ smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm
     const HostGeom& hg = c->hg;
     const GeomDesc& g = hg.g;
     if (g.n_layers < 1) return fail(c, SMG_ERR_BAD_PARAM, "the collimator has no layer");
-    if (c->elliptic_holes)
-        return fail(c, SMG_ERR_BAD_PARAM, "collimator grid scatterers need circular holes");
     if (layer_mask == 0 || (g.n_layers < 32 && (layer_mask >> g.n_layers) != 0))
         return fail(c, SMG_ERR_BAD_PARAM, "layer mask 0x%x does not select layers of the %d-layer collimator",
                     layer_mask, g.n_layers);

@@ -20,7 +20,7 @@ constexpr int kCapHoles = 256;
 constexpr int kSpillWaves = 1024;
 
 // Work items whose candidates overflowed the LDS lists.  Each PE-family
-// launch clears counters[2], its kernel appends the block index of every
+// launch clears counters[2], its kernel appends the work item of every
 // overflowing wave to queue (room for one entry per block of the launch),
 // and the launch's spill kernel (kSpillWaves waves or fewer) redoes the
 // queued items, wave b with the lists in slot b of slots; cap_bars and
@@ -64,6 +64,10 @@ struct PeLaunch {
     const double* use_rows = nullptr;
     int n_use_rows = 0;
     PeSpill spill;                // lists of the waves that overflow LDS
+    // a hole is elliptical: the FP32 matrix and projection launches run
+    // their kernels with the elliptic-cylinder chord compiled in (the FP64
+    // kernels always carry it; the other FP32 PE launches do not)
+    bool ell = false;
 };
 
 // Matrix-free projections with the PE matrix of A's rotation: y = A x

@@ -86,6 +86,20 @@ inline void make_group_boxes(HostGeom& hg) {
     }, hg.hole_groups);
 }
 
+// True when the point (x, y, z) lies inside hole o (its ellipse or circle,
+// strictly, and within its y span): the collimator grid leaves such points out.
+inline bool in_hole(const Hole<double>& o, double x, double y, double z) {
+    const double dx = x - o.xc, dz = z - o.zc;
+    bool in;
+    if (o.erx > 0.0) {
+        const double u = dx * o.ec + dz * o.es, w = -dx * o.es + dz * o.ec;
+        in = u * u / (o.erx * o.erx) + w * w / (o.erz * o.erz) < 1.0;
+    } else {
+        in = dx * dx + dz * dz < o.r * o.r;
+    }
+    return in && y >= o.y1 && y <= o.y2;
+}
+
 // Scatter points on a grid over the plates of the layers in layer_mask (bit
 // m: layer m; the collimator grid-scatterer mode, an extension with no
 // reference counterpart): per layer, n_k = max(1, round(extent / pitch))
@@ -117,11 +131,7 @@ inline void make_coll_grid(const HostGeom& hg, double pitch, unsigned layer_mask
                     const double x = L.lo[0] + (ix + 0.5) * pk[0], y = L.lo[1] + (jy + 0.5) * pk[1],
                                  z = L.lo[2] + (kz + 0.5) * pk[2];
                     bool inside = false;
-                    for (int h = h0; h < h1 && !inside; ++h) {
-                        const Hole<double>& o = hg.holes_d[h];
-                        const double dx = x - o.xc, dz = z - o.zc;
-                        inside = dx * dx + dz * dz < o.r * o.r && y >= o.y1 && y <= o.y2;
-                    }
+                    for (int h = h0; h < h1 && !inside; ++h) inside = in_hole(hg.holes_d[h], x, y, z);
                     if (!inside) pts.push_back(ScatterPt<double>{x, y, z, w});
                 }
     }

@@ -386,7 +386,7 @@ __device__ bool point_used(const PeLaunch& A, const WaveSetup& W, const double q
 // Work item `item` of the launch (a block index: bar, tile and, for point
 // weights, slice of cells) with candidate lists L.  Returns false, having
 // written nothing, when the candidates overflow L.
-template <bool kExact, bool kPoints, class Lists>
+template <bool kExact, bool kPoints, bool kEll, class Lists>
 __device__ __forceinline__ bool pe_forward_body(const PeLaunch& A, int item, Lists& L) {
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;
@@ -451,12 +451,12 @@ __device__ __forceinline__ bool pe_forward_body(const PeLaunch& A, int item, Lis
                 ListCands<Lists> C{&L, nsb, nsh, lane};
                 if constexpr (kPoints) {
                     double aux[8];
-                    pe_cell_ray<T, kExact>(G, d, W.p, W.ps, cc, C, near, nullptr, aux);
+                    pe_cell_ray<T, kEll>(G, d, W.p, W.ps, cc, C, near, nullptr, aux);
                     if (!near)
                         pts[(int64_t)cell * g.n_vox + W.v] =
                             kExact ? point_weight(bd, n_cells, aux) : point_weight_f(bd, n_cells, aux);
                 } else {
-                    T val = pe_cell_ray<T, kExact>(G, d, W.p, W.ps, cc, C, near);
+                    T val = pe_cell_ray<T, kEll>(G, d, W.p, W.ps, cc, C, near);
                     if (!near) acc.add(val);
                 }
             }
@@ -484,10 +484,10 @@ __device__ __forceinline__ bool pe_forward_body(const PeLaunch& A, int item, Lis
                     bool n2 = false;
                     if constexpr (kPoints) {
                         double aux[8];
-                        pe_cell_ray<double, kExact>(G, d, W.p, W.ps, cc, C, n2, nullptr, aux);
+                        pe_cell_ray<double, kEll>(G, d, W.p, W.ps, cc, C, n2, nullptr, aux);
                         pts[(int64_t)redo_cell[i] * g.n_vox + W.v] = point_weight(bd, n_cells, aux);
                     } else {
-                        acc.add((T)pe_cell_ray<double, kExact>(G, d, W.p, W.ps, cc, C, n2));
+                        acc.add((T)pe_cell_ray<double, kEll>(G, d, W.p, W.ps, cc, C, n2));
                     }
                     ++fallbacks;
                 }
@@ -508,11 +508,13 @@ __device__ __forceinline__ bool pe_forward_body(const PeLaunch& A, int item, Lis
 // 3 waves per SIMD: uncapped, the FP32 allocation lands at 169 registers (2
 // waves per SIMD, 36% slower on the full-size step); the cap costs a 12-byte
 // spill.  The same holds for the projection below.
-template <bool kExact>
+// kEll: elliptical holes (the elliptic-cylinder chord compiled in; the
+// all-FP64 kernels always have it).
+template <bool kExact, bool kEll = kExact>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_forward_kernel(PeLaunch A) {
     __shared__ WaveLists L;
     const int item = xcd_item((int)blockIdx.x, (int)gridDim.x);
-    if (!pe_forward_body<kExact, false>(A, item, L)) spill_push(A, item, threadIdx.x);
+    if (!pe_forward_body<kExact, false, kEll>(A, item, L)) spill_push(A, item, threadIdx.x);
 }
 
 // Point weights at 3 waves per SIMD (the register cap costs no spills here;
@@ -521,19 +523,19 @@ template <bool kExact>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_points_kernel(PeLaunch A) {
     __shared__ WaveLists L;
     const int item = xcd_item((int)blockIdx.x, (int)gridDim.x);
-    if (!pe_forward_body<kExact, true>(A, item, L)) spill_push(A, item, threadIdx.x);
+    if (!pe_forward_body<kExact, true, kExact>(A, item, L)) spill_push(A, item, threadIdx.x);
 }
 
 // The work items the kernel above queued (counters[2] of them), redone with
 // candidate lists in global memory: wave b takes queue entries b, b + grid, ...
 // and its own list slot.  The entry count is wave-uniform, so every wave ends.
-template <bool kExact, bool kPoints>
+template <bool kExact, bool kPoints, bool kEll = kExact>
 __global__ __launch_bounds__(64) void pe_forward_spill_kernel(PeLaunch A) {
     SpillLists L = spill_lists(A.spill, (int)blockIdx.x);
     const unsigned long long n = A.counters[2];
     for (unsigned long long i = blockIdx.x; i < n; i += gridDim.x) {
         __syncthreads();  // the previous item's list reads are done
-        pe_forward_body<kExact, kPoints>(A, A.spill.queue[i], L);
+        pe_forward_body<kExact, kPoints, kEll>(A, A.spill.queue[i], L);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&A.counters[1], n);
 }
@@ -943,7 +945,7 @@ __global__ __launch_bounds__(256) void pe_grad_reduce(PeGradScratch S, int n_det
 // A[d][v] of this lane for bar d and the wave's tile (W set up for them) into
 // a: 0 for a bar below the mu threshold.  Returns false (wave-uniform) when
 // the candidates overflow L; adds the lane's FP64 re-computations to fallbacks.
-template <bool kExact, class Lists>
+template <bool kExact, bool kEll, class Lists>
 __device__ bool pe_element(const PeLaunch& A, int d, int lane, const WaveSetup& W, Lists& L,
                            uint16_t* redo_cell, uint64_t* redo_mask, unsigned& fallbacks, double& a) {
     const PeGeom& G = A.geom;
@@ -970,7 +972,7 @@ __device__ bool pe_element(const PeLaunch& A, int d, int lane, const WaveSetup& 
             bool near = false;
             if (W.valid) {
                 ListCands<Lists> C{&L, nsb, nsh, lane};
-                T val = pe_cell_ray<T, kExact>(G, d, W.p, W.ps, cc, C, near);
+                T val = pe_cell_ray<T, kEll>(G, d, W.p, W.ps, cc, C, near);
                 if (!near) acc.add(val);
             }
             if constexpr (!kExact) {
@@ -995,7 +997,7 @@ __device__ bool pe_element(const PeLaunch& A, int d, int lane, const WaveSetup& 
                 if ((redo_mask[i] >> lane) & 1ull) {
                     ListCands<Lists> C{&L, rsb, rsh, lane};
                     bool n2 = false;
-                    acc.add((T)pe_cell_ray<double, kExact>(G, d, W.p, W.ps, cc, C, n2));
+                    acc.add((T)pe_cell_ray<double, kEll>(G, d, W.p, W.ps, cc, C, n2));
                     ++fallbacks;
                 }
             }
@@ -1016,7 +1018,7 @@ __device__ __forceinline__ void add_fallbacks(const PeLaunch& A, int lane, unsig
 // Forward projection of work item `item` (bar d, voxel tile): the wave's share
 // of y[d] goes to partial[item].  Returns false, having written nothing, when
 // the candidates overflow L.
-template <bool kExact, class Lists>
+template <bool kExact, bool kEll, class Lists>
 __device__ __forceinline__ bool pe_project_body(const PeLaunch& A, const float* x, double* partial, int item,
                                                 Lists& L) {
     __shared__ uint16_t redo_cell[kRedoCap];
@@ -1029,24 +1031,24 @@ __device__ __forceinline__ bool pe_project_body(const PeLaunch& A, const float* 
     wave_setup(A, d, tile, lane, W);
     unsigned fallbacks = 0;
     double a;
-    if (!pe_element<kExact>(A, d, lane, W, L, redo_cell, redo_mask, fallbacks, a)) return false;
+    if (!pe_element<kExact, kEll>(A, d, lane, W, L, redo_cell, redo_mask, fallbacks, a)) return false;
     const double s = wave_sum(W.valid ? a * (double)x[W.v] : 0.0);
     if (lane == 0) partial[item] = s;
     if constexpr (!kExact) add_fallbacks(A, lane, fallbacks);
     return true;
 }
 
-template <bool kExact>
+template <bool kExact, bool kEll = kExact>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_project_kernel(PeLaunch A, const float* x, double* partial) {
     __shared__ WaveLists L;
-    if (!pe_project_body<kExact>(A, x, partial, (int)blockIdx.x, L)) spill_push(A, (int)blockIdx.x, threadIdx.x);
+    if (!pe_project_body<kExact, kEll>(A, x, partial, (int)blockIdx.x, L)) spill_push(A, (int)blockIdx.x, threadIdx.x);
 }
 
 // Back-projection of work item `item` (group of `group` consecutive bars,
 // voxel tile), looping over the group's bars (those with y[d] = 0 skipped);
 // each lane's share of x[v] goes to partial[group][v].  Returns false, having
 // written nothing, when the candidates of one of the bars overflow L.
-template <bool kExact, class Lists>
+template <bool kExact, bool kEll, class Lists>
 __device__ __forceinline__ bool pe_backproject_body(const PeLaunch& A, const float* y, int group,
                                                     double* partial, int item, Lists& L) {
     __shared__ uint16_t redo_cell[kRedoCap];
@@ -1065,7 +1067,7 @@ __device__ __forceinline__ bool pe_backproject_body(const PeLaunch& A, const flo
         WaveSetup W;
         wave_setup(A, d, tile, lane, W);
         double a;
-        if (!pe_element<kExact>(A, d, lane, W, L, redo_cell, redo_mask, fallbacks, a)) return false;
+        if (!pe_element<kExact, kEll>(A, d, lane, W, L, redo_cell, redo_mask, fallbacks, a)) return false;
         acc += a * (double)yd;
     }
     // the lane's voxel (it does not depend on the bar)
@@ -1080,16 +1082,16 @@ __device__ __forceinline__ bool pe_backproject_body(const PeLaunch& A, const flo
     return true;
 }
 
-template <bool kExact>
+template <bool kExact, bool kEll = kExact>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_backproject_kernel(
     PeLaunch A, const float* y, int group, double* partial) {
     __shared__ WaveLists L;
-    if (!pe_backproject_body<kExact>(A, y, group, partial, (int)blockIdx.x, L)) spill_push(A, (int)blockIdx.x, threadIdx.x);
+    if (!pe_backproject_body<kExact, kEll>(A, y, group, partial, (int)blockIdx.x, L)) spill_push(A, (int)blockIdx.x, threadIdx.x);
 }
 
 // Queued work items of the projection kernels redone with lists in global
 // memory (as pe_forward_spill_kernel).
-template <bool kExact, bool kBack>
+template <bool kExact, bool kBack, bool kEll = kExact>
 __global__ __launch_bounds__(64) void pe_project_spill_kernel(PeLaunch A, const float* in, int group,
                                                               double* partial) {
     SpillLists L = spill_lists(A.spill, (int)blockIdx.x);
@@ -1097,9 +1099,9 @@ __global__ __launch_bounds__(64) void pe_project_spill_kernel(PeLaunch A, const 
     for (unsigned long long i = blockIdx.x; i < n; i += gridDim.x) {
         __syncthreads();
         if constexpr (kBack)
-            pe_backproject_body<kExact>(A, in, group, partial, A.spill.queue[i], L);
+            pe_backproject_body<kExact, kEll>(A, in, group, partial, A.spill.queue[i], L);
         else
-            pe_project_body<kExact>(A, in, partial, A.spill.queue[i], L);
+            pe_project_body<kExact, kEll>(A, in, partial, A.spill.queue[i], L);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&A.counters[1], n);
 }
@@ -1220,6 +1222,10 @@ hipError_t launch_pe_forward_rows(const PeLaunch& A, bool exact, int bar0, int n
     if (exact) {
         hipLaunchKernelGGL(pe_forward_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, B);
         if (sg) hipLaunchKernelGGL((pe_forward_spill_kernel<true, false>), dim3(sg), dim3(64), 0, stream, B);
+    } else if (A.ell) {
+        hipLaunchKernelGGL((pe_forward_kernel<false, true>), dim3((unsigned)blocks), dim3(64), 0, stream, B);
+        if (sg)
+            hipLaunchKernelGGL((pe_forward_spill_kernel<false, false, true>), dim3(sg), dim3(64), 0, stream, B);
     } else {
         hipLaunchKernelGGL(pe_forward_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, B);
         if (sg) hipLaunchKernelGGL((pe_forward_spill_kernel<false, false>), dim3(sg), dim3(64), 0, stream, B);
@@ -1280,6 +1286,12 @@ hipError_t launch_pe_project(const PeLaunch& A, bool exact, const float* x, doub
         hipLaunchKernelGGL(pe_project_kernel<true>, dim3((unsigned)blocks), dim3(64), 0, stream, A, x, partial);
         if (sg) hipLaunchKernelGGL((pe_project_spill_kernel<true, false>), dim3(sg), dim3(64), 0, stream, A, x, 0,
                            partial);
+    } else if (A.ell) {
+        hipLaunchKernelGGL((pe_project_kernel<false, true>), dim3((unsigned)blocks), dim3(64), 0, stream, A, x,
+                           partial);
+        if (sg)
+            hipLaunchKernelGGL((pe_project_spill_kernel<false, false, true>), dim3(sg), dim3(64), 0, stream, A, x,
+                               0, partial);
     } else {
         hipLaunchKernelGGL(pe_project_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A, x, partial);
         if (sg) hipLaunchKernelGGL((pe_project_spill_kernel<false, false>), dim3(sg), dim3(64), 0, stream, A, x, 0,
@@ -1309,6 +1321,12 @@ hipError_t launch_pe_backproject(const PeLaunch& A, bool exact, const float* y, 
                            partial);
         if (sg) hipLaunchKernelGGL((pe_project_spill_kernel<true, true>), dim3(sg), dim3(64), 0, stream, A, y, grp,
                            partial);
+    } else if (A.ell) {
+        hipLaunchKernelGGL((pe_backproject_kernel<false, true>), dim3((unsigned)blocks), dim3(64), 0, stream, A, y,
+                           grp, partial);
+        if (sg)
+            hipLaunchKernelGGL((pe_project_spill_kernel<false, true, true>), dim3(sg), dim3(64), 0, stream, A, y,
+                               grp, partial);
     } else {
         hipLaunchKernelGGL(pe_backproject_kernel<false>, dim3((unsigned)blocks), dim3(64), 0, stream, A, y, grp,
                            partial);

@@ -262,16 +262,21 @@ SMG_FN T exp_neg(double x) {
 
 // Work item of block b in a grid of n blocks.  The dispatcher deals
 // consecutive blocks round-robin over the 8 XCDs of the chip, each with its
-// own L2; this mapping gives XCD x the contiguous item range
-// [x q + min(x, r), ...) (q = n / 8, r = n % 8), in order, so the waves that
-// write neighbouring voxel runs of an output row (x-adjacent tiles) share an
-// L2, which merges their partial-line stores into whole lines before they go
-// to HBM.  A bijection on [0, n): XCD x holds q + (x < r) blocks.
+// own L2.  Here the items go out in chunks of kXcdChunk consecutive items,
+// chunk j to XCD j % 8 (its blocks b = j % 8 + 8 (kXcdChunk (j / 8) + o),
+// o < kXcdChunk, in order): the waves that write neighbouring voxel runs of
+// an output row (x-adjacent tiles, 16 B each) share an L2, which merges their
+// partial-line stores into whole 128 B lines before they go to HBM, while
+// every XCD still draws its chunks from all rows (the work per row varies
+// with the bar's place in the detector).  The last n % (8 kXcdChunk) blocks
+// keep their own index.  A bijection on [0, n).
 constexpr int kXcds = 8;
+constexpr int kXcdChunk = 8;
 SMG_FN int xcd_item(int b, int n) {
+    const int span = kXcds * kXcdChunk;
+    if (b >= n / span * span) return b;
     const int x = b % kXcds, local = b / kXcds;
-    const int q = n / kXcds, r = n % kXcds;
-    return x * q + (x < r ? x : r) + local;
+    return ((local / kXcdChunk) * kXcds + x) * kXcdChunk + local % kXcdChunk;
 }
 
 // ---- one sub-cell ray ------------------------------------------------------------------

@@ -520,8 +520,8 @@ SMG_FN double cyl_chord(const double a[3], const CylRay& c, double len, double x
 // ellipse's, a = kx'^2 erz^2 + kz'^2 erx^2, b = kx' x0' erz^2 + kz' z0' erx^2,
 // with the cut b^2 - a c <= 1e-5 (PESysMatGen.cu:887-893), formed as
 // erx^2 erz^2 (a - (kx' z0' - kz' x0')^2), the value of b^2 - a c without its
-// cancellation.  FP64 only: the FP32 ray loop hands rays that meet an
-// elliptical hole to the FP64 re-computation, so this code stays out of it.
+// cancellation.  FP64, as the circle's chord, in every kernel that has it
+// compiled in (the FP32 kernels get a variant with it for elliptical holes).
 SMG_FN double ell_chord(const Hole<double>& o, double ax, double ay, double az, double kx_,
                         double kz_, double inv_ky, double len) {
     if (fabs(o.y1 - o.y2) < kCylDy) return 0.0;

@@ -55,8 +55,10 @@ SMG_FN double seg_box_geo(const double a[3], const double b[3], const double lo[
     return (t1 - t0) * sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-SMG_FN double seg_cyl_geo(const double a[3], const double b[3], double xc, double y1, double y2,
-                          double zc, double r) {
+// Fraction of the segment a -> b (its parameter range in [0, 1]) inside the
+// cylinder of radius r about the line x = xc, z = zc, for y1 <= y <= y2.
+SMG_FN double seg_cyl_frac(const double a[3], const double b[3], double xc, double y1, double y2,
+                           double zc, double r) {
     const double D[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
     double t0 = 0.0, t1 = 1.0;
     if (D[1] == 0.0) {
@@ -83,8 +85,29 @@ SMG_FN double seg_cyl_geo(const double a[3], const double b[3], double xc, doubl
         t0 = tmax2(t0, (-B - sq) / A);
         t1 = tmin2(t1, (-B + sq) / A);
     }
-    if (!(t1 > t0)) return 0.0;
-    return (t1 - t0) * sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+    return t1 > t0 ? t1 - t0 : 0.0;
+}
+
+SMG_FN double seg_cyl_geo(const double a[3], const double b[3], double xc, double y1, double y2,
+                          double zc, double r) {
+    const double f = seg_cyl_frac(a, b, xc, y1, y2, zc, r);
+    if (!(f > 0.0)) return 0.0;
+    const double D[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    return f * sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+}
+
+// Elliptical hole o (semi-axes erx, erz along x', z', the (x, z) offsets from
+// the axis turned by theta): in the coordinates x'/erx, z'/erz its section is
+// the unit circle.  That map is linear, so a segment's parameter range inside
+// is the mapped segment's range inside the unit cylinder.
+SMG_FN double seg_ell_geo(const double a[3], const double b[3], const Hole<double>& o) {
+    const double ax = a[0] - o.xc, az = a[2] - o.zc, bx = b[0] - o.xc, bz = b[2] - o.zc;
+    const double am[3] = {(ax * o.ec + az * o.es) / o.erx, a[1], (-ax * o.es + az * o.ec) / o.erz};
+    const double bm[3] = {(bx * o.ec + bz * o.es) / o.erx, b[1], (-bx * o.es + bz * o.ec) / o.erz};
+    const double f = seg_cyl_frac(am, bm, 0.0, o.y1, o.y2, 0.0, 1.0);
+    if (!(f > 0.0)) return 0.0;
+    const double D[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    return f * sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
 }
 
 // Collimator material along a -> b at E0: each layer's plate chord less the
@@ -102,7 +125,7 @@ SMG_FN void coll_material(const PeGeom& G, const double a[3], const double b[3],
         const int first = G.g.ranges[m].first, last = first + G.g.ranges[m].count;
         for (int h = first; h < last; ++h) {
             const Hole<double>& o = G.holes_d[h];
-            const double l = seg_cyl_geo(a, b, o.xc, o.y1, o.y2, o.zc, o.r);
+            const double l = o.erx > 0.0 ? seg_ell_geo(a, b, o) : seg_cyl_geo(a, b, o.xc, o.y1, o.y2, o.zc, o.r);
             holes += l;
             att_t += o.mu * l;
             s_pe += o.mupe * l;

@@ -97,6 +97,8 @@ def lib():
     L.smg_o_seg_box.argtypes = [_dp, _dp, _dp, _dp]
     L.smg_o_seg_cyl.restype = C.c_double
     L.smg_o_seg_cyl.argtypes = [_dp, _dp] + [C.c_double] * 5
+    L.smg_o_seg_ell.restype = C.c_double
+    L.smg_o_seg_ell.argtypes = [_dp, _dp] + [C.c_double] * 7
     L.smg_o_coll_grid_points.restype = C.c_int64
     L.smg_o_coll_grid_points.argtypes = [C.c_void_p, C.c_double, C.c_uint, C.c_void_p, C.c_int64]
     L.smg_o_coll_grid_weight.restype = C.c_double
@@ -154,6 +156,11 @@ def seg_box(a, b, lo, hi) -> float:
 def seg_cyl(a, b, xc, y1, y2, zc, r) -> float:
     """Length of segment a -> b inside the finite cylinder along y."""
     return lib().smg_o_seg_cyl(_v3(a), _v3(b), xc, y1, y2, zc, r)
+
+
+def seg_ell(a, b, xc, y1, y2, zc, x_rho, z_rho, theta) -> float:
+    """Length of segment a -> b inside the finite elliptic cylinder along y."""
+    return lib().smg_o_seg_ell(_v3(a), _v3(b), xc, y1, y2, zc, x_rho, z_rho, theta)
 
 
 def kn(theta, e0) -> float:

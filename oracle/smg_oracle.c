@@ -1020,6 +1020,64 @@ double smg_o_seg_cyl(const double a[3], const double b[3], double xc, double y1,
     return t1 > t0 ? (t1 - t0) * dist3(a, b) : 0.0;
 }
 
+/* Length of the segment a -> b inside the elliptic cylinder of hole shape
+ * (x_rho, z_rho, theta) about the line x = xc, z = zc, for y1 <= y <= y2: the
+ * point's offsets from the axis turned as in smg_o_ellcyl_ray,
+ * x' = dx cos(theta) + dz sin(theta), z' = -dx sin(theta) + dz cos(theta),
+ * lie in x'^2 / x_rho^2 + z'^2 / z_rho^2 <= 1.  Along the segment x', z' are
+ * linear in the parameter t, so the condition is the quadratic
+ * q2 t^2 + 2 q1 t + q0 <= 0 solved here directly. */
+double smg_o_seg_ell(const double a[3], const double b[3], double xc, double y1, double y2,
+                     double zc, double x_rho, double z_rho, double theta) {
+    OPS(50); /* turn of the start and direction, y slab, quadratic, sqrt, roots, length */
+    double cs = cos(theta), sn = sin(theta);
+    double D[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double t0 = 0.0, t1 = 1.0;
+    if (D[1] == 0.0) {
+        if (a[1] < y1 || a[1] > y2) return 0.0;
+    } else {
+        double u = (y1 - a[1]) / D[1], w = (y2 - a[1]) / D[1];
+        if (u > w) { double x = u; u = w; w = x; }
+        if (u > t0) t0 = u;
+        if (w < t1) t1 = w;
+    }
+    double dx = a[0] - xc, dz = a[2] - zc;
+    double px = dx * cs + dz * sn, pz = -dx * sn + dz * cs;       /* start, turned */
+    double vx = D[0] * cs + D[2] * sn, vz = -D[0] * sn + D[2] * cs; /* direction, turned */
+    double ix = 1.0 / (x_rho * x_rho), iz = 1.0 / (z_rho * z_rho);
+    double q2 = vx * vx * ix + vz * vz * iz;
+    double q1 = px * vx * ix + pz * vz * iz;
+    double q0 = px * px * ix + pz * pz * iz - 1.0;
+    if (q2 == 0.0) {
+        if (q0 > 0.0) return 0.0;
+    } else {
+        double disc = q1 * q1 - q2 * q0;
+        if (disc <= 0.0) return 0.0;
+        double sq = sqrt(disc);
+        double u = (-q1 - sq) / q2, w = (-q1 + sq) / q2;
+        if (u > t0) t0 = u;
+        if (w < t1) t1 = w;
+    }
+    return t1 > t0 ? (t1 - t0) * dist3(a, b) : 0.0;
+}
+
+/* Whether grid point (x, y, z) lies inside hole h (strictly inside its
+ * circle or ellipse, and within its y span). */
+static int in_hole(const smg_o_geom* g, int h, double x, double y, double z) {
+    const float* hp = g->coll + 100 + 9 * h;
+    double dx = x - hp[0], dz = z - hp[3];
+    double exr, ezr, eth;
+    int in;
+    if (hole_ellipse(g, h, &exr, &ezr, &eth)) {
+        double cs = cos(eth), sn = sin(eth);
+        double u = dx * cs + dz * sn, w = -dx * sn + dz * cs;
+        in = u * u / (exr * exr) + w * w / (ezr * ezr) < 1.0;
+    } else {
+        in = dx * dx + dz * dz < (double)hp[4] * hp[4];
+    }
+    return in && y >= (double)hp[1] + g->F && y <= (double)hp[2] + g->F;
+}
+
 /* Collimator material along the segment a -> b at E0: for each layer whose
  * plate box the segment meets, its plate chord less the chords of its holes
  * (at least 0) with the layer's mu, plus the holes' own mu.  att_t = sum mu_total l; s_pe, s_c the sums with
@@ -1040,7 +1098,12 @@ static void coll_material(const smg_o_geom* g, const double a[3], const double b
         double holes = 0.0;
         for (int i = 0; i < g->holes_in_layer[m]; ++i) {
             const float* hp = g->coll + 100 + 9 * (h0 + i);
-            double l = smg_o_seg_cyl(a, b, hp[0], (double)hp[1] + g->F, (double)hp[2] + g->F, hp[3], hp[4]);
+            double exr, ezr, eth;
+            double l = hole_ellipse(g, h0 + i, &exr, &ezr, &eth)
+                           ? smg_o_seg_ell(a, b, hp[0], (double)hp[1] + g->F, (double)hp[2] + g->F, hp[3],
+                                           exr, ezr, eth)
+                           : smg_o_seg_cyl(a, b, hp[0], (double)hp[1] + g->F, (double)hp[2] + g->F, hp[3],
+                                           hp[4]);
             holes += l;
             *att_t += (double)hp[5] * l;
             *s_pe += (double)hp[6] * l;
@@ -1084,12 +1147,8 @@ int64_t smg_o_coll_grid_points(const smg_o_geom* g, double pitch, unsigned layer
                     double x = lo[0] + (ix + 0.5) * pk[0], y = lo[1] + (jy + 0.5) * pk[1],
                            z = lo[2] + (kz + 0.5) * pk[2];
                     int inside = 0;
-                    for (int h = first; h < first + g->holes_in_layer[m] && !inside; ++h) {
-                        const float* hp = g->coll + 100 + 9 * h;
-                        double dx = x - hp[0], dz = z - hp[3];
-                        inside = dx * dx + dz * dz < (double)hp[4] * hp[4] &&
-                                 y >= (double)hp[1] + g->F && y <= (double)hp[2] + g->F;
-                    }
+                    for (int h = first; h < first + g->holes_in_layer[m] && !inside; ++h)
+                        inside = in_hole(g, h, x, y, z);
                     if (inside) continue;
                     if (xyzw && cnt < max) {
                         xyzw[4 * cnt] = x;

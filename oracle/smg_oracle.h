@@ -130,6 +130,8 @@ void smg_o_collimator_scatter_pairs(const smg_o_geom* g, const uint32_t* bitmap,
 double smg_o_seg_box(const double a[3], const double b[3], const double lo[3], const double hi[3]);
 double smg_o_seg_cyl(const double a[3], const double b[3], double xc, double y1, double y2,
                      double zc, double r);
+double smg_o_seg_ell(const double a[3], const double b[3], double xc, double y1, double y2,
+                     double zc, double x_rho, double z_rho, double theta);
 int64_t smg_o_coll_grid_points(const smg_o_geom* g, double pitch, unsigned layer_mask, double* xyzw,
                                int64_t max);
 double smg_o_coll_grid_weight(const smg_o_geom* g, const double p[3], const double s[3], double w);

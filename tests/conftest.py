@@ -101,6 +101,8 @@ def load_lane_lib():
     ip = np.ctypeslib.ndpointer(np.int32, flags="C")
     dp = np.ctypeslib.ndpointer(np.float64, flags="C")
     up = np.ctypeslib.ndpointer(np.uint32, flags="C")
+    L.lane_seg_ell.restype = C.c_double
+    L.lane_seg_ell.argtypes = [dp, dp] + [C.c_double] * 7
     L.lane_xcd_items.restype = None
     L.lane_xcd_items.argtypes = [C.c_int, ip]
     L.lane_pe_pairs.restype = C.c_long

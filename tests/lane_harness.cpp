@@ -58,7 +58,7 @@ bool make_ctx(const float* coll, const float* det, const float* img, const float
     return true;
 }
 
-template <bool kExact>
+template <bool kExact, bool kEll>
 double element(const Ctx& c, int d, int v, int rot, int sx, int sy, int sz, long* fallbacks) {
     const PeGeom& G = c.G;
     const GeomDesc& g = G.g;
@@ -88,10 +88,10 @@ double element(const Ctx& c, int d, int v, int rot, int sx, int sy, int sz, long
                 CellCtx cc;
                 make_cell_ctx(bd, a, b, cz, sx, sy, sz, cc);
                 bool near = false;
-                T val = pe_cell_ray<T, kExact>(G, d, p, ps, cc, C, near);
+                T val = pe_cell_ray<T, kEll>(G, d, p, ps, cc, C, near);
                 if (!kExact && near) {
                     bool n2 = false;
-                    val = (T)pe_cell_ray<double, kExact>(G, d, p, ps, cc, C, n2);
+                    val = (T)pe_cell_ray<double, kEll>(G, d, p, ps, cc, C, n2);
                     ++*fallbacks;
                 }
                 acc.add(val);
@@ -128,6 +128,24 @@ struct FastMaskCands : MaskCands {
 }  // namespace
 
 extern "C" {
+
+// Length of the segment a -> b inside the elliptic cylinder (seg_ell_geo, the
+// grid scatterers' collimator material), for a hole of semi-axes erx, erz
+// turned by theta.
+double lane_seg_ell(const double* a, const double* b, double xc, double y1, double y2, double zc,
+                    double erx, double erz, double theta) {
+    Hole<double> o{};
+    o.xc = xc;
+    o.y1 = y1;
+    o.y2 = y2;
+    o.zc = zc;
+    o.erx = erx;
+    o.erz = erz;
+    o.ec = std::cos(theta);
+    o.es = std::sin(theta);
+    o.r = std::max(erx, erz);
+    return seg_ell_geo(a, b, o);
+}
 
 // Work items of the blocks 0 .. n-1 of a PE launch (xcd_item) into out.
 void lane_xcd_items(int n, int* out) {
@@ -335,12 +353,15 @@ long lane_pe_pairs(const float* coll, const float* det, const float* img, const 
                    double* out) {
     Ctx c;
     if (!make_ctx(coll, det, img, phys, c)) return -1;
-    // as the library does: elliptical holes run the all-FP64 path
-    for (const Hole<double>& o : c.hg.holes_d) exact |= o.erx > 0.0;
+    // as the library's matrix kernels do: the elliptic chord is compiled in
+    // while a hole is elliptical
+    bool ell = false;
+    for (const Hole<double>& o : c.hg.holes_d) ell |= o.erx > 0.0;
     long fb = 0;
     for (int i = 0; i < n; ++i)
-        out[i] = exact ? element<true>(c, d[i], v[i], rot, sx, sy, sz, &fb)
-                       : element<false>(c, d[i], v[i], rot, sx, sy, sz, &fb);
+        out[i] = exact ? element<true, true>(c, d[i], v[i], rot, sx, sy, sz, &fb)
+                 : ell ? element<false, true>(c, d[i], v[i], rot, sx, sy, sz, &fb)
+                       : element<false, false>(c, d[i], v[i], rot, sx, sy, sz, &fb);
     return fb;
 }
 

@@ -284,14 +284,36 @@ def test_lane_mu_tables_fp32_form(lane_lib, params_mod, power_law, e0):
     assert rel.max() < 6e-7
 
 
-@pytest.mark.parametrize("n", [1, 7, 8, 9, 64, 1000, 4093, 4_194_304])
+@pytest.mark.parametrize("n", [1, 7, 8, 63, 64, 65, 1000, 4093, 4_194_304])
 def test_xcd_items_are_a_permutation(lane_lib, n):
-    """The PE kernels' block -> work-item map (xcd_item) covers every item once,
-    and the blocks the dispatcher deals to one XCD (b = x, x + 8, ...) take
-    consecutive items in order."""
+    """The PE kernels' block -> work-item map (xcd_item) covers every item once;
+    the blocks the dispatcher deals to one XCD (b = x, x + 8, ...) take whole
+    chunks of 8 consecutive items, chunk j on XCD j % 8, so every XCD draws
+    from all rows; the tail beyond the last whole round of chunks is left in
+    place."""
     got = np.empty(n, np.int32)
     lane_lib.lane_xcd_items(n, got)
     assert np.array_equal(np.sort(got), np.arange(n, dtype=np.int32))
-    for x in range(min(8, n)):
-        seq = got[x::8]
-        assert np.all(np.diff(seq) == 1), x
+    full = n // 64 * 64
+    assert np.array_equal(got[full:], np.arange(full, n, dtype=np.int32))
+    for x in range(8):
+        seq = got[x:full:8].reshape(-1, 8)
+        assert np.all(np.diff(seq, axis=1) == 1), x
+        assert np.all(seq[:, 0] % 8 == 0) and np.all((seq[:, 0] // 8) % 8 == x), x
+
+
+def test_seg_ell_geo_matches_oracle(lane_lib, oracle_mod):
+    """The kernels' elliptic-cylinder segment length (seg_ell_geo: the unit
+    circle after scaling the turned offsets) against the oracle's direct
+    quadratic in the turned coordinates, itself pinned by sampling."""
+    rng = np.random.default_rng(31)
+    for _ in range(300):
+        xc, zc = rng.uniform(-1, 1, 2)
+        xr, zr = rng.uniform(0.3, 1.5, 2)
+        th = rng.uniform(-np.pi, np.pi)
+        a = rng.uniform(-3, 3, 3)
+        b = a + rng.normal(0, 1, 3) * np.array([1.0, 4.0, 1.0])
+        got = lane_lib.lane_seg_ell(np.ascontiguousarray(a), np.ascontiguousarray(b), xc, -1.0, 2.0, zc,
+                                    xr, zr, th)
+        want = oracle_mod.seg_ell(a, b, xc, -1.0, 2.0, zc, xr, zr, th)
+        assert got == pytest.approx(want, rel=1e-10, abs=1e-12)

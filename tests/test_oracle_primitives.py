@@ -417,6 +417,73 @@ def test_seg_box_and_seg_cyl_against_sampling(oracle_mod):
     assert oracle_mod.seg_cyl([xc, -5.0, zc], [xc, 1.0, zc], xc, y1, y2, zc, r) == pytest.approx(2.0)
 
 
+def test_seg_ell_against_sampling(oracle_mod):
+    """The elliptic-cylinder segment length (grid scatterers behind elliptical
+    holes) against dense sampling of random segments through a turned
+    ellipse; a circle's semi-axes give the cylinder's length at any turn."""
+    rng = np.random.default_rng(18)
+    xc, zc, y1, y2 = 0.4, -0.3, -1.0, 2.0
+    for xr, zr, th in ((0.9, 0.35, 0.0), (0.5, 1.1, 0.7), (0.3, 0.8, -2.2)):
+        def inside(x, xr=xr, zr=zr, th=th):
+            dx, dz = x[:, 0] - xc, x[:, 2] - zc
+            u = dx * np.cos(th) + dz * np.sin(th)
+            w = -dx * np.sin(th) + dz * np.cos(th)
+            return (u * u / xr ** 2 + w * w / zr ** 2 <= 1.0) & (x[:, 1] >= y1) & (x[:, 1] <= y2)
+        for _ in range(30):
+            a = rng.uniform(-2, 2, 3)
+            b = a + rng.normal(0, 1, 3) * np.array([0.8, 3.0, 0.8])
+            ref = _sampled_length(a, b, inside)
+            got = oracle_mod.seg_ell(a, b, xc, y1, y2, zc, xr, zr, th)
+            assert got == pytest.approx(ref, abs=2e-4 * np.linalg.norm(b - a)), (xr, zr, th)
+    for _ in range(20):
+        a = rng.uniform(-2, 2, 3)
+        b = a + rng.normal(0, 1, 3) * np.array([0.6, 3.0, 0.6])
+        want = oracle_mod.seg_cyl(a, b, xc, y1, y2, zc, 0.7)
+        assert oracle_mod.seg_ell(a, b, xc, y1, y2, zc, 0.7, 0.7, rng.uniform(-3, 3)) == pytest.approx(
+            want, rel=1e-12, abs=1e-12)
+
+
+def test_coll_grid_points_leave_out_elliptical_holes(oracle_mod, params_mod):
+    """Grid scatterers behind elliptical holes: no grid point lies inside a
+    turned ellipse (within the hole's y span), the points between an ellipse
+    and its bounding circle are kept, and every other cell centre of the plate
+    is a point."""
+    p = params_mod.config(1)
+    o = oracle_mod.Oracle(p)
+    n_h = p.n_holes_layer0
+    shapes = np.zeros((n_h, 3), np.float32)
+    rng = np.random.default_rng(23)
+    shapes[:, 0] = rng.uniform(0.6, 1.4, n_h)
+    shapes[:, 1] = rng.uniform(0.3, 0.7, n_h)
+    shapes[:, 2] = rng.uniform(-np.pi, np.pi, n_h)
+    o.set_hole_shapes(shapes)
+    pitch = 0.25
+    pts, _ = o.coll_grid_points(pitch)
+    w, t, h = (float(p.coll[k]) for k in (11, 12, 13))
+    F = float(p.img[11])
+    xs = -w / 2 + (np.arange(round(w / pitch)) + 0.5) * pitch
+    ys = -t / 2 + F + float(p.coll[14]) + (np.arange(round(t / pitch)) + 0.5) * pitch
+    zs = -h / 2 + (np.arange(round(h / pitch)) + 0.5) * pitch
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    allp = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    inside = np.zeros(len(allp), bool)
+    ring = np.zeros(len(allp), bool)
+    for i in range(n_h):
+        hp = p.coll[100 + 9 * i:100 + 9 * i + 5].astype(np.float64)
+        xr, zr, th = (float(v) for v in shapes[i])
+        dx, dz = allp[:, 0] - hp[0], allp[:, 2] - hp[3]
+        u = dx * np.cos(th) + dz * np.sin(th)
+        v = -dx * np.sin(th) + dz * np.cos(th)
+        iny = (allp[:, 1] >= hp[1] + F) & (allp[:, 1] <= hp[2] + F)
+        ins = (u * u / (xr * xr) + v * v / (zr * zr) < 1.0) & iny
+        inside |= ins
+        ring |= ~ins & iny & (dx * dx + dz * dz < max(xr, zr) ** 2)
+    assert ring.sum() > 100
+    keep = allp[~inside]
+    assert len(pts) == len(keep)
+    assert np.allclose(pts, keep, rtol=0, atol=1e-9)
+
+
 def test_coll_grid_points_and_weight(oracle_mod, params_mod):
     """Grid over the layer-0 plate: the cells tile the plate less the holes,
     and the first-leg weight is dv mu_c / (4 pi L^2) exp(-mu_t l_W) with l_W
