@@ -473,7 +473,8 @@ SMG_FN bool fast_faces(const PeGeom& G, int k, const double q[3], const double t
         dq += eq * eq;
         dt += et * et;
     }
-    return b.sb == 0.0 && b.cb == 1.0 && dq > 1e-4 && sqrt(dt) > rho + 0.01;
+    // distances compared squared: |q - box| > 0.01, |tc - box| > rho + 0.01
+    return b.sb == 0.0 && b.cb == 1.0 && dq > 1e-4 && dt > (rho + 0.01) * (rho + 0.01);
 }
 
 // Chord of the ray through a qualifying bar from its faces relative to q

@@ -461,6 +461,13 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
         const float ab = ax * bx + ay * by + az * bz;
         const float nn = (ax * ax + ay * ay + az * az) * (bx * bx + by * by + bz * bz);
         if (ab < float(ph.c_rej) * tsqrt(nn)) return T(0);
+        // mu(E) tables without an FP32 form (more segments in range than it
+        // holds): the whole term goes to the FP64 path, which keeps FP64
+        // logarithms and exponentials out of the FP32 term
+        if (ph.tab && ph.tab->n > 0 && !(ph.tabf && ph.tabf->n > 0)) {
+            near = true;
+            return T(0);
+        }
     }
     const double alpha = ph.e0 / 511.0;
     double omc, st;  // 1 - cos(theta), sin(theta)
@@ -645,32 +652,36 @@ SMG_FN T scatter_term(const ScatterPhys& ph, const double p[3], const double ps[
 
     // second leg and absorption at E' (scatter.cu:1371-1427); the FP32 path
     // takes E0/E' = 1 + alpha (1 - cos) for the power laws
+    // (FP32: tables here always have their FP32 form, see the entry test)
     double e3, e1;
     const bool tables = ph.tab && ph.tab->n > 0;
-    const bool tables_f = std::is_same<T, float>::value && tables && ph.tabf && ph.tabf->n > 0;
     float Lf = 0.0f;
     int jf = 0;
-    if (std::is_same<T, float>::value && !tables) {
-        e1 = 1.0 + alpha * omc;
-        e3 = e1 * e1 * e1;
-    } else if (tables_f) {
-        Lf = logf(float(1.0 + alpha * omc));  // ln(E0/E')
-        jf = mu_seg_f(*ph.tabf, Lf);
-        e3 = mu_ratio_f(*ph.tabf, 0, jf, Lf);
-        e1 = mu_ratio_f(*ph.tabf, 1, jf, Lf);
+    if constexpr (std::is_same<T, float>::value) {
+        if (!tables) {
+            e1 = 1.0 + alpha * omc;
+            e3 = e1 * e1 * e1;
+        } else {
+            Lf = logf(float(1.0 + alpha * omc));  // ln(E0/E')
+            jf = mu_seg_f(*ph.tabf, Lf);
+            e3 = mu_ratio_f(*ph.tabf, 0, jf, Lf);
+            e1 = mu_ratio_f(*ph.tabf, 1, jf, Lf);
+        }
     } else {
         mu_ratios(ph.tab, ph.e0, ep, e3, e1);
     }
     double att2 = e3 * sc.pt.s_pe + e1 * sc.pt.s_c;
     if (sc.pt.s_wpe != 0.0 || sc.pt.s_wc != 0.0) {  // collimator material (grid scatterers)
         double w3 = e3, w1 = e1;
-        if (tables_f) {
-            w3 = mu_ratio_f(*ph.tabf, 2, jf, Lf);
-            w1 = mu_ratio_f(*ph.tabf, 3, jf, Lf);
-        } else if (tables) {
-            const double le = log(ep);
-            w3 = exp(mu_table_log(*ph.tab, 2, le) - ph.tab->lmu0[2]);
-            w1 = exp(mu_table_log(*ph.tab, 3, le) - ph.tab->lmu0[3]);
+        if (tables) {
+            if constexpr (std::is_same<T, float>::value) {
+                w3 = mu_ratio_f(*ph.tabf, 2, jf, Lf);
+                w1 = mu_ratio_f(*ph.tabf, 3, jf, Lf);
+            } else {
+                const double le = log(ep);
+                w3 = exp(mu_table_log(*ph.tab, 2, le) - ph.tab->lmu0[2]);
+                w1 = exp(mu_table_log(*ph.tab, 3, le) - ph.tab->lmu0[3]);
+            }
         }
         att2 += w3 * sc.pt.s_wpe + w1 * sc.pt.s_wc;
     }
