@@ -39,16 +39,25 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 # measured HBM traffic of one FP32 launch of the full-size PE step (rocprofv3 PMC
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a plain 1 GiB copy)
-TRAFFIC_BYTES = 2.24e9
+TRAFFIC_BYTES = 1.152e9
 # executed work of the same launch (rocprofv3 PMC, FP32 path): FLOPs per sub-cell ray
-# the lanes execute (64 x SQ_INSTS_VALU_FLOPS_FP32 + _FP64 per ray) and VALU wave
-# instructions per ray x 64
+# the lanes execute (64 x SQ_INSTS_VALU_FLOPS_FP32 + _FP64 per ray), VALU wave
+# instructions per ray x 64, transcendental instructions per ray x 64 (FP32 + FP64),
+# VALUBusy, lane utilisation and mean resident waves per SIMD (of the 3 the
+# kernel's registers allow)
 EXEC_SOURCE = "profiles/pe_forward_instruction_mix_fp32_fp64_64x64x32_2048bars.md"
-EXEC_FLOPS_PER_RAY = 362.0 + 267.9
-VALU_INSTS_PER_RAY = 1136.7
-# unpacked FP32 (= FP64) vector FMA peak, and the VALU issue peak (256 CUs x 4 SIMDs x
-# 2.4 GHz, one wave64 instruction per 4 cycles)
+EXEC_FLOPS_PER_RAY = 252.1 + 187.9
+VALU_INSTS_PER_RAY = 794.6
+TRANS_INSTS_PER_RAY = 23.0 + 4.8
+VALU_BUSY = 0.669
+LANE_UTILISATION = 0.758
+WAVES_PER_SIMD = 2.77
+MAX_WAVES_PER_SIMD = 3
+# unpacked FP32 (= FP64) vector FMA peak, the transcendental peak (a quarter of
+# the unpacked FMA rate in operations), and the VALU issue peak (256 CUs x 4 SIMDs
+# x 2.4 GHz, one wave64 instruction per 4 cycles)
 PEAK_FP32_UNPACKED_TFLOPS = 78.6
+PEAK_TRANS_OPS = PEAK_FP32_UNPACKED_TFLOPS * 1e12 / 2 / 4
 PEAK_VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
 
 
@@ -486,6 +495,7 @@ def main():
     if args.precision == "fp32":
         ex_achieved = EXEC_FLOPS_PER_RAY * rays / avg_kernel_s / 1e12
         issue = VALU_INSTS_PER_RAY * rays / 64 / avg_kernel_s
+        trans = TRANS_INSTS_PER_RAY * rays / avg_kernel_s
         roofline["executed"] = {
             "what": "FLOPs the lanes execute (PMC), against the unpacked FP32/FP64 vector peak",
             "flops_per_ray": EXEC_FLOPS_PER_RAY,
@@ -495,6 +505,12 @@ def main():
             "frac": round(ex_achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
             "valu_insts_per_ray": VALU_INSTS_PER_RAY,
             "valu_issue_frac": round(issue / PEAK_VALU_WAVE_INSTS_PER_S, 4),
+            "valu_busy": VALU_BUSY,
+            "lane_utilisation": LANE_UTILISATION,
+            "transcendental_per_ray": TRANS_INSTS_PER_RAY,
+            "transcendental_frac": round(trans / PEAK_TRANS_OPS, 4),
+            "occupancy_waves_per_simd": WAVES_PER_SIMD,
+            "occupancy_limit": f"{MAX_WAVES_PER_SIMD} waves per SIMD (VGPRs)",
             "source": EXEC_SOURCE,
         }
     ex = None if args.no_extras else extras(smg, params, ctx, out, args.precision)

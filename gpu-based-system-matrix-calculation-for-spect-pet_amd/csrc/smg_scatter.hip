@@ -219,13 +219,37 @@ SliceShape slice_shape(const ScatterLaunch& A) {
     return s;
 }
 
+// Per-wave term lists of scatter_kernel, in LDS.  Most terms fail the
+// energy-window test, and those that pass are scattered over the lanes, so a
+// wave that evaluated the term of every lane for each scatter point in turn
+// kept about half its lanes idle.  Instead a batch of scatter points is first
+// tested (each lane for its voxel), the (point, lane) pairs that pass are
+// listed, the terms of the list are evaluated 64 at a time (lane k takes pair
+// k, whatever its voxel), and each lane then adds its own terms in point
+// order: the same sums, in the same order, as the per-point loop.
+constexpr int kScatterBatchPts = 128;  // points with a passing lane per batch
+template <bool kExact>
+struct ScatterLists {
+    static constexpr int kCap = kExact ? 512 : 1024;  // pairs per batch
+    using T = typename std::conditional<kExact, double, float>::type;
+    uint64_t mask[kScatterBatchPts];  // lanes whose term passes, per batch point
+    int pre[kScatterBatchPts];        // pairs listed before the point
+    int pt[kScatterBatchPts];         // the point's index
+    uint16_t pair[kCap];              // (batch point << 6) | lane
+    T val[kCap];                      // the pairs' terms
+    double vp[3][64], vps[3][64];     // the lanes' voxels: world, frame of bar d
+};
+
 // 3 waves per SIMD: the register cap (168) spills a few values that the
 // inline FP64 fallback holds; measured 14% faster (FP32) and 5% (FP64) than the
-// 2 waves the unconstrained allocation gives.
+// 2 waves the unconstrained allocation gives.  LDS: 11.3 KB (FP32) / 10.2 KB
+// (FP64) per wave, 12 waves per CU.
 template <bool kExact>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void scatter_kernel(
     ScatterLaunch A, SliceShape sh) {
     using T = typename std::conditional<kExact, double, float>::type;
+    using Lists = ScatterLists<kExact>;
+    __shared__ Lists S;
     const PeGeom& G = A.geom;
     const GeomDesc& g = G.g;
     const int lane = threadIdx.x;
@@ -291,6 +315,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void sc
         skip_lo = (d - A.pt_bar0) * A.pts_per_bar;
         skip_hi = skip_lo + A.pts_per_bar;
     }
+    const int64_t v0 = v - lane;  // first voxel of the wave
+    for (int k = 0; k < 3; ++k) {
+        S.vp[k][lane] = p[k];
+        S.vps[k][lane] = ps[k];
+    }
+    // weight of scatter point s for voxel vo at po
+    auto weight = [&](int s, int64_t vo, const double po[3]) -> T {
+        if (A.kind == 0) return T(A.pe[(int64_t)s * g.n_vox + vo]) * wscale;
+        if (A.kind == 2 || A.kind == 3) return T(A.w1[(int64_t)s * g.n_vox + vo]);
+        const ScatterPt<double>& sp = A.pts_d[s];
+        double dx = po[0] - sp.x, dy = po[1] - sp.y, dz = po[2] - sp.z;
+        double lv = sqrt(dx * dx + dy * dy + dz * dz);
+        double om;
+        if (A.variant == 0) {  // scatter.cu:1711-1716
+            double rr = (double)0.1f;
+            om = (rr * rr * 3.14159265358979323846 + sp.dy * rr) /
+                 (4.0 * 3.14159265358979323846 * lv * lv) * fabs(sp.y - po[1]) / lv;
+        } else {  // ScatterGen_Collimator/scatter.cu:1212-1215
+            double rr = (double)0.2f;
+            om = (rr * rr + sp.dy * rr) / (4.0 * lv * lv);
+        }
+        return T(om * A.coll_ratio);
+    };
+    const uint64_t below = (1ull << lane) - 1ull;  // the lanes under this one
     KahanSum<T> acc;
     unsigned fallbacks = 0;
     for (int cell = 0; cell < ncell; ++cell) {
@@ -303,46 +351,77 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void sc
             sc.lo[k] = cc.lo[k];
             sc.hi[k] = cc.hi[k];
         }
-        for (int s = s_begin; s < s_end; ++s) {
-            if (s >= skip_lo && s < skip_hi) continue;
-            const ScatterPt<double>& sp = A.pts_d[s];
-            sc.qs[0] = sp.x;
-            sc.qs[1] = sp.y;
-            sc.qs[2] = sp.z;
-            if (!valid || below_window<T>(ph, p, sc)) continue;
-            to_frame(bd, sc.qs, sc.qss);
-            sc.pt = A.pair[((int64_t)r * A.n_pts + s) * ncell + cell];
-            // weight of the scatter point for this voxel
-            T wgt;
-            if (A.kind == 0) {
-                wgt = T(A.pe[(int64_t)s * g.n_vox + v]) * wscale;
-            } else if (A.kind == 2 || A.kind == 3) {
-                wgt = T(A.w1[(int64_t)s * g.n_vox + v]);
-            } else {
-                double dx = p[0] - sp.x, dy = p[1] - sp.y, dz = p[2] - sp.z;
-                double lv = sqrt(dx * dx + dy * dy + dz * dz);
-                double om;
-                if (A.variant == 0) {  // scatter.cu:1711-1716
-                    double r = (double)0.1f;
-                    om = (r * r * 3.14159265358979323846 + sp.dy * r) /
-                         (4.0 * 3.14159265358979323846 * lv * lv) * fabs(sp.y - p[1]) / lv;
-                } else {  // ScatterGen_Collimator/scatter.cu:1212-1215
-                    double r = (double)0.2f;
-                    om = (r * r + sp.dy * r) / (4.0 * lv * lv);
-                }
-                wgt = T(om * A.coll_ratio);
-            }
-            if (wgt == T(0)) continue;
-            bool near = false;
-            T val = scatter_term<T>(ph, p, ps, sc, wgt, near);
-            if constexpr (!kExact) {
-                if (near) {
-                    bool n2 = false;
-                    val = T(scatter_term<double>(ph, p, ps, sc, double(wgt), n2));
-                    ++fallbacks;
+        int s = s_begin;
+        while (s < s_end) {  // wave-uniform
+            // 1. the batch: points in order until kScatterBatchPts of them have
+            // a passing lane or the list could overflow with the next point
+            int n = 0, nb = 0;
+            while (s < s_end && nb < kScatterBatchPts && n <= Lists::kCap - 64) {
+                const int si = s++;
+                if (si >= skip_lo && si < skip_hi) continue;  // wave-uniform
+                const ScatterPt<double>& sp = A.pts_d[si];
+                sc.qs[0] = sp.x;
+                sc.qs[1] = sp.y;
+                sc.qs[2] = sp.z;
+                const bool pass = valid && !below_window<T>(ph, p, sc);
+                const uint64_t m = __ballot(pass);
+                if (m) {  // wave-uniform
+                    if (pass) S.pair[n + __popcll(m & below)] = (uint16_t)((nb << 6) | lane);
+                    if (lane == 0) {
+                        S.mask[nb] = m;
+                        S.pre[nb] = n;
+                        S.pt[nb] = si;
+                    }
+                    n += __popcll(m);
+                    ++nb;
                 }
             }
-            acc.add(val);
+            __syncthreads();
+            // 2. the listed terms, 64 at a time
+            for (int base = 0; base < n; base += 64) {
+                const int k = base + lane;
+                if (k < n) {
+                    const int e = S.pair[k];
+                    const int j = e >> 6, o = e & 63;
+                    const int si = S.pt[j];
+                    const double po[3] = {S.vp[0][o], S.vp[1][o], S.vp[2][o]};
+                    const double pso[3] = {S.vps[0][o], S.vps[1][o], S.vps[2][o]};
+                    ScatterCell sk;
+                    for (int q = 0; q < 3; ++q) {
+                        sk.q[q] = sc.q[q];
+                        sk.lo[q] = sc.lo[q];
+                        sk.hi[q] = sc.hi[q];
+                    }
+                    const ScatterPt<double>& sp = A.pts_d[si];
+                    sk.qs[0] = sp.x;
+                    sk.qs[1] = sp.y;
+                    sk.qs[2] = sp.z;
+                    to_frame(bd, sk.qs, sk.qss);
+                    sk.pt = A.pair[((int64_t)r * A.n_pts + si) * ncell + cell];
+                    const T wgt = weight(si, v0 + o, po);
+                    T val = T(0);
+                    if (wgt != T(0)) {
+                        bool near = false;
+                        val = scatter_term<T>(ph, po, pso, sk, wgt, near);
+                        if constexpr (!kExact) {
+                            if (near) {
+                                bool n2 = false;
+                                val = T(scatter_term<double>(ph, po, pso, sk, double(wgt), n2));
+                                ++fallbacks;
+                            }
+                        }
+                    }
+                    S.val[k] = val;
+                }
+            }
+            __syncthreads();
+            // 3. each lane's terms in point order (a term of weight 0 adds
+            // an exact 0)
+            for (int j = 0; j < nb; ++j) {
+                const uint64_t m = S.mask[j];
+                if ((m >> lane) & 1ull) acc.add(S.val[S.pre[j] + __popcll(m & below)]);
+            }
+            __syncthreads();  // the lists are refilled by the next batch
         }
     }
     if (valid) {
