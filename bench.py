@@ -125,10 +125,25 @@ def ops_per_ray(p, sub):
     return o.pe_ops_per_ray(d, v, sub=sub)
 
 
-def scatter_roofline(params, p, kind, d, v, kernel_ms, terms, sample, **kw):
+# executed work of scatter_kernel<false> per (row, voxel, scatter point, cell) term
+# (rocprofv3 PMC, FP32): FLOPs the lanes execute, VALU instructions x 64, lane
+# utilisation and VALUBusy
+SCATTER_EXEC_SOURCE = "profiles/scatter_kernels_4_workloads.md"
+SCATTER_EXEC = {
+    "crystal_scatter": {"flops_per_term": 81.4, "valu_insts_per_term": 99.1, "lane_utilisation": 0.755,
+                        "valu_busy": 0.62},
+    "collimator_grid": {"flops_per_term": 273.6, "valu_insts_per_term": 350.3, "lane_utilisation": 0.717,
+                        "valu_busy": 0.48},
+}
+
+
+def scatter_roofline(params, p, kind, d, v, kernel_ms, terms, sample, exec_key=None, **kw):
     """Roofline of a scatter launch: algorithmic operations per (row, voxel,
     scatter point, cell) term of the reference's scatter element body, counted
-    by the oracle's op counter on a seeded sample, times the launch's terms."""
+    by the oracle's op counter on a seeded sample, times the launch's terms;
+    with exec_key, also the FLOPs the lanes execute (PMC profile) over the
+    workload's kernel time (all its kernels, so a lower bound for the scatter
+    kernel itself)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
@@ -136,9 +151,21 @@ def scatter_roofline(params, p, kind, d, v, kernel_ms, terms, sample, **kw):
     o.set_mu_tables(*params.synthetic_mu_tables(True))
     f, n = o.scatter_ops_per_term(kind, d, v, **kw)
     achieved = f * terms / (kernel_ms * 1e-3) / 1e12
-    return {"bound": "valu", "alg_ops_per_term": round(f, 1), "achieved": round(achieved, 3),
-            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "op_count_sample": f"{sample} ({int(n)} terms)"}
+    out = {"bound": "valu", "alg_ops_per_term": round(f, 1), "achieved": round(achieved, 3),
+           "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+           "op_count_sample": f"{sample} ({int(n)} terms)"}
+    if exec_key:
+        e = SCATTER_EXEC[exec_key]
+        ex = e["flops_per_term"] * terms / (kernel_ms * 1e-3) / 1e12
+        out["executed"] = {"what": "FLOPs the lanes execute (PMC) over the workload's kernel time, "
+                                   "against the unpacked FP32/FP64 vector peak",
+                           "flops_per_term": e["flops_per_term"], "achieved": round(ex, 2),
+                           "peak": PEAK_FP32_UNPACKED_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(ex / PEAK_FP32_UNPACKED_TFLOPS, 4),
+                           "valu_insts_per_term": e["valu_insts_per_term"],
+                           "lane_utilisation": e["lane_utilisation"], "valu_busy": e["valu_busy"],
+                           "source": SCATTER_EXEC_SOURCE}
+    return out
 
 
 def timed(fn, reps):
@@ -249,7 +276,8 @@ def _crystal_scatter(np, smg, params, ex, precs):
                 rng = np.random.default_rng(13)
                 d, v = rng.integers(0, p4.n_det, 16), rng.integers(0, p4.n_vox, 16)
                 ex["crystal_scatter_fp32"]["roofline"] = scatter_roofline(
-                    params, p4, 0, d, v, ms, st["rays"], "16 seeded elements", bitmap=c.bitmap_crystal())
+                    params, p4, 0, d, v, ms, st["rays"], "16 seeded elements", exec_key="crystal_scatter",
+                    bitmap=c.bitmap_crystal())
         ex["crystal_scatter_workload"] = ("32x32x1 voxels x 2048 bars, scatter points at the "
                                           "2047 other bar centres (reference semantics), 1x1x1 cells, "
                                           "64-node mu(E) tables")
@@ -297,7 +325,8 @@ def _collimator_grid(np, smg, params, ex, precs):
                 rng = np.random.default_rng(15)
                 d, v = rng.integers(0, p3.n_det, 2), rng.integers(0, p3.n_vox, 2)
                 ex["collimator_grid_scatter_fp32"]["roofline"] = scatter_roofline(
-                    params, p3, 2, d, v, st["kernel_ms"], st["rays"], "2 seeded elements", pitch=0.5)
+                    params, p3, 2, d, v, st["kernel_ms"], st["rays"], "2 seeded elements",
+                    exec_key="collimator_grid", pitch=0.5)
         ex["collimator_grid_scatter_workload"] = (f"32x32x16 voxels x 64 bars, {n} scatter points on "
                                                   "a 0.5 mm grid over the 60x5x60 mm tungsten, "
                                                   "1x1x1 cells, 64-node mu(E) tables")
