@@ -10,10 +10,11 @@
 
 namespace smg {
 
-// LDS capacity of one wave's candidate lists (bars, holes).  A wave whose
+// LDS capacity of one wave's candidate lists (bars, holes): with the rest of
+// a PE wave's LDS 10 KB, so 16 one-wave blocks fit a CU.  A wave whose
 // candidates exceed these is not truncated: it queues its work item, and a
 // second kernel redoes the queued items with lists in global memory (PeSpill).
-constexpr int kCapBars = 512;
+constexpr int kCapBars = 448;
 constexpr int kCapHoles = 256;
 
 // Waves of a spill kernel (one list slot each).

@@ -30,6 +30,19 @@ namespace smg {
 
 namespace {
 
+// A wave-uniform double held in scalar registers: values every lane holds
+// alike (the tile ball, the sub-cell context) would otherwise take a vector
+// register per lane pair; the VALU reads them as scalar operands.
+__device__ __forceinline__ double uniform_d(double x) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float uniform_f(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
 // Capacity of the per-wave queue of sub-cells to redo in FP64.
 constexpr int kRedoCap = 64;
 
@@ -159,7 +172,11 @@ __device__ __forceinline__ void wave_setup(const PeLaunch& A, int d, int tile, i
     for (int k = 0; k < 3; ++k) W.udf[k] = float(W.sc.ud[k]);
     W.critf = float(W.sc.crit);
     tile_ball(g, ix0, iy0, iz0, A.tile, A.cphi, A.sphi, W.tc, W.rho);
+    for (int k = 0; k < 3; ++k) W.tc[k] = uniform_d(W.tc[k]);
+    W.rho = uniform_d(W.rho);
     tile_ball_f(W.tc, W.rho, W.tcf, W.rhof);
+    for (int k = 0; k < 3; ++k) W.tcf[k] = uniform_f(W.tcf[k]);
+    W.rhof = uniform_f(W.rhof);
     // plate gates that no ray from any point of bar d (inside its bounding
     // ball, the half diagonal with a rounding margin) to the tile can trigger
     const double qc[3] = {bd.cx, bd.cy, bd.cz};
@@ -270,6 +287,14 @@ __device__ __forceinline__ void tile_cell(const PeLaunch& A, const Bar<double>& 
         cc = A.cells[(int64_t)(d - A.cells_bar0) * (A.sub[0] * A.sub[1] * A.sub[2]) + cell];
     else
         make_cell_ctx_linear(bd, cell, A.sub, cc);
+    for (int k = 0; k < 3; ++k) {
+        cc.q[k] = uniform_d(cc.q[k]);
+        cc.qs[k] = uniform_d(cc.qs[k]);
+        cc.lo[k] = uniform_d(cc.lo[k]);
+        cc.hi[k] = uniform_d(cc.hi[k]);
+        cc.area[k] = uniform_d(cc.area[k]);
+        cc.hc[k] = uniform_f(cc.hc[k]);
+    }
     const GeomDesc& g = A.geom.g;
     const uint32_t all = (uint32_t)((1ull << g.n_layers) - 1ull);
     if ((W.bar_gate & all) == all) {  // wave-uniform
@@ -505,13 +530,13 @@ __device__ __forceinline__ bool pe_forward_body(const PeLaunch& A, int item, Lis
     return true;
 }
 
-// 3 waves per SIMD: uncapped, the FP32 allocation lands at 169 registers (2
-// waves per SIMD, 36% slower on the full-size step); the cap costs a 12-byte
-// spill.  The same holds for the projection below.
+// 4 waves per SIMD: 128 VGPRs (the sub-cell context and the tile ball in
+// scalar registers) with a ~20-byte spill, and 16 one-wave blocks per CU
+// within the LDS (10 KB of lists each).  The projections below keep 3 waves.
 // kEll: elliptical holes (the elliptic-cylinder chord compiled in; the
 // all-FP64 kernels always have it).
 template <bool kExact, bool kEll = kExact>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_forward_kernel(PeLaunch A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pe_forward_kernel(PeLaunch A) {
     __shared__ WaveLists L;
     const int item = xcd_item((int)blockIdx.x, (int)gridDim.x);
     if (!pe_forward_body<kExact, false, kEll>(A, item, L)) spill_push(A, item, threadIdx.x);
