@@ -30,19 +30,6 @@ namespace smg {
 
 namespace {
 
-// A wave-uniform double held in scalar registers: values every lane holds
-// alike (the tile ball, the sub-cell context) would otherwise take a vector
-// register per lane pair; the VALU reads them as scalar operands.
-__device__ __forceinline__ double uniform_d(double x) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-__device__ __forceinline__ float uniform_f(float x) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
-}
-
 // Capacity of the per-wave queue of sub-cells to redo in FP64.
 constexpr int kRedoCap = 64;
 
@@ -532,7 +519,7 @@ __device__ __forceinline__ bool pe_forward_body(const PeLaunch& A, int item, Lis
 
 // 4 waves per SIMD: 128 VGPRs (the sub-cell context and the tile ball in
 // scalar registers) with a ~20-byte spill, and 16 one-wave blocks per CU
-// within the LDS (10 KB of lists each).  The projections below keep 3 waves.
+// within the LDS (10 KB of lists each).  The projections below likewise.
 // kEll: elliptical holes (the elliptic-cylinder chord compiled in; the
 // all-FP64 kernels always have it).
 template <bool kExact, bool kEll = kExact>
@@ -686,7 +673,7 @@ __device__ __forceinline__ bool pe_tangent_body(const PeLaunch& A, const PeTange
 }
 
 template <bool kExact>
-__global__ __launch_bounds__(64) void pe_tangent_kernel(PeLaunch A, PeTangent TG) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_tangent_kernel(PeLaunch A, PeTangent TG) {
     __shared__ WaveLists L;
     if (!pe_tangent_body<kExact>(A, TG, (int)blockIdx.x, L)) spill_push(A, (int)blockIdx.x, threadIdx.x);
 }
@@ -1064,7 +1051,7 @@ __device__ __forceinline__ bool pe_project_body(const PeLaunch& A, const float* 
 }
 
 template <bool kExact, bool kEll = kExact>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_project_kernel(PeLaunch A, const float* x, double* partial) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pe_project_kernel(PeLaunch A, const float* x, double* partial) {
     __shared__ WaveLists L;
     if (!pe_project_body<kExact, kEll>(A, x, partial, (int)blockIdx.x, L)) spill_push(A, (int)blockIdx.x, threadIdx.x);
 }
@@ -1108,7 +1095,7 @@ __device__ __forceinline__ bool pe_backproject_body(const PeLaunch& A, const flo
 }
 
 template <bool kExact, bool kEll = kExact>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_backproject_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pe_backproject_kernel(
     PeLaunch A, const float* y, int group, double* partial) {
     __shared__ WaveLists L;
     if (!pe_backproject_body<kExact, kEll>(A, y, group, partial, (int)blockIdx.x, L)) spill_push(A, (int)blockIdx.x, threadIdx.x);

@@ -82,6 +82,29 @@ SMG_FN T tsqrt(T x) {
     return sqrt(x);
 }
 
+// A wave-uniform value held in scalar registers: values every lane holds
+// alike (a tile's bounding ball, a sub-cell's context, a row's physics) would
+// otherwise take vector registers; the VALU reads them as scalar operands.
+// The caller guarantees the value is the same in every active lane.  Host
+// build: the value itself.
+SMG_FN double uniform_d(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+#else
+    return x;
+#endif
+}
+SMG_FN float uniform_f(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+#else
+    return x;
+#endif
+}
+
 // FP64 reciprocal, and reciprocal and square root, for the FP32 path's FP64
 // set-up: the hardware approximations (v_rcp_f64, v_rsq_f64) refined by two
 // Newton steps, within a few ulp of the correctly rounded values at a third of

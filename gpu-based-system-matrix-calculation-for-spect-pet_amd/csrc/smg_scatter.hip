@@ -303,6 +303,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void sc
         ph.rcell = sqrt(bd.w * bd.w / sx / sx + bd.h * bd.h / sz / sz + bd.t * bd.t / sy / sy) / 2.0;
     }
     ph.c_rej = window_reject_cos(ph, kExact ? 1e-9 : 1e-5);
+    // the row's physics is the same for every lane: scalar registers
+    ph.e0 = uniform_d(ph.e0);
+    ph.lo = uniform_d(ph.lo);
+    ph.hi = uniform_d(ph.hi);
+    ph.norm = uniform_d(ph.norm);
+    ph.res = uniform_d(ph.res);
+    ph.mupe = uniform_d(ph.mupe);
+    ph.muc = uniform_d(ph.muc);
+    ph.rcell = uniform_d(ph.rcell);
+    ph.c_rej = uniform_d(ph.c_rej);
     const int ncell = A.sub[0] * A.sub[1] * A.sub[2];
     const T wscale = T(bd.muc / bd.mupe);  // crystal weight uses bar d's mu (scatter.cu:1218-1219)
     // scatter points inside bar d itself do not count: its centre (kind 0),
@@ -346,10 +356,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void sc
         CellCtx cc;
         make_cell_ctx(bd, a, b, c, A.sub[0], A.sub[1], A.sub[2], cc);
         ScatterCell sc;
-        for (int k = 0; k < 3; ++k) {
-            sc.q[k] = cc.q[k];
-            sc.lo[k] = cc.lo[k];
-            sc.hi[k] = cc.hi[k];
+        for (int k = 0; k < 3; ++k) {  // the same for every lane
+            sc.q[k] = uniform_d(cc.q[k]);
+            sc.lo[k] = uniform_d(cc.lo[k]);
+            sc.hi[k] = uniform_d(cc.hi[k]);
         }
         int s = s_begin;
         while (s < s_end) {  // wave-uniform
