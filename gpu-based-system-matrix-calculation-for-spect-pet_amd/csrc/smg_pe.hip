@@ -37,18 +37,25 @@ constexpr int kRedoCap = 64;
 // kept for fast_bar_chord (the first kFastBars).
 constexpr int kFastBars = 32;
 
-// Candidate lists of one wave, in LDS.
-struct WaveLists {
-    static constexpr int cap_bars = kCapBars, cap_holes = kCapHoles;
-    uint64_t bar_mask[kCapBars];
-    uint64_t hole_mask[kCapHoles];
-    uint16_t bar_idx[kCapBars];
-    uint16_t hole_idx[kCapHoles];
-    uint16_t bar_sel[kCapBars];
-    uint16_t hole_sel[kCapHoles];
+// Candidate lists of one wave, in LDS, for up to CapB bars and CapH holes.
+template <int CapB, int CapH>
+struct WaveListsT {
+    static constexpr int cap_bars = CapB, cap_holes = CapH;
+    uint64_t bar_mask[CapB];
+    uint64_t hole_mask[CapH];
+    uint16_t bar_idx[CapB];
+    uint16_t hole_idx[CapH];
+    uint16_t bar_sel[CapB];
+    uint16_t hole_sel[CapH];
     float faces[kFastBars][6];   // narrowed bar j: faces - sub-cell point (fast_faces)
     uint8_t fast_ok[kFastBars];  // narrowed bar j qualifies for fast_bar_chord
 };
+using WaveLists = WaveListsT<kCapBars, kCapHoles>;
+
+// The adjoint's lists hold fewer bars: with its per-candidate FP64 sums and
+// kept chords, 13 KB of LDS per wave, so 12 waves (3 per SIMD) fit a CU.
+constexpr int kCapBarsAdjoint = 320;
+using AdjointLists = WaveListsT<kCapBarsAdjoint, kCapHoles>;
 
 // Candidate lists of a wave whose tile overflowed the LDS lists, in its own
 // slot of global memory with room for every bar and hole of the geometry (so
@@ -894,10 +901,13 @@ __device__ __forceinline__ bool pe_backward_body(const PeLaunch& A, const PeGrad
     return true;
 }
 
+// 3 waves per SIMD (LDS and a 168-VGPR cap); uncapped the allocation lands
+// at 188 VGPRs, 2 waves.
 template <bool kExact>
-__global__ __launch_bounds__(64) void pe_backward_kernel(PeLaunch A, PeGradScratch S) {
-    __shared__ WaveLists L;
-    __shared__ double cand_acc[kCapBars];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pe_backward_kernel(
+    PeLaunch A, PeGradScratch S) {
+    __shared__ AdjointLists L;
+    __shared__ double cand_acc[kCapBarsAdjoint];
     if (!pe_backward_body<kExact>(A, S, (int)blockIdx.x, L, cand_acc)) spill_push(A, (int)blockIdx.x, threadIdx.x);
 }
 
