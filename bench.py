@@ -39,20 +39,20 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 # measured HBM traffic of one FP32 launch of the full-size PE step (rocprofv3 PMC
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a plain 1 GiB copy)
-TRAFFIC_BYTES = 1.152e9
+TRAFFIC_BYTES = 1.157e9
 # executed work of the same launch (rocprofv3 PMC, FP32 path): FLOPs per sub-cell ray
 # the lanes execute (64 x SQ_INSTS_VALU_FLOPS_FP32 + _FP64 per ray), VALU wave
 # instructions per ray x 64, transcendental instructions per ray x 64 (FP32 + FP64),
-# VALUBusy, lane utilisation and mean resident waves per SIMD (of the 3 the
-# kernel's registers allow)
+# VALUBusy, lane utilisation and mean resident waves per SIMD (of the 4 the
+# kernel's registers and LDS allow)
 EXEC_SOURCE = "profiles/pe_forward_instruction_mix_fp32_fp64_64x64x32_2048bars.md"
-EXEC_FLOPS_PER_RAY = 252.1 + 187.9
-VALU_INSTS_PER_RAY = 794.6
+EXEC_FLOPS_PER_RAY = 250.9 + 187.9
+VALU_INSTS_PER_RAY = 836.0
 TRANS_INSTS_PER_RAY = 23.0 + 4.8
-VALU_BUSY = 0.669
-LANE_UTILISATION = 0.758
-WAVES_PER_SIMD = 2.77
-MAX_WAVES_PER_SIMD = 3
+VALU_BUSY = 0.824
+LANE_UTILISATION = 0.725
+WAVES_PER_SIMD = 3.72
+MAX_WAVES_PER_SIMD = 4
 # unpacked FP32 (= FP64) vector FMA peak, the transcendental peak (a quarter of
 # the unpacked FMA rate in operations), and the VALU issue peak (256 CUs x 4 SIMDs
 # x 2.4 GHz, one wave64 instruction per 4 cycles)
@@ -539,7 +539,7 @@ def main():
             "transcendental_per_ray": TRANS_INSTS_PER_RAY,
             "transcendental_frac": round(trans / PEAK_TRANS_OPS, 4),
             "occupancy_waves_per_simd": WAVES_PER_SIMD,
-            "occupancy_limit": f"{MAX_WAVES_PER_SIMD} waves per SIMD (VGPRs)",
+            "occupancy_limit": f"{MAX_WAVES_PER_SIMD} waves per SIMD (128 VGPRs, 10 KB of LDS per wave)",
             "source": EXEC_SOURCE,
         }
     ex = None if args.no_extras else extras(smg, params, ctx, out, args.precision)

@@ -158,7 +158,7 @@ smg_status smg_pe_tangent(smg_ctx* ctx, int rotation, smg_precision prec, const 
  * context on that GPU, so calls that overlap on one GPU race.  Limits
  * (SMG_ERR_CAPACITY): bars x sub-cells x 136 bytes must fit in 32 MiB - 256
  * bytes (2048 bars at 3 x 3 x 3 sub-cells take 7.5 MB), and a voxel tile
- * that needs more than 512 bars or 256 holes of candidates is refused (its
+ * that needs more than 448 bars or 256 holes of candidates is refused (its
  * elements are then left unwritten; smg_pe_forward handles such tiles). */
 smg_status smg_pe_forward_noalloc(smg_ctx* ctx, int rotation, smg_precision prec, float* dst_device,
                                   smg_stats* stats);

@@ -171,7 +171,7 @@ def narrow_plate_geometry(params_mod):
 
 def overflow_geometry(params_mod, n_layers=24, bars_per_side=12):
     """A geometry whose voxel tiles need more candidates than the PE kernels'
-    LDS lists hold (512 bars, 256 holes): a 20 mm tungsten plate with 1,600
+    LDS lists hold (448 bars, 256 holes; 320 bars in the adjoint): a 20 mm tungsten plate with 1,600
     holes on a 1 mm grid, a deep block of n_layers layers of 2 mm cubic bars
     right behind it, and 8 mm voxels, so a 4x4x4-voxel tile sees a wide
     frustum through a thick plate.  The kernels then redo such tiles with

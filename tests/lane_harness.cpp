@@ -165,7 +165,7 @@ void lane_xcd_items(int n, int* out) {
 // gates skipped; stats[7] = share of (wave, layer) gates cleared for the
 // whole bar (plate_gate_clear_ball), the rest tested per sub-cell;
 // stats[8], stats[9] = the largest tile candidate lists (bars, holes) over the
-// waves, which the kernels hold in LDS up to 512 bars and 256 holes;
+// waves, which the kernels hold in LDS up to 448 bars and 256 holes;
 // stats[10] = the most narrowed bars of one sub-cell, stats[11] = the share of
 // sub-cells with more than 8 (the adjoint keeps the chords of the first 8 in
 // LDS and recomputes the others); the FP32 term with the narrowed lists and

@@ -1,4 +1,4 @@
-"""Tiles whose candidate lists exceed the PE kernels' LDS lists (512 bars,
+"""Tiles whose candidate lists exceed the PE kernels' LDS lists (448 bars, 320 in the adjoint,
 256 holes; the reference culls up to 3000 bars and 1000 holes per element,
 PEGen_RayTracing_CircularHole/PESysMatGen.cu:1087,1094): the kernels queue
 them and redo them with lists in global memory.  Every PE-family entry point

@@ -228,7 +228,7 @@ def test_window_reject_cos(lane_lib):
 
 def test_lane_overflow_geometry(lane_lib, oracle_mod, params_mod):
     """The overflow geometry's tiles need more candidates than the kernels'
-    LDS lists hold (512 bars, 256 holes), so the kernels take their spill
+    LDS lists hold (448 bars, 256 holes), so the kernels take their spill
     path (lists in global memory with room for every bar and hole); the
     narrowing stays exact there, and the lane arithmetic with full candidate
     lists, which is what the spill path computes, matches the oracle."""
