@@ -124,13 +124,14 @@ smg_status smg_set_mu_tables(smg_ctx* ctx, int n_nodes, const double* energy_kev
  * the hole axis, as the reference's routine takes it); a hole with x_rho <= 0
  * keeps the parameter file's circle.  shapes = NULL makes every hole a circle.
  * The PE matrix, its adjoint, the projections and the crystal grid weights
- * trace elliptical holes with the reference routine's chord, always on the
- * all-FP64 kernels (an FP32 request runs them too: the FP32 kernels carry
- * only the circular chord); culling and the candidate scans use the bounding
- * radius max(x_rho, z_rho).  The collimator
- * grid scatterers (smg_scatter_collimator_grid) need circular holes
- * (SMG_ERR_BAD_PARAM otherwise).  smg_set_geometry resets every hole to a
- * circle. */
+ * trace elliptical holes with the reference routine's chord: the matrix and
+ * the projections in the requested precision (FP32 kernels with the elliptic
+ * chord compiled in), the adjoint, tangent and crystal grid weights on their
+ * all-FP64 kernels whatever the request; culling and the candidate scans use
+ * the bounding radius max(x_rho, z_rho).  The collimator grid scatterers
+ * (smg_scatter_collimator_grid) leave out the grid points inside an ellipse
+ * and subtract the elliptic-cylinder chords from the plate material.
+ * smg_set_geometry resets every hole to a circle. */
 smg_status smg_set_hole_shapes(smg_ctx* ctx, const float* shapes, int n_holes);
 
 /* ---- photoelectric matrix (replaces PESysMatGen) ---------------------------
