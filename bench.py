@@ -54,8 +54,10 @@ LANE_UTILISATION = 0.725
 WAVES_PER_SIMD = 3.72
 MAX_WAVES_PER_SIMD = 4
 # unpacked FP32 (= FP64) vector FMA peak, the transcendental peak (a quarter of
-# the unpacked FMA rate in operations), and the VALU issue peak (256 CUs x 4 SIMDs
-# x 2.4 GHz, one wave64 instruction per 4 cycles)
+# the unpacked FMA rate in operations), and the VALU issue reference rate (256 CUs x
+# 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles: the unpacked FMA rate; the
+# bitmap kernel's FP32 compare-and-select mix issues ~1.2 per 4 cycles, so this is a
+# yardstick, not a ceiling for every instruction mix)
 PEAK_FP32_UNPACKED_TFLOPS = 78.6
 PEAK_TRANS_OPS = PEAK_FP32_UNPACKED_TFLOPS * 1e12 / 2 / 4
 PEAK_VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
@@ -130,10 +132,10 @@ def ops_per_ray(p, sub):
 # utilisation and VALUBusy
 SCATTER_EXEC_SOURCE = "profiles/scatter_kernels_4_workloads.md"
 SCATTER_EXEC = {
-    "crystal_scatter": {"flops_per_term": 81.4, "valu_insts_per_term": 99.1, "lane_utilisation": 0.755,
-                        "valu_busy": 0.62},
-    "collimator_grid": {"flops_per_term": 273.6, "valu_insts_per_term": 350.3, "lane_utilisation": 0.717,
-                        "valu_busy": 0.48},
+    "crystal_scatter": {"flops_per_term": 89.3, "valu_insts_per_term": 110.9, "lane_utilisation": 0.754,
+                        "valu_busy": 0.68},
+    "collimator_grid": {"flops_per_term": 275.8, "valu_insts_per_term": 359.7, "lane_utilisation": 0.703,
+                        "valu_busy": 0.56},
 }
 
 

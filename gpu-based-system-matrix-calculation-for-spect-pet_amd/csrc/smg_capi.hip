@@ -97,6 +97,7 @@ struct smg_ctx {
     DevBuf<ScatterPt<double>> pts_hole;  // layer-0 hole centres (collimator scatter)
     DevBuf<PairTerm> pair;
     DevBuf<double> sc_partial;   // per-slice scatter sums
+    DevBuf<double> sc_row_cut;   // energy-window cut per output row of a scatter launch
     DevBuf<double> proj;         // projection partial sums
     DevBuf<double> kn_tab;       // KN prefix table of kn_tab_e0
     double kn_tab_e0 = -1.0;
@@ -343,6 +344,16 @@ hipError_t kn_table(smg_ctx* c, ScatterLaunch& A) {
     return hipSuccess;
 }
 
+// Energy-window cut of each output row of a scatter launch (A.row_cut), on
+// the context's stream ahead of the pair terms and scatter kernels.
+hipError_t scatter_row_cuts(smg_ctx* c, ScatterLaunch& A) {
+    const size_t rows = (size_t)(A.rows ? A.n_rows : A.geom.g.n_det);
+    hipError_t e = c->sc_row_cut.reserve(rows > 0 ? rows : 1);
+    if (e != hipSuccess) return e;
+    A.row_cut = c->sc_row_cut.ptr;
+    return launch_row_cuts(A, c->sc_row_cut.ptr, c->stream);
+}
+
 // Partial-sum buffer of a scatter launch whose scatter points are split over
 // several waves per element.  It only grows; hipFree waits for the device.
 hipError_t scatter_partials(smg_ctx* c, ScatterLaunch& A) {
@@ -450,6 +461,7 @@ smg_status scatter_launch(smg_ctx* c, bool collimator, int rotation, int variant
     A.n_rows = 0;
     A.pts_per_bar = 1;
     A.pt_bar0 = 0;
+    SMG_HIP(c, scatter_row_cuts(c, A));
     SMG_HIP(c, launch_pair_terms(A, c->stream));
     SMG_HIP(c, scatter_partials(c, A));
     SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));
@@ -590,6 +602,7 @@ smg_status smg_destroy(smg_ctx* c) {
     c->pts_hole.release();
     c->pair.release();
     c->sc_partial.release();
+    c->sc_row_cut.release();
     c->proj.release();
     c->kn_tab.release();
     c->cell_tab.release();
@@ -1151,6 +1164,7 @@ smg_status smg_scatter_collimator_grid(smg_ctx* c, int rotation, double pitch_mm
         A.n_pts = (int)std::min<int64_t>(chunk, (int64_t)pts.size() - s0);
         A.accumulate = s0 > 0;
         SMG_HIP(c, launch_first_leg(A, c->w1.ptr, c->stream));
+        SMG_HIP(c, scatter_row_cuts(c, A));
         SMG_HIP(c, launch_pair_terms(A, c->stream));
         SMG_HIP(c, scatter_partials(c, A));
         SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));
@@ -1313,6 +1327,7 @@ smg_status smg_scatter_crystal_grid(smg_ctx* c, int rotation, int gx, int gy, in
         A.n_pts = nb * npb;
         A.pt_bar0 = b0;
         A.accumulate = b0 > 0;
+        SMG_HIP(c, scatter_row_cuts(c, A));
         SMG_HIP(c, launch_pair_terms(A, c->stream));
         SMG_HIP(c, scatter_partials(c, A));
         SMG_HIP(c, launch_scatter(A, prec == SMG_PREC_FP64, c->stream));

@@ -156,6 +156,8 @@ struct ScatterLaunch {
     double* partial;                 // [slices][rows][V] per-slice sums when the scatter points
                                      // are split over several waves per element
     const double* kn_table;          // KN prefix table of e0 (FP32 path), or nullptr
+    const double* row_cut;           // [rows] energy-window cut of each output row
+                                     // (window_reject_cos, no margin; launch_row_cuts)
 };
 
 // KN prefix table of e0 ((kKnPhase + 1) x kKnSteps doubles) into tab.
@@ -165,6 +167,9 @@ hipError_t launch_kn_table(double e0, double* tab, hipStream_t stream);
 // the chunk's scatter points (w_s = dv mu_c of the point's layer, in its dy).
 hipError_t launch_first_leg(const ScatterLaunch& a, float* w1, hipStream_t stream);
 
+// The energy-window cut of each output row into cut[rows], read by the pair
+// terms and scatter kernels of the same launch (a.row_cut).
+hipError_t launch_row_cuts(const ScatterLaunch& a, double* cut, hipStream_t stream);
 hipError_t launch_pair_terms(const ScatterLaunch& a, hipStream_t stream);
 // out[i] = a[i] + b[i], i < n (the PE + scatter block).
 hipError_t launch_add_blocks(const float* a, const float* b, float* out, size_t n, hipStream_t stream);

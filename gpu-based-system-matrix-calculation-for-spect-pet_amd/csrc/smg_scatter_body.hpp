@@ -276,6 +276,12 @@ SMG_FN double window_reject_cos(const ScatterPhys& ph, double margin) {
     return c_lo - margin;
 }
 
+// window_reject_cos(ph, margin) from its value at margin 0: the special
+// values -2 (nothing rejected) and 2 (everything rejected) carry no margin.
+SMG_FN double window_cut_margin(double cut, double margin) {
+    return (cut < -1.5 || cut > 1.5) ? cut : cut - margin;
+}
+
 // Klein-Nishina in the reference's factorised form (scatter.cu:962-977), from
 // cos(theta).
 template <class T>

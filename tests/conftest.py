@@ -118,6 +118,8 @@ def load_lane_lib():
     L.lane_mu_ratio_f.argtypes = [C.c_double, C.c_int] + [dp] * 6 + [C.c_int, dp, dp]
     L.lane_window_reject_cos.restype = C.c_double
     L.lane_window_reject_cos.argtypes = [C.c_double] * 4
+    L.lane_window_cut_margin.restype = C.c_double
+    L.lane_window_cut_margin.argtypes = [C.c_double] * 2
     L.lane_set_hole_shapes.restype = None
     L.lane_set_hole_shapes.argtypes = [fp, C.c_int]
     L.lane_bitmap.argtypes = [fp, fp, fp, fp, C.c_int, up, C.c_long]

@@ -460,7 +460,8 @@ double scatter_element(const Ctx& c, const float* pe, const uint32_t* bitmap, in
     P.mupe = bd.mupe;
     P.muc = bd.muc;
     P.rcell = std::sqrt(bd.w * bd.w / sx / sx + bd.h * bd.h / sz / sz + bd.t * bd.t / sy / sy) / 2.0;
-    P.c_rej = window_reject_cos(P, std::is_same<T, double>::value ? 1e-9 : 1e-5);
+    // the row cut at margin 0 (row_cut_kernel), then the path's margin
+    P.c_rej = window_cut_margin(window_reject_cos(P, 0.0), std::is_same<T, double>::value ? 1e-9 : 1e-5);
     // the KN prefix table the GPU's FP32 path reads (built with the same code)
     std::vector<double> knt((size_t)(kKnPhase + 1) * kKnSteps);
     for (int j = 0; j <= kKnPhase; ++j) kn_prefix_row(P.e0 / 511.0, j, knt.data() + (size_t)j * kKnSteps);
@@ -597,6 +598,9 @@ double lane_window_reject_cos(double e0, double lo, double res, double margin) {
     P.res = res;
     return window_reject_cos(P, margin);
 }
+
+// The scatter kernel's threshold from the row cut at margin 0.
+double lane_window_cut_margin(double cut, double margin) { return window_cut_margin(cut, margin); }
 
 // Elliptical holes (x_rho, z_rho, theta per hole) for the following calls;
 // n = 0 restores circles.

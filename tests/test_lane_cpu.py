@@ -226,6 +226,25 @@ def test_window_reject_cos(lane_lib):
     assert lane_lib.lane_window_reject_cos(140.5, 1.2 * 140.5, 0.1, 1e-5) == 2.0
 
 
+def test_window_cut_once_per_row(lane_lib):
+    """The scatter kernels read each row's cut at margin 0 (one thread per row)
+    and apply their path's margin: the same threshold, bit for bit, as the
+    cut formed with the margin, the special values -2 (nothing rejected) and
+    2 (everything rejected) included."""
+    rng = np.random.default_rng(10)
+    cases = [(140.5, 0.0, 0.1), (140.5, 1.2 * 140.5, 0.1), (140.5, -5.0, 0.1), (0.0, 100.0, 0.1)]
+    for _ in range(500):
+        e0 = rng.uniform(20.0, 600.0)
+        cases.append((e0, e0 * rng.uniform(0.2, 1.3), rng.uniform(0.01, 0.5)))
+    specials = 0
+    for e0, lo, res in cases:
+        cut = lane_lib.lane_window_reject_cos(e0, lo, res, 0.0)
+        specials += abs(cut) == 2.0
+        for m in (1e-5, 1e-9):
+            assert lane_lib.lane_window_cut_margin(cut, m) == lane_lib.lane_window_reject_cos(e0, lo, res, m)
+    assert specials >= 4
+
+
 def test_lane_overflow_geometry(lane_lib, oracle_mod, params_mod):
     """The overflow geometry's tiles need more candidates than the kernels'
     LDS lists hold (448 bars, 256 holes), so the kernels take their spill
