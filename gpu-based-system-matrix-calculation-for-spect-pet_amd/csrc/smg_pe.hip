@@ -47,7 +47,8 @@ struct WaveListsT {
     uint16_t hole_idx[CapH];
     uint16_t bar_sel[CapB];
     uint16_t hole_sel[CapH];
-    float faces[kFastBars][6];   // narrowed bar j: faces - sub-cell point (fast_faces)
+    float faces[kFastBars][7];   // narrowed bar j: faces - sub-cell point (fast_faces),
+                                 // then the bar's mu_total (a float value: exact)
     uint8_t fast_ok[kFastBars];  // narrowed bar j qualifies for fast_bar_chord
 };
 using WaveLists = WaveListsT<kCapBars, kCapHoles>;
@@ -65,7 +66,7 @@ struct SpillLists {
     uint64_t* bar_mask;
     uint64_t* hole_mask;
     double* cand_acc;
-    float (*faces)[6];
+    float (*faces)[7];
     uint16_t* bar_idx;
     uint16_t* hole_idx;
     uint16_t* bar_sel;
@@ -84,8 +85,8 @@ __device__ __forceinline__ SpillLists spill_lists(const PeSpill& S, int slot) {
     p += sizeof(uint64_t) * S.cap_holes;
     L.cand_acc = (double*)p;
     p += sizeof(double) * S.cap_bars;
-    L.faces = (float(*)[6])p;
-    p += sizeof(float) * 6 * kFastBars;
+    L.faces = (float(*)[7])p;
+    p += sizeof(float) * 7 * kFastBars;
     L.bar_idx = (uint16_t*)p;
     p += sizeof(uint16_t) * S.cap_bars;
     L.hole_idx = (uint16_t*)p;
@@ -116,6 +117,9 @@ struct ListCands {
     int nb, nh, lane;
     __device__ bool fast(int j) const { return j < kFastBars && L->fast_ok[j]; }
     __device__ const float* faces(int j) const { return L->faces[j]; }
+    // mu_total of narrowed bar j, kept beside its faces for the first kFastBars
+    __device__ bool has_mu(int j) const { return j < kFastBars; }
+    __device__ float mu(int j) const { return L->faces[j][6]; }
     __device__ int nbars() const { return nb; }
     // the index is wave-uniform: readfirstlane lets the record loads go
     // through the scalar unit
@@ -341,6 +345,7 @@ __device__ void narrow_candidates(const PeLaunch& A, const double qd[3], const W
                     L.fast_ok[pos] = fast_faces(G, L.bar_idx[j], qd, W.tc, W.rho, f) ? 1 : 0;
 #pragma unroll
                     for (int i = 0; i < 6; ++i) L.faces[pos][i] = f[i];
+                    L.faces[pos][6] = G.bars_f[L.bar_idx[j]].mut;
                 }
             } else {
                 L.hole_sel[nsh + lanes_below(bh, lane)] = (uint16_t)(j - nb);

@@ -633,19 +633,30 @@ SMG_FN T pe_cell_ray(const PeGeom& G, int d, const double p[3], const double ps[
             if (keep && j < kKeepChords) keep[j * keep_stride] = 0.0f;
             continue;
         }
-        int k = C.bar(j);
         T l;
+        double mu_k;
         if constexpr (Cands::kFastFaces && std::is_same<T, float>::value) {
-            if (sw.ns == 0 && C.fast(j))
-                l = fast_bar_chord(sw, C.faces(j), near);
-            else
+            // the first narrowed bars carry their mu_total in the lists: a
+            // fast chord then reads no bar record
+            if (C.has_mu(j)) {
+                mu_k = double(C.mu(j));
+                if (sw.ns == 0 && C.fast(j))
+                    l = fast_bar_chord(sw, C.faces(j), near);
+                else
+                    l = other_bar_chord<T>(G, C.bar(j), p, D, dd, inv_dd, cc.q, sw, tol_d, near);
+            } else {
+                const int k = C.bar(j);
+                mu_k = G.bars_d[k].mut;
                 l = other_bar_chord<T>(G, k, p, D, dd, inv_dd, cc.q, sw, tol_d, near);
+            }
         } else {
+            const int k = C.bar(j);
+            mu_k = G.bars_d[k].mut;
             l = other_bar_chord<T>(G, k, p, D, dd, inv_dd, cc.q, sw, tol_d, near);
         }
         if (keep && j < kKeepChords) keep[j * keep_stride] = float(l);
-        att += double(l) * G.bars_d[k].mut;
-        if (tan_bar) *tan_att += double(l) * tan_bar[k];
+        att += double(l) * mu_k;
+        if (tan_bar) *tan_att += double(l) * tan_bar[C.bar(j)];
     }
 
     // own bar from its surface to the sub-cell (PESysMatGen.cu:1329-1346), and the

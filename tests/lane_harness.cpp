@@ -114,15 +114,17 @@ struct MaskCands {
     bool hole_on(int j) const { return (hm[j] >> lane) & 1ull; }
 };
 
-// MaskCands with the face records of the first 32 narrowed bars, as the GPU
-// kernels keep them in LDS (fast_faces), so pe_cell_ray<float> takes
-// fast_bar_chord for the bars that qualify.
+// MaskCands with the face records and mu_total of the first 32 narrowed bars,
+// as the GPU kernels keep them in LDS (fast_faces), so pe_cell_ray<float>
+// takes fast_bar_chord for the bars that qualify.
 struct FastMaskCands : MaskCands {
     static constexpr bool kFastFaces = true;
     std::vector<char> ok;
-    std::vector<std::array<float, 6>> f;
+    std::vector<std::array<float, 7>> f;
     bool fast(int j) const { return j < 32 && j < (int)ok.size() && ok[j]; }
     const float* faces(int j) const { return f[j].data(); }
+    bool has_mu(int j) const { return j < (int)f.size(); }
+    float mu(int j) const { return f[j][6]; }
 };
 
 }  // namespace
@@ -305,8 +307,9 @@ int lane_narrow_check(const float* coll, const float* det, const float* img, con
                     FastMaskCands fm;
                     static_cast<MaskCands&>(fm) = nar;
                     for (int j = 0; j < (int)nar.b.size() && j < 32; ++j) {
-                        std::array<float, 6> fr;
+                        std::array<float, 7> fr;
                         fm.ok.push_back(fast_faces(G, nar.b[j], cc.q, tc, rho, fr.data()) ? 1 : 0);
+                        fr[6] = G.bars_f[nar.b[j]].mut;
                         fm.f.push_back(fr);
                         n_fast += fm.ok.back();
                         n_fastable += 1;
