@@ -47,11 +47,11 @@ TRAFFIC_BYTES = 1.157e9
 # kernel's registers and LDS allow)
 EXEC_SOURCE = "profiles/pe_forward_instruction_mix_fp32_fp64_64x64x32_2048bars.md"
 EXEC_FLOPS_PER_RAY = 250.9 + 187.9
-VALU_INSTS_PER_RAY = 836.0
+VALU_INSTS_PER_RAY = 832.5
 TRANS_INSTS_PER_RAY = 23.0 + 4.8
-VALU_BUSY = 0.824
+VALU_BUSY = 0.841
 LANE_UTILISATION = 0.725
-WAVES_PER_SIMD = 3.72
+WAVES_PER_SIMD = 3.71
 MAX_WAVES_PER_SIMD = 4
 # unpacked FP32 (= FP64) vector FMA peak, the transcendental peak (a quarter of
 # the unpacked FMA rate in operations), and the VALU issue reference rate (256 CUs x
