@@ -40,7 +40,7 @@ struct PeSpill {
 // hole, and the face records of the first narrowed bars, in 256-byte units.
 inline int64_t pe_spill_slot_bytes(int n_det, int n_holes) {
     const int64_t b = n_det > 0 ? n_det : 1, h = n_holes > 0 ? n_holes : 1;
-    const int64_t bytes = b * (8 + 8 + 2 + 2) + h * (8 + 2 + 2) + 32 * (7 * 4 + 1);
+    const int64_t bytes = b * (8 + 8 + 2 + 2) + h * (8 + 2 + 2) + 32 * (7 * 4 + 1) + (b + 63) / 64 * 8;
     return (bytes + 255) / 256 * 256;
 }
 

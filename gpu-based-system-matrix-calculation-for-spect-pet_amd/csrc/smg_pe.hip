@@ -43,6 +43,7 @@ struct WaveListsT {
     static constexpr int cap_bars = CapB, cap_holes = CapH;
     uint64_t bar_mask[CapB];
     uint64_t hole_mask[CapH];
+    uint64_t tile_far[(CapB + 63) / 64];  // bit i: candidate bar i passes fast_tile_far
     uint16_t bar_idx[CapB];
     uint16_t hole_idx[CapH];
     uint16_t bar_sel[CapB];
@@ -65,6 +66,7 @@ struct SpillLists {
     int cap_bars, cap_holes;
     uint64_t* bar_mask;
     uint64_t* hole_mask;
+    uint64_t* tile_far;
     double* cand_acc;
     float (*faces)[7];
     uint16_t* bar_idx;
@@ -83,6 +85,8 @@ __device__ __forceinline__ SpillLists spill_lists(const PeSpill& S, int slot) {
     p += sizeof(uint64_t) * S.cap_bars;
     L.hole_mask = (uint64_t*)p;
     p += sizeof(uint64_t) * S.cap_holes;
+    L.tile_far = (uint64_t*)p;
+    p += sizeof(uint64_t) * ((S.cap_bars + 63) / 64);
     L.cand_acc = (double*)p;
     p += sizeof(double) * S.cap_bars;
     L.faces = (float(*)[7])p;
@@ -198,6 +202,7 @@ __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveS
     const Bar<float>& bdf = G.bars_f[d];
     nb = 0;
     nh = 0;
+    for (int w = lane; w < (L.cap_bars + 63) / 64; w += 64) L.tile_far[w] = 0ull;
     // groups of kGroup bars first (one group box per lane), then the members
     // of each group that can be hit
     const int n_bg = (g.n_det + kGroup - 1) / kGroup;
@@ -213,13 +218,15 @@ __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveS
             const int base = (g0 + __ffsll((unsigned long long)gbal) - 1) * kGroup;
             gbal &= gbal - 1;
             int k = base + lane;
-            bool pass = false;
+            bool pass = false, far = false;
             if (k < g.n_det && k != d) {
                 const Bar<float>& b = G.bars_f[k];
                 pass = bar_tile_may_hit(bdf, W.tcf, W.rhof, b.cx, b.cy, b.cz, b.cb, b.sb, 0.5f * b.w,
                                         0.5f * b.t, 0.5f * b.h);
+                if (pass) far = fast_tile_far(G, k, W.tc, W.rho);
             }
             uint64_t bal = __ballot(pass);
+            const uint64_t far_bal = __ballot(far);
             while (bal) {
                 int bit = __ffsll((unsigned long long)bal) - 1;
                 bal &= bal - 1;
@@ -230,6 +237,7 @@ __device__ bool build_candidates(const PeLaunch& A, int d, int lane, const WaveS
                     if (nb < L.cap_bars && lane == 0) {
                         L.bar_idx[nb] = (uint16_t)kk;
                         L.bar_mask[nb] = m;
+                        if ((far_bal >> bit) & 1ull) L.tile_far[nb >> 6] |= 1ull << (nb & 63);
                     }
                     ++nb;
                 }
@@ -342,7 +350,9 @@ __device__ void narrow_candidates(const PeLaunch& A, const double qd[3], const W
                 L.bar_sel[pos] = (uint16_t)j;
                 if (pos < kFastBars) {
                     float f[6];
-                    L.fast_ok[pos] = fast_faces(G, L.bar_idx[j], qd, W.tc, W.rho, f) ? 1 : 0;
+                    // fast_faces: the tile part once per tile (build_candidates)
+                    const bool near_q_ok = fast_faces_q(G, L.bar_idx[j], qd, f);
+                    L.fast_ok[pos] = ((L.tile_far[j >> 6] >> (j & 63)) & 1ull) && near_q_ok ? 1 : 0;
 #pragma unroll
                     for (int i = 0; i < 6; ++i) L.faces[pos][i] = f[i];
                     L.faces[pos][6] = G.bars_f[L.bar_idx[j]].mut;

@@ -458,23 +458,43 @@ SMG_FN T other_bar_chord(const PeGeom& G, int k, const double p[3], const double
 // Faces of unturned bar k relative to q: out = (lo - q, hi - q) per axis;
 // true when the bar qualifies (unturned, more than 0.01 mm from q and from
 // the ball (tc, rho)).
-SMG_FN bool fast_faces(const PeGeom& G, int k, const double q[3], const double tc[3], double rho,
-                       float out[6]) {
+// The part of the fast_faces test that depends on the tile only: bar k is
+// unturned and its box lies more than rho + 0.01 from the tile ball's centre
+// (distances compared squared).
+SMG_FN bool fast_tile_far(const PeGeom& G, int k, const double tc[3], double rho) {
     const Bar<double>& b = G.bars_d[k];
     const double c[3] = {b.cx, b.cy, b.cz}, h[3] = {0.5 * b.w, 0.5 * b.t, 0.5 * b.h};
-    double dq = 0.0, dt = 0.0;
+    double dt = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double lo = c[i] - h[i], hi = c[i] + h[i];
+        const double et = tmax2(tmax2(lo - tc[i], tc[i] - hi), 0.0);
+        dt += et * et;
+    }
+    return b.sb == 0.0 && b.cb == 1.0 && dt > (rho + 0.01) * (rho + 0.01);
+}
+
+// The part that depends on the sub-cell point q: the faces relative to q, and
+// true when the box lies more than 0.01 from q.
+SMG_FN bool fast_faces_q(const PeGeom& G, int k, const double q[3], float out[6]) {
+    const Bar<double>& b = G.bars_d[k];
+    const double c[3] = {b.cx, b.cy, b.cz}, h[3] = {0.5 * b.w, 0.5 * b.t, 0.5 * b.h};
+    double dq = 0.0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const double lo = c[i] - h[i], hi = c[i] + h[i];
         out[i] = float(lo - q[i]);
         out[3 + i] = float(hi - q[i]);
         const double eq = tmax2(tmax2(lo - q[i], q[i] - hi), 0.0);
-        const double et = tmax2(tmax2(lo - tc[i], tc[i] - hi), 0.0);
         dq += eq * eq;
-        dt += et * et;
     }
-    // distances compared squared: |q - box| > 0.01, |tc - box| > rho + 0.01
-    return b.sb == 0.0 && b.cb == 1.0 && dq > 1e-4 && dt > (rho + 0.01) * (rho + 0.01);
+    return dq > 1e-4;
+}
+
+SMG_FN bool fast_faces(const PeGeom& G, int k, const double q[3], const double tc[3], double rho,
+                       float out[6]) {
+    const bool near_q_ok = fast_faces_q(G, k, q, out);
+    return fast_tile_far(G, k, tc, rho) && near_q_ok;
 }
 
 // Chord of the ray through a qualifying bar from its faces relative to q
