@@ -39,25 +39,25 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 # measured HBM traffic of one FP32 launch of the full-size PE step (rocprofv3 PMC
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a plain 1 GiB copy)
-TRAFFIC_BYTES = 1.157e9
+TRAFFIC_BYTES = 1.154e9
 # executed work of the same launch (rocprofv3 PMC, FP32 path): FLOPs per sub-cell ray
 # the lanes execute (64 x SQ_INSTS_VALU_FLOPS_FP32 + _FP64 per ray), VALU wave
 # instructions per ray x 64, transcendental instructions per ray x 64 (FP32 + FP64),
 # VALUBusy, lane utilisation and mean resident waves per SIMD (of the 4 the
 # kernel's registers and LDS allow)
 EXEC_SOURCE = "profiles/pe_forward_instruction_mix_fp32_fp64_64x64x32_2048bars.md"
-EXEC_FLOPS_PER_RAY = 250.9 + 187.9
-VALU_INSTS_PER_RAY = 832.5
+EXEC_FLOPS_PER_RAY = 250.9 + 182.1
+VALU_INSTS_PER_RAY = 819.6
 TRANS_INSTS_PER_RAY = 23.0 + 4.8
 VALU_BUSY = 0.841
 # the same instructions by issue cost (bin/valu_issue_probe on this GPU: 4 cycles
 # per wave instruction for FP32/FP64 arithmetic, integer and conversion, ~2 for
 # compares, selects, min/max and moves, ~8 for a transcendental)
-VALU_4CYCLE_INSTS_PER_RAY = 445.8
-VALU_2CYCLE_INSTS_PER_RAY = 386.7
+VALU_4CYCLE_INSTS_PER_RAY = 441.7
+VALU_2CYCLE_INSTS_PER_RAY = 377.9
 VALU_CYCLE_SOURCE = "profiles/valu_issue_rates_gfx950.md"
-LANE_UTILISATION = 0.725
-WAVES_PER_SIMD = 3.71
+LANE_UTILISATION = 0.738
+WAVES_PER_SIMD = 3.73
 MAX_WAVES_PER_SIMD = 4
 # unpacked FP32 (= FP64) vector FMA peak, the transcendental peak (a quarter of
 # the unpacked FMA rate in operations), and the VALU issue reference rate (256 CUs x
